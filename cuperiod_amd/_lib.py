@@ -1,0 +1,227 @@
+"""ctypes binding of ``libcuperiod_bls.so`` (the C ABI of ``include/cuperiod_bls.h``).
+
+The library is built in-tree for gfx950 (``python -c "import __graft_entry__ as g;
+g.build()"`` or ``make -C cuperiod_amd/csrc``). There is no fallback: if the library is
+missing, or no HIP device is visible, every entry point raises
+:class:`~cuperiod_amd.core.errors.BackendUnavailableError`.
+
+Device buffers are torch tensors on the HIP device (torch is used for allocation and
+copies only); their raw pointers and the current torch stream go through the ABI.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+from cuperiod_amd.core.errors import (
+    BackendUnavailableError,
+    DeviceError,
+)
+
+LIB_NAME = "libcuperiod_bls.so"
+LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
+
+CPB_OK = 0
+CPB_E_INVALID = 1
+CPB_E_DEVICE = 2
+CPB_E_RESOURCE = 3
+CPB_E_NODEVICE = 4
+N_OUTPUTS = 7
+
+#: Every symbol the header declares, in declaration order.
+EXPORTED_SYMBOLS = (
+    "cpb_version",
+    "cpb_device_count",
+    "cpb_create",
+    "cpb_destroy",
+    "cpb_last_error",
+    "cpb_status_string",
+    "cpb_bls_search",
+    "cpb_bls_search_batch",
+    "cpb_linspace_grid",
+    "cpb_synchronize",
+)
+
+
+class Segment(ctypes.Structure):
+    """``cpb_segment``: one duration-homogeneous period block."""
+
+    _fields_ = [
+        ("bin_duration", ctypes.c_double),
+        ("period_offset", ctypes.c_int64),
+        ("n_periods", ctypes.c_int64),
+        ("out_offset", ctypes.c_int64),
+        ("width", ctypes.c_int32),
+        ("dur_offset", ctypes.c_int32),
+        ("n_dur", ctypes.c_int32),
+        ("curve", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("lin_start", ctypes.c_double),
+        ("lin_stop", ctypes.c_double),
+        ("lin_num", ctypes.c_int64),
+        ("lin_k0", ctypes.c_int64),
+    ]
+
+
+class Curve(ctypes.Structure):
+    """``cpb_curve``: one light curve of a batched search."""
+
+    _fields_ = [
+        ("point_offset", ctypes.c_int64),
+        ("n_points", ctypes.c_int64),
+        ("t_min", ctypes.c_double),
+        ("ivar_floor", ctypes.c_double),
+    ]
+
+
+_LOCK = threading.Lock()
+_LIB: ctypes.CDLL | None = None
+_LOAD_ERROR: str | None = None
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    vp = ctypes.c_void_p
+    i32, i64, f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    lib.cpb_version.restype = ctypes.c_int
+    lib.cpb_version.argtypes = []
+    lib.cpb_device_count.restype = ctypes.c_int
+    lib.cpb_device_count.argtypes = []
+    lib.cpb_create.restype = ctypes.c_int
+    lib.cpb_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.cpb_destroy.restype = None
+    lib.cpb_destroy.argtypes = [vp]
+    lib.cpb_last_error.restype = ctypes.c_char_p
+    lib.cpb_last_error.argtypes = [vp]
+    lib.cpb_status_string.restype = ctypes.c_char_p
+    lib.cpb_status_string.argtypes = [ctypes.c_int]
+    lib.cpb_bls_search.restype = ctypes.c_int
+    lib.cpb_bls_search.argtypes = [
+        vp, vp, vp, vp, vp, i64, f64, f64, vp, vp,
+        ctypes.POINTER(Segment), i32, i32, i32, vp, i64,
+    ]
+    lib.cpb_bls_search_batch.restype = ctypes.c_int
+    lib.cpb_bls_search_batch.argtypes = [
+        vp, vp, vp, vp, vp, i64, ctypes.POINTER(Curve), i32, vp, vp,
+        ctypes.POINTER(Segment), i32, i32, i32, vp, i64,
+    ]
+    lib.cpb_linspace_grid.restype = ctypes.c_int
+    lib.cpb_linspace_grid.argtypes = [f64, f64, i64, ctypes.POINTER(f64)]
+    lib.cpb_synchronize.restype = ctypes.c_int
+    lib.cpb_synchronize.argtypes = [vp, vp]
+
+
+def load_library() -> ctypes.CDLL:
+    """The loaded library; :class:`BackendUnavailableError` if it cannot be loaded."""
+    global _LIB, _LOAD_ERROR
+    with _LOCK:
+        if _LIB is not None:
+            return _LIB
+        if not LIB_PATH.exists():
+            _LOAD_ERROR = (
+                f"BLS: the HIP library {LIB_PATH} is not built; run "
+                "`make -C cuperiod_amd/csrc` (hipcc, gfx950)"
+            )
+            raise BackendUnavailableError(_LOAD_ERROR)
+        try:
+            lib = ctypes.CDLL(str(LIB_PATH))
+        except OSError as exc:
+            _LOAD_ERROR = f"BLS: cannot load {LIB_PATH}: {exc}"
+            raise BackendUnavailableError(_LOAD_ERROR) from exc
+        _declare(lib)
+        _LIB = lib
+        return lib
+
+
+def library_available() -> bool:
+    """Whether the HIP library loads (no device needed)."""
+    try:
+        load_library()
+    except BackendUnavailableError:
+        return False
+    return True
+
+
+def device_count() -> int:
+    """HIP devices the library sees (0 without the library or a device)."""
+    try:
+        return int(load_library().cpb_device_count())
+    except BackendUnavailableError:
+        return 0
+
+
+def status_error(status: int, message: str) -> Exception:
+    """The Python exception for a C-ABI status code."""
+    if status == CPB_E_INVALID:
+        return ValueError(message)
+    if status in (CPB_E_RESOURCE, CPB_E_NODEVICE):
+        return BackendUnavailableError(message)
+    return DeviceError(message)
+
+
+class Context:
+    """A ``cpb_context`` on one device; calls are serialised by a lock, so one
+    context may be shared by threads (each call runs alone)."""
+
+    def __init__(self, device: int = 0) -> None:
+        self.lib = load_library()
+        handle = ctypes.c_void_p()
+        status = self.lib.cpb_create(int(device), ctypes.byref(handle))
+        if status != CPB_OK:
+            reason = self.lib.cpb_status_string(status).decode()
+            raise status_error(
+                status if status != CPB_E_INVALID else CPB_E_NODEVICE,
+                f"BLS: cannot open HIP device {device} ({reason})",
+            )
+        self.handle = handle
+        self.device = int(device)
+        self.lock = threading.RLock()
+
+    def check(self, status: int, what: str) -> None:
+        if status == CPB_OK:
+            return
+        detail = self.lib.cpb_last_error(self.handle).decode(errors="replace")
+        raise status_error(status, f"BLS {what}: {detail}")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.cpb_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_CONTEXTS: dict[int, Context] = {}
+
+
+def context(device: int = 0) -> Context:
+    """The process-wide context of ``device`` (created on first use)."""
+    with _LOCK:
+        ctx = _CONTEXTS.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _LOCK:
+            _CONTEXTS.setdefault(device, ctx)
+            ctx = _CONTEXTS[device]
+    return ctx
+
+
+__all__ = [
+    "EXPORTED_SYMBOLS",
+    "LIB_PATH",
+    "N_OUTPUTS",
+    "Context",
+    "Curve",
+    "Segment",
+    "context",
+    "device_count",
+    "library_available",
+    "load_library",
+    "status_error",
+]

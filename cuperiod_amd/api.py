@@ -1,0 +1,115 @@
+"""Single light-curve API: :func:`periodogram` (src/cuperiod/api.py:47-307).
+
+Accepts a :class:`LightCurve` or a ``(t, y[, dy])`` tuple (read as magnitudes unless
+``domain`` says otherwise, as in cuPeriod), one or several method names, and optional
+settings / grid / backend; returns a :class:`Periodogram` (one method) or a
+:class:`MultiResult`.
+"""
+
+from __future__ import annotations
+
+import warnings
+from collections.abc import Sequence
+from typing import Any
+
+import numpy as np
+
+from cuperiod_amd.core.columns import Domain, coerce_domain
+from cuperiod_amd.core.grid import GridSpec
+from cuperiod_amd.core.lightcurve import LightCurve, raise_if_unusable
+from cuperiod_amd.core.result import MultiResult, Periodogram, peak_defaults_from_settings
+from cuperiod_amd.methods.base import PeriodogramMethod, get_method, select_settings
+
+_NORMALISED_FLUX_RANGE = (0.5, 1.5)
+
+
+def _looks_like_arrays(obj: Any) -> bool:
+    if not isinstance(obj, (tuple, list)) or not (2 <= len(obj) <= 3):
+        return False
+    first = obj[0]
+    return hasattr(first, "__len__") and not isinstance(first, (str, bytes))
+
+
+def to_input(data: Any, *, domain: Domain | str | None = None) -> LightCurve:
+    """A :class:`LightCurve` from a light curve or a ``(t, y[, dy])`` tuple."""
+    dom = coerce_domain(domain)
+    if isinstance(data, LightCurve):
+        return data if dom is None else data.in_domain(dom)
+    if _looks_like_arrays(data):
+        t, v = data[0], data[1]
+        e = data[2] if len(data) == 3 else None
+        return LightCurve.from_arrays(t, v, e, domain=dom or Domain.MAGNITUDE)
+    raise TypeError(
+        f"unsupported light-curve input {type(data).__name__}; pass a LightCurve or a "
+        "(t, y[, dy]) tuple"
+    )
+
+
+def _warn_flux_tuple(data: Any, lc: LightCurve, methods: Sequence[PeriodogramMethod]) -> None:
+    """Warn when a box search is about to read a normalised flux as magnitudes."""
+    if not _looks_like_arrays(data):
+        return
+    flux_methods = [m.name for m in methods if m.natural_domain is Domain.FLUX]
+    if not flux_methods or lc.value.size == 0:
+        return
+    with np.errstate(all="ignore"), warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        median = float(np.nanmedian(lc.value))
+    lo, hi = _NORMALISED_FLUX_RANGE
+    if lo < median < hi:
+        warnings.warn(
+            f"{', '.join(flux_methods)}: the (t, y, dy) tuple is read as magnitudes "
+            f"(median {median:.3g}), but its values look like a normalised flux; pass "
+            "domain='flux' if they are fluxes, or a LightCurve with an explicit domain "
+            "to silence this.",
+            UserWarning,
+            stacklevel=3,
+        )
+
+
+def _run_one(
+    method: PeriodogramMethod,
+    lc: LightCurve,
+    settings: Any,
+    backend: str | None,
+    grid: GridSpec | None,
+) -> Periodogram:
+    resolved = method.resolve_backend_for(backend, settings)
+    single = lc.in_domain(method.natural_domain) if method.natural_domain else lc
+    raise_if_unusable(single, method.name)
+    g = grid or method.default_grid(single, settings)
+    pg = method.power(g, single, settings, resolved, engine=None)
+    return pg.with_peak_defaults(peak_defaults_from_settings(settings))
+
+
+def periodogram(
+    data: Any,
+    method: str | Sequence[str] = "BLS",
+    *,
+    backend: str | None = "auto",
+    grid: GridSpec | None = None,
+    settings: Any = None,
+    domain: Domain | str | None = None,
+) -> Periodogram | MultiResult:
+    """Periodogram(s) of one light curve. ``backend="auto"`` defers to the
+    settings' ``backend`` and resolves to ``"hip"`` on an AMD GPU."""
+    lc = to_input(data, domain=domain)
+    names = [method] if isinstance(method, str) else list(method)
+    methods = [get_method(name) for name in names]
+    chosen = select_settings(methods, settings)
+    if domain is None:
+        _warn_flux_tuple(data, lc, methods)
+    results: dict[str, Periodogram] = {}
+    for m in methods:
+        results[m.name] = _run_one(m, lc, chosen[m.name], backend, grid)
+    if isinstance(method, str):
+        return results[methods[0].name]
+    return MultiResult(results)
+
+
+def best_periods(result: Periodogram | MultiResult, n: int | None = None, **kwargs: Any):
+    """``result.best_periods(n)``."""
+    return result.best_periods(n, **kwargs)
+
+
+__all__ = ["best_periods", "periodogram", "to_input"]

@@ -1,0 +1,181 @@
+"""Peak candidates and N-best selection on a score to maximise.
+
+The selection rules of cuPeriod's peak picker (src/cuperiod/core/peaks.py:57-231):
+interior local maxima (strictly above the left neighbour, at least the right one, so a
+flat top reports its left edge), greedy selection by descending score with a minimum
+frequency separation, and the alias-diverse variant that also skips harmonics,
+sub-harmonics and the sidereal/solar/yearly alias lattice of every chosen peak. The
+peak-preserving downsample keeps each bin's first maximum (``:234-288``).
+"""
+
+from __future__ import annotations
+
+from typing import Literal
+
+import numpy as np
+
+#: Sidereal-day frequency (cycles/day): the dominant ground-based alias spacing.
+SIDEREAL_FREQ_CPD = 1.0027379
+#: Solar-day frequency (cycles/day).
+SOLAR_FREQ_CPD = 1.0
+#: Yearly sidelobe spacing (cycles/day).
+YEARLY_FREQ_CPD = 1.0 / 365.25
+
+_DOWNSAMPLE_VECTOR_MAX_RATIO = 64
+
+
+def _sum_is_finite(values: np.ndarray) -> bool:
+    with np.errstate(invalid="ignore", over="ignore"):
+        return bool(np.isfinite(values.sum()))
+
+
+def _finite_or_minus_inf(score: np.ndarray) -> np.ndarray:
+    if _sum_is_finite(score):
+        return score
+    return np.where(np.isfinite(score), score, -np.inf)
+
+
+def _interior_maxima(safe: np.ndarray) -> np.ndarray:
+    if safe.size < 3:
+        return np.empty(0, dtype=np.int64)
+    interior = (safe[1:-1] > safe[:-2]) & (safe[1:-1] >= safe[2:])
+    return np.flatnonzero(interior).astype(np.int64) + 1
+
+
+def local_maxima(score: np.ndarray) -> np.ndarray:
+    """Indices of interior local maxima (non-finite samples compare as ``-inf``)."""
+    if score.size < 3:
+        return np.empty(0, dtype=np.int64)
+    return _interior_maxima(_finite_or_minus_inf(score))
+
+
+def _descending(score: np.ndarray, candidates: np.ndarray) -> np.ndarray:
+    """``candidates`` by descending score; equal scores in reverse index order, as
+    ``np.argsort(score)[::-1]`` orders them."""
+    return candidates[np.argsort(score[candidates])[::-1]]
+
+
+def select_top_peaks(
+    frequency: np.ndarray,
+    score: np.ndarray,
+    candidates: np.ndarray,
+    n_peaks: int,
+    min_separation: float,
+) -> np.ndarray:
+    """Greedy highest-score peaks at least ``min_separation`` apart in frequency."""
+    if n_peaks <= 0:
+        return np.empty(0, dtype=np.int64)
+    chosen: list[int] = []
+    for idx in _descending(score, candidates):
+        if all(abs(frequency[idx] - frequency[j]) >= min_separation for j in chosen):
+            chosen.append(int(idx))
+            if len(chosen) >= n_peaks:
+                break
+    return np.asarray(chosen, dtype=np.int64)
+
+
+def is_alias_related(
+    f_new: float,
+    f_chosen: float,
+    tol: float,
+    harmonic_max: int,
+    *,
+    min_separation: float | None = None,
+) -> bool:
+    """Whether ``f_new`` duplicates ``f_chosen`` (closer than ``min_separation``), is
+    one of its harmonics / sub-harmonics up to ``harmonic_max`` (tolerance ``m*tol``
+    around the m-th harmonic, ``tol`` around a sub-harmonic), or sits on its
+    sidereal/solar (k = 1, 2) or yearly alias."""
+    separation = tol if min_separation is None else min_separation
+    if abs(f_new - f_chosen) < separation:
+        return True
+    for m in range(2, harmonic_max + 1):
+        if abs(f_new - m * f_chosen) < m * tol:
+            return True
+        if abs(f_new - f_chosen / m) < tol:
+            return True
+    offsets = [k * day for day in (SIDEREAL_FREQ_CPD, SOLAR_FREQ_CPD) for k in (1, 2)]
+    offsets.append(YEARLY_FREQ_CPD)
+    for off in offsets:
+        if abs(f_new - abs(f_chosen - off)) < tol or abs(f_new - (f_chosen + off)) < tol:
+            return True
+    return False
+
+
+def select_diverse_peaks(
+    frequency: np.ndarray,
+    score: np.ndarray,
+    candidates: np.ndarray,
+    n_peaks: int,
+    tol: float,
+    harmonic_max: int,
+    *,
+    min_separation: float | None = None,
+) -> np.ndarray:
+    """Greedy highest-score peaks skipping any candidate alias-related (either way
+    round) to an already chosen one."""
+    if n_peaks <= 0:
+        return np.empty(0, dtype=np.int64)
+    chosen: list[int] = []
+    for idx in _descending(score, candidates):
+        f_new = float(frequency[idx])
+        if all(
+            not is_alias_related(
+                f_new, float(frequency[j]), tol, harmonic_max,
+                min_separation=min_separation,
+            )
+            and not is_alias_related(
+                float(frequency[j]), f_new, tol, harmonic_max,
+                min_separation=min_separation,
+            )
+            for j in chosen
+        ):
+            chosen.append(int(idx))
+            if len(chosen) >= n_peaks:
+                break
+    return np.asarray(chosen, dtype=np.int64)
+
+
+def peak_preserving_downsample(
+    frequency: np.ndarray,
+    power: np.ndarray,
+    n_points: int,
+    *,
+    sense: Literal["max", "min"] = "max",
+) -> tuple[np.ndarray, np.ndarray]:
+    """Keep the most significant sample (first on ties) of each of ``n_points``
+    index bins; float32 output."""
+    if sense not in ("max", "min"):
+        raise ValueError(f"sense must be 'max' or 'min', got {sense!r}")
+    n = int(frequency.size)
+    if n_points <= 0:
+        return np.empty(0, dtype=np.float32), np.empty(0, dtype=np.float32)
+    if n <= n_points:
+        return frequency.astype(np.float32), power.astype(np.float32)
+    score = power if sense == "max" else -power
+    if not _sum_is_finite(score):
+        score = np.where(np.isnan(score), -np.inf, score)
+    edges = np.linspace(0, n, n_points + 1).astype(np.int64)
+    if n <= _DOWNSAMPLE_VECTOR_MAX_RATIO * n_points:
+        starts = edges[:-1]
+        bin_max = np.maximum.reduceat(score, starts)
+        hits = np.flatnonzero(score == np.repeat(bin_max, np.diff(edges)))
+        picks = hits[np.searchsorted(hits, starts)]
+    else:
+        picks = np.empty(n_points, dtype=np.int64)
+        for i in range(n_points):
+            lo, hi = int(edges[i]), int(edges[i + 1])
+            picks[i] = lo + int(np.argmax(score[lo:hi]))
+    return frequency[picks].astype(np.float32), power[picks].astype(np.float32)
+
+
+__all__ = [
+    "SIDEREAL_FREQ_CPD",
+    "SOLAR_FREQ_CPD",
+    "YEARLY_FREQ_CPD",
+    "is_alias_related",
+    "local_maxima",
+    "peak_preserving_downsample",
+    "select_diverse_peaks",
+    "select_top_peaks",
+]

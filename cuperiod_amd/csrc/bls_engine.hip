@@ -1,0 +1,1008 @@
+// MI355X (gfx950, CDNA4) BLS engine: HIP kernels and the C ABI of include/cuperiod_bls.h.
+//
+// One workgroup processes one (light curve, period segment, frequency tile) work item
+// at a time and, within it, one trial period after another:
+//
+//   (a) fold    every point's phase bin from the reference's FP64 expression
+//               w = x - P*floor(x/P), ind = int(w/bd) + 1 (cuPeriod
+//               src/cuperiod/methods/_bls_core.py:369-378), evaluated through
+//               reciprocal multiplies with an exact-division fallback whenever the
+//               quotient lies within a few ulps of an integer, so every bin index is
+//               the one the FP64 division gives; the point's weighted flux and weight
+//               are added to LDS bins as 64-bit fixed-point integers (ds_add_u64):
+//               exact, so the bins do not depend on the atomics' order;
+//   (a') wrap   the reference's wrap COPY of `oversample` bins (:379-383);
+//   (b) scan    wave-level inclusive prefix sums (cross-lane shuffles, one barrier
+//               between waves), stored split into 32-bit high and low words;
+//               then the box sweep over (duration, start): an FP32 screen on the high
+//               words finds the best box value per (thread, duration), and only boxes
+//               the screen cannot rule out are re-evaluated exactly in FP64 from the
+//               full 64-bit sums, keeping the first strict maximum in (duration index,
+//               start) order (:398-442, :603-625);
+//   (c) final   the seven outputs in FP64 from the winning box (:443-461, :627-654),
+//               written to HBM.
+//
+// Compiled with -ffp-contract=off: the fold must round every product and difference
+// exactly as the FP64 reference does.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cuperiod_bls.h"
+
+namespace {
+
+constexpr int kWave = 64;
+// Fixed point: a curve's sum of |y*ivar| (and of ivar) maps to at most 2^59 units, so
+// prefix sums that also count the `oversample` wrapped bins twice stay below 2^61 and
+// their high words (value >> 32) below 2^29 in magnitude.
+constexpr int kFixBits = 59;
+// Screen margin: boxes whose FP32 value reaches 7/8 of the best FP32 value are
+// re-evaluated exactly. The screen's relative error is below 1/16 whenever the best
+// value is above the curve's `vdeg` bound; below it every box is evaluated exactly.
+constexpr float kKeep = 0.875f;
+
+struct CurveParams {
+    int64_t point_offset;
+    int64_t y_tot;    // sum of fixed-point y*ivar
+    int64_t w_tot;    // sum of fixed-point ivar
+    double y_unit;    // float64 value of one fixed-point unit of y*ivar: 2^(ey-59)
+    double w_unit;    // same for ivar
+    double t_min;
+    double ivar_floor;
+    float ybar;       // y_tot / w_tot
+    float vdeg;       // below: the screen's error may exceed the margin -> exact sweep
+    float vbnd;       // bound on near-empty / near-full boxes the screen leaves out
+    int32_t n_points;
+    int32_t s_hi;     // w_tot >> 32
+    int32_t e_unc;    // high-word weight below which a box side is "uncertain"
+    int32_t flags;    // bit 0: every box exactly; bit 1: non-finite input (no box)
+};
+
+struct WorkItem {
+    int32_t seg;
+    int32_t count;
+    int64_t begin;
+};
+
+// ---------------------------------------------------------------------------------
+// Device helpers
+
+__device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
+    const uint32_t lo = __shfl((uint32_t)(uint64_t)v, src, kWave);
+    const uint32_t hi = __shfl((uint32_t)((uint64_t)v >> 32), src, kWave);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int64_t shfl_up_i64(int64_t v, int delta) {
+    const uint32_t lo = __shfl_up((uint32_t)(uint64_t)v, delta, kWave);
+    const uint32_t hi = __shfl_up((uint32_t)((uint64_t)v >> 32), delta, kWave);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int64_t wave_scan_incl_i64(int64_t v, int lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int64_t o = shfl_up_i64(v, off);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ float wave_max_f32(float v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)(uint64_t)v, off, kWave);
+        const uint32_t hi = __shfl_xor((uint32_t)((uint64_t)v >> 32), off, kWave);
+        v += (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    return v;
+}
+
+// First-wins order of two box candidates: the larger objective, then the smaller
+// (duration index, start) key.
+__device__ __forceinline__ bool better(double a, uint32_t ka, double b, uint32_t kb) {
+    return a > b || (a == b && ka < kb);
+}
+
+__device__ __forceinline__ void wave_argbest(double& obj, uint32_t& key) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double o = __shfl_xor(obj, off, kWave);
+        const uint32_t k = __shfl_xor(key, off, kWave);
+        if (better(o, k, obj, key)) {
+            obj = o;
+            key = k;
+        }
+    }
+}
+
+// Phase bin of one point: `int(w/bd) + 1` with `w = x - P*floor(x/P)`, clipped to
+// [0, width-1], bit for bit as the FP64 divisions give it. The quotients are formed
+// with reciprocals (|error| <= 2^-52 |q|); when a quotient lies within 2^-49 |q| of
+// the integer that decides floor/trunc, the exact division is done instead.
+__device__ __forceinline__ int fold_bin(double x, double P, double invP, double bd,
+                                        double inv_bd, int width) {
+    const double q = x * invP;
+    double k = floor(q);
+    const double fr = q - k;
+    const double tol = q * 0x1p-49;
+    if ((fr < tol && k >= 1.0) || (1.0 - fr) <= tol) k = floor(x / P);
+    const double w = x - P * k;
+    const double r = w * inv_bd;
+    double j = trunc(r);
+    const double rn = rint(r);
+    if (rn != 0.0 && fabs(r - rn) <= fabs(r) * 0x1p-49) j = trunc(w / bd);
+    int ind = (int)j + 1;
+    ind = ind < 0 ? 0 : ind;
+    ind = ind > width - 1 ? width - 1 : ind;
+    return ind;
+}
+
+__device__ __forceinline__ double segment_period(const cpb_segment& sg,
+                                                 const double* __restrict__ periods,
+                                                 int64_t k) {
+    if (sg.mode == 0) return periods[sg.period_offset + k];
+    const int64_t kk = sg.lin_k0 + k;
+    const int64_t i = sg.lin_num - 1 - kk;
+    double f;
+    if (i == sg.lin_num - 1) {
+        f = sg.lin_stop;
+    } else {
+        const double step = (sg.lin_stop - sg.lin_start) / (double)(sg.lin_num - 1);
+        f = (double)i * step + sg.lin_start;
+    }
+    return 1.0 / f;
+}
+
+__device__ __forceinline__ int64_t join64(int32_t hi, uint32_t lo) {
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)lo);
+}
+
+// Exact FP64 statistics of box (n, d) from the 64-bit prefix sums; returns false
+// for an inadmissible box (floor or not a dip), as the reference's sweep skips it.
+template <int OBJ>
+__device__ __forceinline__ bool exact_box(const int2* __restrict__ H,
+                                          const uint2* __restrict__ L, int n, int d,
+                                          const CurveParams& cv, double& obj) {
+    const int2 h0 = H[n], h1 = H[n + d];
+    const uint2 l0 = L[n], l1 = L[n + d];
+    const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
+    const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
+    const double iv_in = (double)win_q * cv.w_unit;
+    const double iv_out = (double)(cv.w_tot - win_q) * cv.w_unit;
+    if (iv_in < cv.ivar_floor || iv_out < cv.ivar_floor) return false;
+    const double y_in = (double)yin_q * cv.y_unit;
+    const double y_rest = (double)(cv.y_tot - yin_q) * cv.y_unit;
+    const double yin = y_in / iv_in;
+    const double yout = y_rest / iv_out;
+    if (yout < yin) return false;
+    const double depth = yout - yin;
+    if (OBJ == 0)
+        obj = depth / sqrt(1.0 / iv_in + 1.0 / iv_out);
+    else
+        obj = 0.5 * iv_in * depth * depth;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------
+// Per-curve preparation: fixed-point conversion and the screen's bounds.
+
+constexpr int kPrepThreads = 256;
+
+__device__ __forceinline__ double block_sum_f64(double v, double* red) {
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    v = wave_sum_f64(v);
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < kPrepThreads / kWave; ++w) s += red[w];
+    return s;
+}
+
+__device__ __forceinline__ double block_max_f64(double v, double* red) {
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    v = wave_max_f64(v);
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int w = 1; w < kPrepThreads / kWave; ++w) s = fmax(s, red[w]);
+    return s;
+}
+
+__device__ __forceinline__ int64_t block_sum_i64(int64_t v, int64_t* red) {
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    v = wave_sum_i64(v);
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    int64_t s = 0;
+    for (int w = 0; w < kPrepThreads / kWave; ++w) s += red[w];
+    return s;
+}
+
+__global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
+    const double* __restrict__ yw, const double* __restrict__ wv,
+    const cpb_curve* __restrict__ curves, int32_t obj_flag,
+    CurveParams* __restrict__ params, int64_t* __restrict__ yq,
+    int64_t* __restrict__ wq) {
+    __shared__ double red_d[kPrepThreads / kWave];
+    __shared__ int64_t red_i[kPrepThreads / kWave];
+    const cpb_curve c = curves[blockIdx.x];
+    const int64_t off = c.point_offset;
+    const int64_t n = c.n_points;
+    const int tid = threadIdx.x;
+
+    double sa = 0.0, sw = 0.0;
+    for (int64_t i = tid; i < n; i += kPrepThreads) {
+        sa += fabs(yw[off + i]);
+        sw += wv[off + i];
+    }
+    sa = block_sum_f64(sa, red_d);
+    sw = block_sum_f64(sw, red_d);
+    const bool finite = isfinite(sa) && isfinite(sw);
+    int ey = 0, ew = 0;
+    if (finite && sa > 0.0) (void)frexp(sa, &ey);
+    if (finite && sw > 0.0) (void)frexp(sw, &ew);
+    const double scale_y = ldexp(1.0, kFixBits - ey);
+    const double scale_w = ldexp(1.0, kFixBits - ew);
+
+    int64_t ty = 0, tw = 0;
+    for (int64_t i = tid; i < n; i += kPrepThreads) {
+        int64_t a = 0, b = 0;
+        if (finite) {
+            a = llrint(yw[off + i] * scale_y);
+            b = llrint(wv[off + i] * scale_w);
+        }
+        yq[off + i] = a;
+        wq[off + i] = b;
+        ty += a;
+        tw += b;
+    }
+    ty = block_sum_i64(ty, red_i);
+    tw = block_sum_i64(tw, red_i);
+    const double ybar = tw > 0 ? (double)ty / (double)tw : 0.0;
+
+    double cmax = 0.0;
+    for (int64_t i = tid; i < n; i += kPrepThreads) {
+        const int64_t b = wq[off + i];
+        if (b > 0) cmax = fmax(cmax, fabs((double)yq[off + i] / (double)b - ybar));
+    }
+    cmax = block_max_f64(cmax, red_d);
+
+    if (tid == 0) {
+        CurveParams p;
+        p.point_offset = off;
+        p.y_tot = ty;
+        p.w_tot = tw;
+        p.y_unit = ldexp(1.0, ey - kFixBits);
+        p.w_unit = ldexp(1.0, ew - kFixBits);
+        p.t_min = c.t_min;
+        p.ivar_floor = c.ivar_floor;
+        p.ybar = (float)ybar;
+        p.n_points = (int32_t)n;
+        const int64_t s_hi = tw >> 32;
+        p.s_hi = (int32_t)s_hi;
+        const int64_t e = s_hi >> 20 > 1 ? s_hi >> 20 : 1;
+        p.e_unc = (int32_t)e;
+        const double terr = 1.0 + fabs(ybar);
+        const double s = (double)s_hi, ed = (double)e;
+        double vdeg, vbnd;
+        if (obj_flag == 0) {
+            vdeg = 4096.0 * terr * terr / (ed * (s - ed));
+            vbnd = 2.0 * cmax * cmax * (ed + 2.0) / (s - ed - 2.0);
+        } else {
+            vdeg = 4096.0 * terr * terr / (ed * ed * (s - ed));
+            vbnd = 2.0 * cmax * cmax * (ed + 2.0) / ((s - ed - 2.0) * (s - ed - 2.0));
+        }
+        p.vdeg = (float)fmin(vdeg, 3.0e38);
+        p.vbnd = (float)fmin(vbnd, 3.0e38);
+        int flags = 0;
+        if (!finite) flags |= 2;
+        if (s_hi < (int64_t(1) << 22)) flags |= 1;  // no weight to speak of: exact sweep
+        p.flags = flags;
+        params[blockIdx.x] = p;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Main kernel.
+
+template <int T, int DMAX, int OBJ, bool WIDE>
+__global__ __launch_bounds__(T) void bls_main_kernel(
+    const double* __restrict__ tau, const int64_t* __restrict__ yq,
+    const int64_t* __restrict__ wq, const CurveParams* __restrict__ curves,
+    const cpb_segment* __restrict__ segs, const WorkItem* __restrict__ items,
+    int64_t n_items, const double* __restrict__ periods,
+    const int32_t* __restrict__ dur_bins, int32_t oversample, int32_t wl,
+    int64_t* __restrict__ wide_scratch, double* __restrict__ out, int64_t n_out) {
+    extern __shared__ int64_t dyn_lds[];
+    constexpr int NW = T / kWave;
+    __shared__ int32_t s_dur[DMAX];
+    __shared__ int64_t s_red_i[2 * NW];
+    __shared__ float s_red_f[NW];
+    __shared__ double s_red_d[NW];
+    __shared__ uint32_t s_red_k[NW];
+
+    const int tid = threadIdx.x;
+    const int lane = tid % kWave;
+    const int wave = tid / kWave;
+
+    int64_t* HY;
+    int64_t* HW;
+    if constexpr (WIDE) {
+        HY = wide_scratch + (int64_t)blockIdx.x * 2 * (int64_t)wl;
+        HW = HY + wl;
+    } else {
+        HY = dyn_lds;
+        HW = dyn_lds + wl;
+    }
+    const int2* H = reinterpret_cast<const int2*>(HY);
+    const uint2* L = reinterpret_cast<const uint2*>(HW);
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const WorkItem it = items[item];
+        const cpb_segment sg = segs[it.seg];
+        const CurveParams cv = curves[sg.curve];
+        const int n_dur = sg.n_dur;
+        __syncthreads();
+        if (tid < n_dur) s_dur[tid] = dur_bins[sg.dur_offset + tid];
+        __syncthreads();
+        const double bd = sg.bin_duration;
+        const double inv_bd = 1.0 / bd;
+        const int width = sg.width;
+        const int64_t poff = cv.point_offset;
+        const int npts = cv.n_points;
+
+        for (int q = 0; q < it.count; ++q) {
+            const int64_t k = it.begin + q;
+            const double P = segment_period(sg, periods, k);
+            const double invP = 1.0 / P;
+            int64_t nb = (int64_t)ceil(P / bd) + oversample;
+            if (nb > width - 1) nb = width - 1;
+            const int n_bins = (int)nb;
+            const int64_t col = sg.out_offset + k;
+
+            if (cv.flags & 2) {
+                if (tid == 0) {
+                    out[col] = -INFINITY;
+                    for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
+                }
+                continue;
+            }
+
+            // (a) zero bins [0, n_bins], then fold every point into them.
+            for (int i = tid; i <= n_bins; i += T) {
+                HY[i] = 0;
+                HW[i] = 0;
+            }
+            if constexpr (WIDE) __threadfence();
+            __syncthreads();
+            for (int j = tid; j < npts; j += T) {
+                const double x = tau[poff + j];
+                const int ind = fold_bin(x, P, invP, bd, inv_bd, width);
+                atomicAdd(reinterpret_cast<unsigned long long*>(HY + ind),
+                          (unsigned long long)yq[poff + j]);
+                atomicAdd(reinterpret_cast<unsigned long long*>(HW + ind),
+                          (unsigned long long)wq[poff + j]);
+            }
+            if constexpr (WIDE) __threadfence();
+            __syncthreads();
+            // (a') wrap copy, in order (a copy may read a bin an earlier one wrote).
+            if (tid == 0) {
+                for (int kk = 0; kk < oversample; ++kk) {
+                    const int dst = n_bins - oversample + kk;
+                    if (dst >= 0 && dst < width) {
+                        HY[dst] = HY[1 + kk];
+                        HW[dst] = HW[1 + kk];
+                    }
+                }
+            }
+            if constexpr (WIDE) __threadfence();
+            __syncthreads();
+
+            // (b1) inclusive prefix over [0, n_bins]: each wave scans a contiguous
+            // quarter with cross-lane shuffles, then adds the earlier waves' totals.
+            {
+                const int span = n_bins + 1;
+                const int per = (span + NW - 1) / NW;
+                const int lo = wave * per;
+                const int hi = min(span, lo + per);
+                int64_t cy = 0, cw = 0;
+                for (int base = lo; base < hi; base += kWave) {
+                    const int i = base + lane;
+                    int64_t vy = 0, vw = 0;
+                    if (i < hi) {
+                        vy = HY[i];
+                        vw = HW[i];
+                    }
+                    vy = wave_scan_incl_i64(vy, lane) + cy;
+                    vw = wave_scan_incl_i64(vw, lane) + cw;
+                    if (i < hi) {
+                        HY[i] = vy;
+                        HW[i] = vw;
+                    }
+                    cy = shfl_i64(vy, kWave - 1);
+                    cw = shfl_i64(vw, kWave - 1);
+                }
+                if (lane == 0) {
+                    s_red_i[2 * wave] = cy;
+                    s_red_i[2 * wave + 1] = cw;
+                }
+                if constexpr (WIDE) __threadfence();
+                __syncthreads();
+                int64_t oy = 0, ow = 0;
+                for (int w = 0; w < wave; ++w) {
+                    oy += s_red_i[2 * w];
+                    ow += s_red_i[2 * w + 1];
+                }
+                int2* Hw = reinterpret_cast<int2*>(HY);
+                uint2* Lw = reinterpret_cast<uint2*>(HW);
+                for (int i = lo + lane; i < hi; i += kWave) {
+                    const uint64_t vy = (uint64_t)(HY[i] + oy);
+                    const uint64_t vw = (uint64_t)(HW[i] + ow);
+                    Hw[i] = make_int2((int32_t)(vy >> 32), (int32_t)(vw >> 32));
+                    Lw[i] = make_uint2((uint32_t)vy, (uint32_t)vw);
+                }
+            }
+            if constexpr (WIDE) __threadfence();
+            __syncthreads();
+
+            // (b2) FP32 screen: best value per (thread, duration).
+            const float ybar = cv.ybar;
+            const int s_hi = cv.s_hi;
+            const int e_unc = cv.e_unc;
+            float cm[DMAX];
+            uint32_t unc_bits = 0;
+            float vmax = -INFINITY;
+#pragma unroll
+            for (int di = 0; di < DMAX; ++di) {
+                cm[di] = -INFINITY;
+                if (di < n_dur) {
+                    const int d = s_dur[di];
+                    if (d >= 1 && d < n_bins) {
+                        const int m = n_bins - d + 1;
+                        float vm = -INFINITY;
+                        bool uf = false;
+                        for (int n = tid; n < m; n += T) {
+                            const int2 h0 = H[n], h1 = H[n + d];
+                            const int dw = h1.y - h0.y;
+                            const int dwo = s_hi - dw;
+                            const float Y = (float)(h1.x - h0.x);
+                            const float W = (float)dw;
+                            const float Wo = (float)dwo;
+                            const float t = fmaf(-ybar, W, Y);
+                            const float den = OBJ == 0 ? W * Wo : W * Wo * Wo;
+                            float v = (-t) * fabsf(t) * __builtin_amdgcn_rcpf(den);
+                            const bool cert = (dw > e_unc) & (dwo > e_unc);
+                            v = cert ? v : -INFINITY;
+                            vm = fmaxf(vm, v);
+                            if (OBJ == 1) uf |= (dw > e_unc) & (dwo <= e_unc) & (dwo > -2);
+                        }
+                        cm[di] = vm;
+                        if (uf) unc_bits |= 1u << di;
+                    }
+                }
+                vmax = fmaxf(vmax, cm[di]);
+            }
+            vmax = wave_max_f32(vmax);
+            if (lane == 0) s_red_f[wave] = vmax;
+            __syncthreads();
+            float V = s_red_f[0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) V = fmaxf(V, s_red_f[w]);
+            const bool force = (cv.flags & 1) || !(V >= cv.vdeg) || !(V >= cv.vbnd);
+            const float thr = V * kKeep;
+
+            // (b3) exact FP64 re-evaluation of what the screen kept, first wins.
+            double best = -INFINITY;
+            uint32_t bkey = 0xFFFFFFFFu;
+#pragma unroll
+            for (int di = 0; di < DMAX; ++di) {
+                if (di < n_dur) {
+                    const int d = s_dur[di];
+                    const bool want = force || (cm[di] >= thr) || ((unc_bits >> di) & 1u);
+                    if (d >= 1 && d < n_bins && want) {
+                        const int m = n_bins - d + 1;
+                        for (int n = tid; n < m; n += T) {
+                            const int2 h0 = H[n], h1 = H[n + d];
+                            const int dw = h1.y - h0.y;
+                            const int dwo = s_hi - dw;
+                            const float Y = (float)(h1.x - h0.x);
+                            const float W = (float)dw;
+                            const float Wo = (float)dwo;
+                            const float t = fmaf(-ybar, W, Y);
+                            const float den = OBJ == 0 ? W * Wo : W * Wo * Wo;
+                            const float v = (-t) * fabsf(t) * __builtin_amdgcn_rcpf(den);
+                            const bool cert = (dw > e_unc) & (dwo > e_unc);
+                            const bool unc =
+                                OBJ == 1 && (dw > e_unc) & (dwo <= e_unc) & (dwo > -2);
+                            if (force || (cert && v >= thr) || unc) {
+                                double obj;
+                                if (exact_box<OBJ>(H, L, n, d, cv, obj) && obj > best) {
+                                    best = obj;
+                                    bkey = ((uint32_t)di << 24) | (uint32_t)n;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            wave_argbest(best, bkey);
+            if (lane == 0) {
+                s_red_d[wave] = best;
+                s_red_k[wave] = bkey;
+            }
+            __syncthreads();
+
+            // (c) finalize in FP64 and write the seven outputs.
+            if (tid == 0) {
+                double b = s_red_d[0];
+                uint32_t bk = s_red_k[0];
+                for (int w = 1; w < NW; ++w) {
+                    if (better(s_red_d[w], s_red_k[w], b, bk)) {
+                        b = s_red_d[w];
+                        bk = s_red_k[w];
+                    }
+                }
+                if (!(b > -INFINITY)) {
+                    out[col] = -INFINITY;
+                    for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
+                } else {
+                    const int n = (int)(bk & 0xFFFFFFu);
+                    const int d = s_dur[bk >> 24];
+                    const int2 h0 = H[n], h1 = H[n + d];
+                    const uint2 l0 = L[n], l1 = L[n + d];
+                    const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
+                    const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
+                    const double iv_in = (double)win_q * cv.w_unit;
+                    const double iv_out = (double)(cv.w_tot - win_q) * cv.w_unit;
+                    const double y_in = (double)yin_q * cv.y_unit;
+                    const double y_rest = (double)(cv.y_tot - yin_q) * cv.y_unit;
+                    const double yin = y_in / iv_in;
+                    const double yout = y_rest / iv_out;
+                    const double depth = yout - yin;
+                    const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
+                    const double dsnr = depth / derr;
+                    const double ll = 0.5 * iv_in * depth * depth;
+                    const double dur = (double)d * bd;
+                    out[col] = OBJ == 0 ? dsnr : ll;
+                    out[1 * n_out + col] = depth;
+                    out[2 * n_out + col] = derr;
+                    out[3 * n_out + col] = dsnr;
+                    out[4 * n_out + col] = dur;
+                    out[5 * n_out + col] = fmod((double)n * bd + 0.5 * dur, P) + cv.t_min;
+                    out[6 * n_out + col] = ll;
+                }
+            }
+            if constexpr (WIDE) __threadfence();
+            __syncthreads();
+        }
+    }
+}
+
+// Columns of a segment with no box width: power -inf, every other output 0.
+__global__ void fill_empty_kernel(double* __restrict__ out, int64_t n_out, int64_t off,
+                                  int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    out[off + i] = -INFINITY;
+    for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + off + i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------------
+// Host side.
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct cpb_context {
+    int device = 0;
+    std::string error;
+    int lds_limit = 65536;
+    int n_cu = 1;
+    // `tables` holds the curve, segment and work-item descriptors of one call; they
+    // are staged in pinned host memory (`h_tables`) and sent in one copy. The event
+    // marks that copy: the staging buffer is not rewritten before it has completed.
+    DevBuf tables, curves, yq, wq, wide;
+    void* h_tables = nullptr;
+    size_t h_tables_bytes = 0;
+    hipEvent_t copied = nullptr;
+    bool copy_pending = false;
+    std::vector<WorkItem> h_items;
+};
+
+namespace {
+
+int fail(cpb_context* ctx, int status, const std::string& msg) {
+    if (ctx) ctx->error = msg;
+    return status;
+}
+
+int hip_fail(cpb_context* ctx, hipError_t e, const char* what) {
+    std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(ctx, CPB_E_DEVICE, msg);
+}
+
+#define CPB_HIP(ctx, expr)                                          \
+    do {                                                            \
+        hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return hip_fail((ctx), _e, #expr);    \
+    } while (0)
+
+int ensure(cpb_context* ctx, DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes && b.ptr) return CPB_OK;
+    if (b.ptr) {
+        hipError_t e = hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipFree");
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    want = want + want / 4;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    b.ptr = p;
+    b.bytes = want;
+    return CPB_OK;
+}
+
+template <int T, int DMAX, int OBJ, bool WIDE>
+hipError_t launch_main(dim3 grid, size_t lds, hipStream_t st, const double* tau,
+                       const int64_t* yq, const int64_t* wq, const CurveParams* curves,
+                       const cpb_segment* segs, const WorkItem* items, int64_t n_items,
+                       const double* periods, const int32_t* dur, int32_t os, int32_t wl,
+                       int64_t* wide, double* out, int64_t n_out) {
+    auto fn = bls_main_kernel<T, DMAX, OBJ, WIDE>;
+    if (!WIDE && lds > 65536) {
+        // Opt in to more than 64 KiB of dynamic LDS where the runtime asks for it; a
+        // refusal surfaces at launch.
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    }
+    hipLaunchKernelGGL(fn, grid, dim3(T), lds, st, tau, yq, wq, curves, segs, items,
+                       n_items, periods, dur, os, wl, wide, out, n_out);
+    return hipGetLastError();
+}
+
+template <int DMAX, int OBJ, bool WIDE>
+hipError_t dispatch_t(dim3 grid, size_t lds, hipStream_t st, const double* tau,
+                      const int64_t* yq, const int64_t* wq, const CurveParams* curves,
+                      const cpb_segment* segs, const WorkItem* items, int64_t n_items,
+                      const double* periods, const int32_t* dur, int32_t os, int32_t wl,
+                      int64_t* wide, double* out, int64_t n_out) {
+    return launch_main<256, DMAX, OBJ, WIDE>(grid, lds, st, tau, yq, wq, curves, segs,
+                                             items, n_items, periods, dur, os, wl, wide,
+                                             out, n_out);
+}
+
+template <int OBJ, bool WIDE>
+hipError_t dispatch_d(int dmax, dim3 grid, size_t lds, hipStream_t st, const double* tau,
+                      const int64_t* yq, const int64_t* wq, const CurveParams* curves,
+                      const cpb_segment* segs, const WorkItem* items, int64_t n_items,
+                      const double* periods, const int32_t* dur, int32_t os, int32_t wl,
+                      int64_t* wide, double* out, int64_t n_out) {
+#define CPB_D(DM)                                                                     \
+    return dispatch_t<DM, OBJ, WIDE>(grid, lds, st, tau, yq, wq, curves, segs, items, \
+                                     n_items, periods, dur, os, wl, wide, out, n_out)
+    if (dmax <= 8) CPB_D(8);
+    if (dmax <= 16) CPB_D(16);
+    CPB_D(32);
+#undef CPB_D
+}
+
+constexpr int kMaxDur = 32;
+
+int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double* yw,
+               const double* wv, int64_t total_points, const cpb_curve* curves,
+               int32_t n_curves, const double* periods, const int32_t* dur_bins,
+               const cpb_segment* segments, int32_t n_segments, int32_t oversample,
+               int32_t obj_flag, double* out, int64_t n_out) {
+    if (!ctx) return CPB_E_INVALID;
+    if (obj_flag != 0 && obj_flag != 1) return fail(ctx, CPB_E_INVALID, "obj_flag must be 0 or 1");
+    if (oversample < 1) return fail(ctx, CPB_E_INVALID, "oversample must be >= 1");
+    if (n_curves < 1 || n_segments < 0 || total_points < 0 || n_out < 0)
+        return fail(ctx, CPB_E_INVALID, "negative size");
+    if (!curves || (n_segments > 0 && !segments))
+        return fail(ctx, CPB_E_INVALID, "null descriptor array");
+    for (int32_t c = 0; c < n_curves; ++c) {
+        const cpb_curve& cv = curves[c];
+        if (cv.point_offset < 0 || cv.n_points < 0 ||
+            cv.point_offset + cv.n_points > total_points)
+            return fail(ctx, CPB_E_INVALID, "curve points out of range");
+        if (cv.n_points > (int64_t(1) << 30))
+            return fail(ctx, CPB_E_RESOURCE, "more than 2^30 points in one light curve");
+    }
+    int dmax = 1, wmax = 2;
+    int64_t total_periods = 0;
+    for (int32_t s = 0; s < n_segments; ++s) {
+        const cpb_segment& sg = segments[s];
+        if (sg.curve < 0 || sg.curve >= n_curves)
+            return fail(ctx, CPB_E_INVALID, "segment curve index out of range");
+        if (sg.n_periods < 0 || sg.out_offset < 0 || sg.out_offset + sg.n_periods > n_out)
+            return fail(ctx, CPB_E_INVALID, "segment output range out of bounds");
+        if (sg.n_dur < 0 || sg.n_dur > kMaxDur)
+            return fail(ctx, CPB_E_RESOURCE,
+                        "more than 32 distinct box widths in one segment");
+        if (sg.width < 2) return fail(ctx, CPB_E_INVALID, "segment width must be >= 2");
+        if (!(sg.bin_duration > 0.0))
+            return fail(ctx, CPB_E_INVALID, "bin_duration must be positive");
+        if (sg.mode == 0 && sg.n_periods > 0 && !periods)
+            return fail(ctx, CPB_E_INVALID, "explicit periods needed");
+        if (sg.mode == 1 && (sg.lin_num < 2 || sg.lin_k0 < 0 ||
+                             sg.lin_k0 + sg.n_periods > sg.lin_num))
+            return fail(ctx, CPB_E_INVALID, "bad linspace segment");
+        if (sg.n_dur > 0 && sg.n_periods > 0) {
+            dmax = std::max(dmax, (int)sg.n_dur);
+            wmax = std::max(wmax, (int)sg.width);
+        }
+        total_periods += sg.n_periods;
+    }
+    (void)total_periods;
+
+    if (wmax > (1 << 24))
+        return fail(ctx, CPB_E_RESOURCE, "more than 2^24 phase bins per period");
+
+    CPB_HIP(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure(ctx, ctx->curves, sizeof(CurveParams) * n_curves))) return rc;
+    if ((rc = ensure(ctx, ctx->yq, sizeof(int64_t) * std::max<int64_t>(1, total_points)))) return rc;
+    if ((rc = ensure(ctx, ctx->wq, sizeof(int64_t) * std::max<int64_t>(1, total_points)))) return rc;
+
+    // Segments with no periods or no box widths: power -inf, the rest 0 (the
+    // reference's `_empty_output`, src/cuperiod/methods/_bls_core.py:843-847).
+    // Work items: tiles of consecutive periods of one segment.
+    int64_t live = 0;
+    for (int32_t s = 0; s < n_segments; ++s)
+        if (segments[s].n_dur > 0) live += segments[s].n_periods;
+    const int64_t target_items = (int64_t)ctx->n_cu * 64;
+    int64_t tile = live / std::max<int64_t>(1, target_items);
+    tile = std::max<int64_t>(1, std::min<int64_t>(tile, 32));
+    ctx->h_items.clear();
+    std::vector<cpb_segment> empties;
+    for (int32_t s = 0; s < n_segments; ++s) {
+        const cpb_segment& sg = segments[s];
+        if (sg.n_periods == 0) continue;
+        if (sg.n_dur == 0) {
+            empties.push_back(sg);
+            continue;
+        }
+        for (int64_t b = 0; b < sg.n_periods; b += tile) {
+            WorkItem w;
+            w.seg = s;
+            w.begin = b;
+            w.count = (int32_t)std::min<int64_t>(tile, sg.n_periods - b);
+            ctx->h_items.push_back(w);
+        }
+    }
+    const int64_t n_items = (int64_t)ctx->h_items.size();
+
+    // Descriptor tables: [curves | segments | work items], 256-byte aligned parts.
+    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t off_seg = align(sizeof(cpb_curve) * n_curves);
+    const size_t off_item = off_seg + align(sizeof(cpb_segment) * std::max(1, n_segments));
+    const size_t tbytes = off_item + align(sizeof(WorkItem) * std::max<int64_t>(1, n_items));
+    if ((rc = ensure(ctx, ctx->tables, tbytes))) return rc;
+    if (ctx->copy_pending) {
+        CPB_HIP(ctx, hipEventSynchronize(ctx->copied));
+        ctx->copy_pending = false;
+    }
+    if (ctx->h_tables_bytes < tbytes) {
+        if (ctx->h_tables) CPB_HIP(ctx, hipHostFree(ctx->h_tables));
+        ctx->h_tables = nullptr;
+        ctx->h_tables_bytes = 0;
+        CPB_HIP(ctx, hipHostMalloc(&ctx->h_tables, tbytes + tbytes / 4, hipHostMallocDefault));
+        ctx->h_tables_bytes = tbytes + tbytes / 4;
+    }
+    char* hb = static_cast<char*>(ctx->h_tables);
+    std::memcpy(hb, curves, sizeof(cpb_curve) * n_curves);
+    if (n_segments > 0) std::memcpy(hb + off_seg, segments, sizeof(cpb_segment) * n_segments);
+    if (n_items > 0)
+        std::memcpy(hb + off_item, ctx->h_items.data(), sizeof(WorkItem) * n_items);
+    CPB_HIP(ctx, hipMemcpyAsync(ctx->tables.ptr, hb, tbytes, hipMemcpyHostToDevice, st));
+    CPB_HIP(ctx, hipEventRecord(ctx->copied, st));
+    ctx->copy_pending = true;
+    char* db = static_cast<char*>(ctx->tables.ptr);
+    const auto* curve_in = reinterpret_cast<const cpb_curve*>(db);
+    const auto* sp = reinterpret_cast<const cpb_segment*>(db + off_seg);
+    const auto* ip = reinterpret_cast<const WorkItem*>(db + off_item);
+
+    hipLaunchKernelGGL(prep_curves_kernel, dim3(n_curves), dim3(kPrepThreads), 0, st, yw,
+                       wv, curve_in, obj_flag,
+                       static_cast<CurveParams*>(ctx->curves.ptr),
+                       static_cast<int64_t*>(ctx->yq.ptr), static_cast<int64_t*>(ctx->wq.ptr));
+    CPB_HIP(ctx, hipGetLastError());
+
+    for (const cpb_segment& sg : empties) {
+        const unsigned blocks = (unsigned)((sg.n_periods + 255) / 256);
+        hipLaunchKernelGGL(fill_empty_kernel, dim3(blocks), dim3(256), 0, st, out, n_out,
+                           sg.out_offset, sg.n_periods);
+        CPB_HIP(ctx, hipGetLastError());
+    }
+    if (n_items == 0) return CPB_OK;
+
+    const int wl = (wmax + 63) / 64 * 64;
+    const size_t static_lds = 2048;
+    const size_t lds_need = (size_t)16 * wl;
+    const bool wide = lds_need + static_lds > (size_t)ctx->lds_limit;
+    int64_t* wide_ptr = nullptr;
+    dim3 grid;
+    size_t lds = 0;
+    if (wide) {
+        const int64_t g = std::min<int64_t>(n_items, (int64_t)ctx->n_cu * 4);
+        if ((rc = ensure(ctx, ctx->wide, (size_t)g * 16 * (size_t)wl))) return rc;
+        wide_ptr = static_cast<int64_t*>(ctx->wide.ptr);
+        grid = dim3((unsigned)g);
+    } else {
+        grid = dim3((unsigned)std::min<int64_t>(n_items, (int64_t)1 << 30));
+        lds = lds_need;
+    }
+    const auto* cp = static_cast<const CurveParams*>(ctx->curves.ptr);
+    const auto* yqp = static_cast<const int64_t*>(ctx->yq.ptr);
+    const auto* wqp = static_cast<const int64_t*>(ctx->wq.ptr);
+    hipError_t e;
+    if (obj_flag == 0) {
+        e = wide ? dispatch_d<0, true>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
+                                       periods, dur_bins, oversample, wl, wide_ptr, out, n_out)
+                 : dispatch_d<0, false>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
+                                        periods, dur_bins, oversample, wl, wide_ptr, out, n_out);
+    } else {
+        e = wide ? dispatch_d<1, true>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
+                                       periods, dur_bins, oversample, wl, wide_ptr, out, n_out)
+                 : dispatch_d<1, false>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
+                                        periods, dur_bins, oversample, wl, wide_ptr, out, n_out);
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "bls_main_kernel launch");
+    return CPB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpb_version(void) { return 100; }
+
+int cpb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int cpb_create(int device, cpb_context** out) {
+    if (!out) return CPB_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CPB_E_NODEVICE;
+    if (device < 0 || device >= n) return CPB_E_INVALID;
+    auto* ctx = new cpb_context();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete ctx;
+        return CPB_E_DEVICE;
+    }
+    if (hipEventCreateWithFlags(&ctx->copied, hipEventDisableTiming) != hipSuccess) {
+        delete ctx;
+        return CPB_E_DEVICE;
+    }
+    int lds = 0, cus = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
+            hipSuccess && lds > 0)
+        ctx->lds_limit = lds;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) ==
+            hipSuccess && cus > 0)
+        ctx->n_cu = cus;
+    *out = ctx;
+    return CPB_OK;
+}
+
+void cpb_destroy(cpb_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->copy_pending) (void)hipEventSynchronize(ctx->copied);
+    for (DevBuf* b : {&ctx->tables, &ctx->curves, &ctx->yq, &ctx->wq, &ctx->wide})
+        if (b->ptr) (void)hipFree(b->ptr);
+    if (ctx->h_tables) (void)hipHostFree(ctx->h_tables);
+    if (ctx->copied) (void)hipEventDestroy(ctx->copied);
+    delete ctx;
+}
+
+const char* cpb_last_error(const cpb_context* ctx) {
+    return ctx ? ctx->error.c_str() : "null context";
+}
+
+const char* cpb_status_string(int status) {
+    switch (status) {
+        case CPB_OK: return "ok";
+        case CPB_E_INVALID: return "invalid argument";
+        case CPB_E_DEVICE: return "HIP runtime error";
+        case CPB_E_RESOURCE: return "resource limit exceeded";
+        case CPB_E_NODEVICE: return "no HIP device";
+        default: return "unknown status";
+    }
+}
+
+int cpb_bls_search(cpb_context* ctx, void* stream, const double* tau, const double* yw,
+                   const double* wv, int64_t n_points, double t_min, double ivar_floor,
+                   const double* periods, const int32_t* dur_bins,
+                   const cpb_segment* segments, int32_t n_segments, int32_t oversample,
+                   int32_t obj_flag, double* out, int64_t n_out) {
+    if (!ctx) return CPB_E_INVALID;
+    for (int32_t s = 0; s < n_segments; ++s)
+        if (segments[s].curve != 0)
+            return fail(ctx, CPB_E_INVALID, "single-curve search: segment.curve must be 0");
+    cpb_curve c;
+    c.point_offset = 0;
+    c.n_points = n_points;
+    c.t_min = t_min;
+    c.ivar_floor = ivar_floor;
+    return run_search(ctx, static_cast<hipStream_t>(stream), tau, yw, wv, n_points, &c, 1,
+                      periods, dur_bins, segments, n_segments, oversample, obj_flag, out,
+                      n_out);
+}
+
+int cpb_bls_search_batch(cpb_context* ctx, void* stream, const double* tau,
+                         const double* yw, const double* wv, int64_t total_points,
+                         const cpb_curve* curves, int32_t n_curves,
+                         const double* periods, const int32_t* dur_bins,
+                         const cpb_segment* segments, int32_t n_segments,
+                         int32_t oversample, int32_t obj_flag, double* out,
+                         int64_t n_out) {
+    return run_search(ctx, static_cast<hipStream_t>(stream), tau, yw, wv, total_points,
+                      curves, n_curves, periods, dur_bins, segments, n_segments,
+                      oversample, obj_flag, out, n_out);
+}
+
+int cpb_linspace_grid(double start, double stop, int64_t num, double* out) {
+    if (num < 0 || (num > 0 && !out)) return CPB_E_INVALID;
+    if (num == 0) return CPB_OK;
+    if (num == 1) {
+        out[0] = start;
+        return CPB_OK;
+    }
+    const double step = (stop - start) / (double)(num - 1);
+    for (int64_t i = 0; i < num - 1; ++i) out[i] = (double)i * step + start;
+    out[num - 1] = stop;
+    return CPB_OK;
+}
+
+int cpb_synchronize(cpb_context* ctx, void* stream) {
+    if (!ctx) return CPB_E_INVALID;
+    CPB_HIP(ctx, hipSetDevice(ctx->device));
+    CPB_HIP(ctx, hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return CPB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+/*
+ * C ABI of the MI355X (gfx950) BLS engine.
+ *
+ * Plain pointers and sizes only. Every array argument that names a device buffer is a
+ * HIP device pointer on the context's device; `stream` is a hipStream_t (NULL = the
+ * null stream). Calls enqueue work on `stream` and return without waiting, unless
+ * stated otherwise. A context is not thread-safe: use one per thread or process
+ * (as cuPeriod's engines, src/cuperiod/methods/base.py:388-396). Distinct contexts may
+ * be used concurrently.
+ *
+ * Status codes: CPB_OK, or one of the CPB_E_* values; cpb_last_error() returns the
+ * message of the last failure on a context.
+ */
+#ifndef CUPERIOD_BLS_H
+#define CUPERIOD_BLS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPB_OK 0
+#define CPB_E_INVALID 1     /* bad argument (maps to ValueError) */
+#define CPB_E_DEVICE 2      /* HIP runtime failure (maps to DeviceError) */
+#define CPB_E_RESOURCE 3    /* a resource limit, e.g. too many durations (BackendUnavailableError) */
+#define CPB_E_NODEVICE 4    /* no usable GPU (BackendUnavailableError) */
+
+#define CPB_N_OUTPUTS 7     /* power, depth, depth_err, depth_snr, duration, transit_time, log_likelihood */
+
+typedef struct cpb_context cpb_context;
+
+/* One duration-homogeneous period block of a light curve: the `_SearchSpec` of the
+ * reference (src/cuperiod/methods/_bls_core.py:687-706). Periods are either explicit
+ * (`periods[period_offset + k]`) or the reciprocals of an np.linspace frequency grid
+ * taken in ascending-period order (mode 1; see cpb_linspace_grid). */
+typedef struct cpb_segment {
+    double bin_duration;    /* min(durations) / oversample */
+    int64_t period_offset;  /* mode 0: index of the first period in `periods` */
+    int64_t n_periods;      /* periods in this segment */
+    int64_t out_offset;     /* column of the first period in the outputs */
+    int32_t width;          /* ceil(max period / bin_duration) + oversample + 1 */
+    int32_t dur_offset;     /* first box width of this segment in `dur_bins` */
+    int32_t n_dur;          /* box widths (bins) of this segment */
+    int32_t curve;          /* light-curve index (0 for single-curve calls) */
+    int32_t mode;           /* 0: explicit periods; 1: reciprocal linspace */
+    int32_t reserved;
+    double lin_start;       /* mode 1: np.linspace(start, stop, num) frequencies */
+    double lin_stop;
+    int64_t lin_num;
+    int64_t lin_k0;         /* mode 1: ascending-period index of the first period */
+} cpb_segment;
+
+/* Per light curve inputs of a batched search. */
+typedef struct cpb_curve {
+    int64_t point_offset;   /* first point in the concatenated tau/yw/wv arrays */
+    int64_t n_points;
+    double t_min;           /* time origin subtracted from tau (added to transit_time) */
+    double ivar_floor;      /* `_ivar_floor(N, sum(ivar))` (src/cuperiod/methods/_bls_core.py:75-94) */
+} cpb_curve;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int cpb_version(void);
+
+/* Number of HIP devices visible to the process (0 when none). */
+int cpb_device_count(void);
+
+/* Create a context bound to HIP device `device`. */
+int cpb_create(int device, cpb_context** out);
+
+/* Release a context and its device buffers. NULL is accepted. */
+void cpb_destroy(cpb_context* ctx);
+
+/* Message of the last failure on `ctx` ("" when none). */
+const char* cpb_last_error(const cpb_context* ctx);
+
+/* Static description of a status code. */
+const char* cpb_status_string(int status);
+
+/*
+ * BLS box search of one light curve over several period segments, with one set of
+ * device inputs and one (7, n_out) float64 output.
+ *
+ * Replaces `_bls_search_cuda_specs` (src/cuperiod/methods/_bls_core.py:741-840) and the
+ * `bls_block` kernel it launches per segment (:505-656): the same inputs (tau = t - t_min,
+ * yw = y*ivar, wv = ivar; the segment binning; oversample; objective; ivar floor) and
+ * the same seven outputs per period. Results are exact-arithmetic box statistics
+ * (64-bit fixed-point phase bins), rounded once to float64.
+ *
+ * tau, yw, wv   device float64[n_points]
+ * periods       device float64[...], explicit periods of mode-0 segments (may be NULL)
+ * dur_bins      device int32[...], box widths of every segment, concatenated
+ * segments      HOST array of n_segments descriptors (curve must be 0)
+ * obj_flag      0 = depth SNR, 1 = log-likelihood
+ * out           device float64[CPB_N_OUTPUTS * n_out], row k at out + k*n_out
+ */
+int cpb_bls_search(cpb_context* ctx, void* stream, const double* tau, const double* yw,
+                   const double* wv, int64_t n_points, double t_min, double ivar_floor,
+                   const double* periods, const int32_t* dur_bins,
+                   const cpb_segment* segments, int32_t n_segments, int32_t oversample,
+                   int32_t obj_flag, double* out, int64_t n_out);
+
+/*
+ * Batched search: many light curves, each with its own segments, in one launch
+ * sequence (the reference loops over curves, src/cuperiod/batch/runner.py:214-250).
+ * Points of all curves are concatenated; `curves` and `segments` are HOST arrays;
+ * segment.curve indexes `curves`; outputs of all segments share one (7, n_out) array.
+ */
+int cpb_bls_search_batch(cpb_context* ctx, void* stream, const double* tau,
+                         const double* yw, const double* wv, int64_t total_points,
+                         const cpb_curve* curves, int32_t n_curves,
+                         const double* periods, const int32_t* dur_bins,
+                         const cpb_segment* segments, int32_t n_segments,
+                         int32_t oversample, int32_t obj_flag, double* out,
+                         int64_t n_out);
+
+/* Frequencies of np.linspace(start, stop, num) by numpy's recipe (value_i = i*step +
+ * start, last = stop), computed on the HOST: the values mode-1 segments use. */
+int cpb_linspace_grid(double start, double stop, int64_t num, double* out);
+
+/* Block until all work queued on `stream` has finished. */
+int cpb_synchronize(cpb_context* ctx, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUPERIOD_BLS_H */
