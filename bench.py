@@ -1,0 +1,236 @@
+"""Benchmark: batched plain BLS on one MI355X (and the CPU oracle beside it).
+
+Workload (default): the 4,096-curve survey batch -- N log-uniform in [100, 5,000],
+baselines 50-2,000 d, 200,000 frequencies per curve in [1/baseline, 2] /d, 6
+period-scaled durations per log segment (``cuperiod_amd.workloads.config3``).
+
+A step is one batched search of every curve (fold, box scan, argmax, per-curve
+summary with the best period) with the inputs already resident in HBM. W untimed warm-up
+steps, then K timed steps between two device synchronisations. One JSON line is printed.
+
+``roofline``: the main kernel's algorithmic FLOP (4 FP64 per point-fold, 8 FP32 per box
+evaluation) per launch over its mean HIP-event duration, against the 157.3 TFLOP/s FP32
+peak; ``detail`` adds point-folds/s, box-evaluations/s and the LDS bytes they imply
+(16 B each) against 256 CUs x 128 B/clk x 2.4 GHz.
+
+``cpu_baseline``: the FP64 C++ oracle (the reference's numba kernel restated, with its
+screen) on the first ``--cpu-curves`` curves of the same batch, all host threads.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("light curves/s, 200k-freq BLS, 1 GPU: learned+refine vs plain, best-period "
+          "match %")
+PEAK_FP32 = 157.3e12
+PEAK_LDS = 256 * 128 * 2.4e9
+PEAK_HBM = 8.0e12
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def count_work(plans, n_points, oversample) -> tuple[float, float]:
+    """Exact point-folds and box evaluations of one launch over a batch."""
+    from cuperiod_amd.methods._bls_core import _duration_bins
+
+    folds = 0.0
+    boxes = 0.0
+    for plan, n in zip(plans, n_points):
+        for sg in plan.segments:
+            if sg.mode == 0:
+                periods = sg.periods
+            else:
+                k = np.arange(sg.n_periods, dtype=np.int64)
+                i = sg.lin_num - 1 - (sg.lin_k0 + k)
+                f = i.astype(np.float64) * sg.lin_step + sg.lin_start
+                if sg.mode == 1:
+                    f = np.where(i == sg.lin_num - 1, sg.lin_stop, f)
+                else:
+                    f = np.where(i == 0, sg.lin_start, f)
+                periods = 1.0 / f
+            bd = float(sg.durations.min()) / oversample
+            width = int(np.ceil(sg.p_max / bd)) + oversample + 1
+            nb = np.minimum(np.ceil(periods / bd).astype(np.int64) + oversample, width - 1)
+            folds += float(n) * sg.n_periods
+            for d in _duration_bins(sg.durations, bd):
+                m = nb[nb > d]
+                boxes += float((m - d + 1).sum())
+    return folds, boxes
+
+
+def cpu_baseline(wl, settings, n_curves: int):
+    """The oracle on the first ``n_curves`` curves (same grids, same settings)."""
+    import oracle
+    from cuperiod_amd.methods.bls import _grid_segments
+
+    threads = oracle.default_threads()
+    jobs = []
+    for c in range(n_curves):
+        t, y, dy = wl.curves[c]
+        segs = _grid_segments(wl.grids[c].as_gridspec(), settings)
+        jobs.append((t, y, dy, segs))
+    oracle.bls_power(jobs[0][0][:50], jobs[0][1][:50], jobs[0][2][:50],
+                     np.array([1.0, 2.0]), np.array([0.1]), 10)  # load + warm
+    results = []
+    t0 = time.perf_counter()
+    for t, y, dy, segs in jobs:
+        parts = [oracle.bls_power(t, y, dy, p, d, settings.bins_per_duration,
+                                  objective=settings.objective, threads=threads)
+                 for p, d in segs]
+        results.append(parts)
+    elapsed = time.perf_counter() - t0
+    best, powers = [], []
+    from cuperiod_amd.core.result import Periodogram
+
+    for parts in results:
+        power = np.nan_to_num(np.concatenate([p["power"] for p in parts]), nan=0.0,
+                              posinf=0.0, neginf=0.0)
+        period = np.concatenate([p["period"] for p in parts])
+        pg = Periodogram.from_spectrum(method="BLS", backend="oracle", frequency=1.0 / period,
+                                       power=power, objective_sense="max", n_samples=1,
+                                       baseline=1.0)
+        best.append(pg.best_period())
+        powers.append(power)
+    return elapsed, threads, np.asarray(best), powers
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--curves", type=int, default=4096)
+    ap.add_argument("--freqs", type=int, default=200_000)
+    ap.add_argument("--cpu-curves", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the batch "
+              "shards by curve with no exchange; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from cuperiod_amd import _lib
+    from cuperiod_amd.batch import engine
+    from cuperiod_amd.core.config import BLSSettings
+    from cuperiod_amd.workloads import config3
+
+    log(f"building workload: {args.curves} curves x {args.freqs} frequencies")
+    wl = config3(args.curves, args.freqs)
+    settings = BLSSettings(n_durations=wl.n_durations)
+    ctx = _lib.context(0)
+    t0 = time.perf_counter()
+    pb = engine.pack(wl.curves, wl.grids, settings)
+    log(f"packed in {time.perf_counter() - t0:.1f} s: {pb.n_segments} segments, "
+        f"{pb.n_out} periods, {pb.total_points} points")
+    folds, boxes = count_work(pb.plans, [c[0].size for c in wl.curves], pb.oversample)
+    log(f"work per step: {folds:.3e} point-folds, {boxes:.3e} box evaluations")
+    device = torch.device("cuda", ctx.device)
+    out = torch.empty((_lib.N_OUTPUTS, pb.n_out), dtype=torch.float64, device=device)
+    summary = torch.empty((len(pb.plans), 4), dtype=torch.float64, device=device)
+    ctx.lib.cpb_set_timing(ctx.handle, 1)
+    for w in range(args.warmup):
+        engine.run_packed(pb, out=out, summary=summary)
+        torch.cuda.synchronize()
+        log(f"warm-up step {w + 1}/{args.warmup} done")
+    kernel_ms = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        engine.run_packed(pb, out=out, summary=summary)
+        import ctypes
+
+        ms = ctypes.c_double()
+        ctx.check(ctx.lib.cpb_main_kernel_ms(ctx.handle, ctypes.byref(ms)), "timing")
+        kernel_ms.append(ms.value)
+        log(f"step {s + 1}/{args.steps}: main kernel {ms.value:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    res = engine.finish(pb, out, summary)
+    ms_per_step = 1000.0 * elapsed / args.steps
+    curves_per_s = len(pb.plans) / (elapsed / args.steps)
+    k_ms = float(np.mean(kernel_ms))
+    flops = 4.0 * folds + 8.0 * boxes
+    achieved = flops / (k_ms * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": round(curves_per_s, 3),
+        "unit": "light curves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 phase, int64 fixed-point bins, f32 screen, f64 final",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain batched BLS, {len(pb.plans)} curves x {args.freqs} "
+                        "linspace frequencies in [1/baseline, 2] /d, 6 segment durations",
+            "curves": len(pb.plans),
+            "freqs_per_curve": args.freqs,
+            "point_folds_per_step": folds,
+            "box_evals_per_step": boxes,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(achieved / 1e12, 4),
+            "peak": PEAK_FP32 / 1e12,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32, 5),
+            "traffic": None,
+        },
+        "detail": {
+            "kernel": "bls_main_kernel",
+            "kernel_ms_mean": round(k_ms, 3),
+            "point_folds_per_s": folds / (k_ms * 1e-3),
+            "box_evals_per_s": boxes / (k_ms * 1e-3),
+            "lds_bytes_per_s": 16.0 * (folds + boxes) / (k_ms * 1e-3),
+            "lds_frac": 16.0 * (folds + boxes) / (k_ms * 1e-3) / PEAK_LDS,
+            "best_period_sample": [float(x) for x in res.best_period[:4]],
+        },
+        "learned": None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu and args.cpu_curves > 0:
+        n = min(args.cpu_curves, len(pb.plans))
+        log(f"CPU oracle on {n} curves ...")
+        el, threads, best_cpu, cpu_powers = cpu_baseline(wl, settings, n)
+        identical = best_cpu == res.best_period[:n]
+        # A different sample of a flat-topped peak is an exact tie: the oracle's power
+        # at the GPU's choice equals the oracle's maximum to 1e-12.
+        tie = np.array([abs(p[int(res.best_index[c])] - p.max()) <= 1e-12 * abs(p.max())
+                        for c, p in enumerate(cpu_powers)])
+        match = float(np.mean(identical | tie))
+        line["cpu_baseline"] = {
+            "value": round(n / el, 5),
+            "unit": "light curves/s",
+            "cores": int(threads),
+            "kind": "port",
+            "sample": f"first {n} curves of the same batch, same grids and settings "
+                      f"({el:.1f} s)",
+            "best_period_match": match,
+            "best_period_identical": float(np.mean(identical)),
+            "gpu_over_cpu": round(curves_per_s / (n / el), 1),
+        }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

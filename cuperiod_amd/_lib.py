@@ -40,7 +40,10 @@ EXPORTED_SYMBOLS = (
     "cpb_status_string",
     "cpb_bls_search",
     "cpb_bls_search_batch",
+    "cpb_bls_summarize",
     "cpb_linspace_grid",
+    "cpb_set_timing",
+    "cpb_main_kernel_ms",
     "cpb_synchronize",
 )
 
@@ -60,6 +63,7 @@ class Segment(ctypes.Structure):
         ("mode", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("lin_start", ctypes.c_double),
+        ("lin_step", ctypes.c_double),
         ("lin_stop", ctypes.c_double),
         ("lin_num", ctypes.c_int64),
         ("lin_k0", ctypes.c_int64),
@@ -107,8 +111,14 @@ def _declare(lib: ctypes.CDLL) -> None:
         vp, vp, vp, vp, vp, i64, ctypes.POINTER(Curve), i32, vp, vp,
         ctypes.POINTER(Segment), i32, i32, i32, vp, i64,
     ]
+    lib.cpb_bls_summarize.restype = ctypes.c_int
+    lib.cpb_bls_summarize.argtypes = [vp, vp, vp, vp, vp, i32, vp]
     lib.cpb_linspace_grid.restype = ctypes.c_int
     lib.cpb_linspace_grid.argtypes = [f64, f64, i64, ctypes.POINTER(f64)]
+    lib.cpb_set_timing.restype = ctypes.c_int
+    lib.cpb_set_timing.argtypes = [vp, ctypes.c_int]
+    lib.cpb_main_kernel_ms.restype = ctypes.c_int
+    lib.cpb_main_kernel_ms.argtypes = [vp, ctypes.POINTER(f64)]
     lib.cpb_synchronize.restype = ctypes.c_int
     lib.cpb_synchronize.argtypes = [vp, vp]
 
@@ -125,6 +135,12 @@ def load_library() -> ctypes.CDLL:
                 "`make -C cuperiod_amd/csrc` (hipcc, gfx950)"
             )
             raise BackendUnavailableError(_LOAD_ERROR)
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+        # Loading torch first makes this library bind to that one runtime, so torch's
+        # streams and allocations and this library's launches share it; loading this
+        # library first would hand torch a runtime it was not built against.
+        import torch  # noqa: F401
+
         try:
             lib = ctypes.CDLL(str(LIB_PATH))
         except OSError as exc:

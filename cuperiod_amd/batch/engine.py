@@ -1,0 +1,467 @@
+"""Batched BLS over many light curves in one launch sequence.
+
+cuPeriod searches a batch one light curve at a time, each with its own uploads, launches
+and a (7, F) download (src/cuperiod/batch/runner.py:214-250,
+src/cuperiod/methods/_bls_core.py:741-840). Here a chunk of curves is packed into one
+set of device arrays and searched by one work list of (curve, segment, period tile)
+items; each curve's spectrum stays on the device, where ``cpb_bls_summarize`` reduces
+it to the SDE statistics and the best period. Only full spectra that a caller asks for
+come back to the host.
+
+Grids are described parametrically so that no per-curve period array is built or sent:
+
+* :class:`LinspaceGrid` -- ``np.linspace(start, stop, num)`` frequencies (a custom
+  uniform frequency grid, split into log period segments as ``_log_segments`` does,
+  src/cuperiod/methods/bls.py:128-152);
+* ``None`` -- the curve's default BLS grid (``_segment_periods``, ``:71-105``): one
+  ``np.arange`` of frequencies per log segment;
+* a :class:`~cuperiod_amd.core.grid.GridSpec` -- explicit periods (sent as values).
+
+The device regenerates every parametric frequency with numpy's own recipe, so the
+searched periods are bit for bit those of the materialised grid.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Any, Sequence
+
+import numpy as np
+
+from cuperiod_amd import _lib
+from cuperiod_amd.core.config import BLSSettings
+from cuperiod_amd.core.errors import BackendUnavailableError, InsufficientDataError
+from cuperiod_amd.core.grid import GridSpec, check_grid_size, linspace_frequencies
+from cuperiod_amd.methods import _bls_core as core
+from cuperiod_amd.methods.bls import (
+    _grid_segments,
+    _log_segment_index,
+    _segment_bounds,
+    _segment_durations,
+)
+
+
+@dataclass(frozen=True)
+class LinspaceGrid:
+    """Frequencies ``np.linspace(start, stop, num)`` (cycles/day), ``0 < start < stop``."""
+
+    start: float
+    stop: float
+    num: int
+
+    def __post_init__(self) -> None:
+        if not (math.isfinite(self.start) and math.isfinite(self.stop)):
+            raise ValueError("LinspaceGrid bounds must be finite")
+        if not 0.0 < self.start < self.stop:
+            raise ValueError("LinspaceGrid needs 0 < start < stop (finite periods)")
+        if int(self.num) < 2:
+            raise ValueError("LinspaceGrid needs num >= 2")
+        check_grid_size(int(self.num), context="LinspaceGrid")
+
+    @property
+    def size(self) -> int:
+        return int(self.num)
+
+    def frequencies(self) -> np.ndarray:
+        return linspace_frequencies(self.start, self.stop, self.num)
+
+    def periods(self) -> np.ndarray:
+        """Ascending periods ``1/f[::-1]`` (what ``GridSpec.period`` gives)."""
+        return np.ascontiguousarray(1.0 / self.frequencies()[::-1])
+
+    def as_gridspec(self) -> GridSpec:
+        return GridSpec(kind="frequency", values=self.frequencies())
+
+    def period_at(self, k: np.ndarray | int) -> np.ndarray:
+        """Period of ascending-period index ``k`` by the device's recipe."""
+        k = np.asarray(k, dtype=np.int64)
+        i = self.num - 1 - k
+        step = (float(self.stop) - float(self.start)) / (self.num - 1)
+        f = np.where(i == self.num - 1, float(self.stop), i.astype(np.float64) * step + float(self.start))
+        return 1.0 / f
+
+
+@dataclass
+class _SegPlan:
+    """One segment of one curve: binning plus where its periods come from."""
+
+    curve: int
+    n_periods: int
+    durations: np.ndarray
+    p_max: float
+    mode: int
+    lin_start: float = 0.0
+    lin_step: float = 0.0
+    lin_stop: float = 0.0
+    lin_num: int = 0
+    lin_k0: int = 0
+    periods: np.ndarray | None = None
+
+
+@dataclass
+class CurvePlan:
+    """The segments of one curve and a way to name its periods."""
+
+    segments: list[_SegPlan] = field(default_factory=list)
+    n_periods: int = 0
+
+    def period_at(self, k: int) -> float:
+        """Period of ascending index ``k`` of this curve's grid."""
+        for sg in self.segments:
+            if k < sg.n_periods:
+                if sg.mode == 0:
+                    return float(sg.periods[k])
+                i = sg.lin_num - 1 - (sg.lin_k0 + k)
+                if sg.mode == 1 and i == sg.lin_num - 1:
+                    f = sg.lin_stop
+                elif sg.mode == 2 and i == 0:
+                    f = sg.lin_start
+                else:
+                    f = float(i) * sg.lin_step + sg.lin_start
+                return 1.0 / f
+            k -= sg.n_periods
+        raise IndexError("period index out of range")
+
+    def periods(self) -> np.ndarray:
+        """Every period of the curve's grid, ascending (materialised on request)."""
+        parts = []
+        for sg in self.segments:
+            if sg.mode == 0:
+                parts.append(sg.periods)
+                continue
+            k = np.arange(sg.n_periods, dtype=np.int64)
+            i = sg.lin_num - 1 - (sg.lin_k0 + k)
+            f = i.astype(np.float64) * sg.lin_step + sg.lin_start
+            if sg.mode == 1:
+                f = np.where(i == sg.lin_num - 1, sg.lin_stop, f)
+            else:
+                f = np.where(i == 0, sg.lin_start, f)
+            parts.append(1.0 / f)
+        return np.concatenate(parts) if parts else np.zeros(0)
+
+
+def _arange_params(start: float, stop: float, step: float) -> tuple[float, int]:
+    """``(step', n_kept)`` of ``np.arange(start, stop, step)`` restricted to values
+    ``< stop``: numpy fills ``start + i*step'`` with ``step' = (start+step) - start``."""
+    n = max(0, math.ceil((stop - start) / step))
+    if n == 0:
+        return step, 0
+    step2 = (start + step) - start
+    while n > 0:
+        last = start if n == 1 else start + float(n - 1) * step2
+        if last < stop:
+            break
+        n -= 1
+    return step2, n
+
+
+def _default_plan(curve: int, baseline: float, time: np.ndarray, settings: BLSSettings) -> CurvePlan:
+    """The default grid of a curve (``_segment_periods``) as arange segments."""
+    bounds = _segment_bounds(baseline, settings)
+    plan = CurvePlan()
+    if not bounds:
+        return plan
+    df = settings.grid_duration_frac / (settings.grid_oversample * baseline)
+    total = sum(max(0, math.ceil((1.0 / lo - 1.0 / hi) / df)) for lo, hi in bounds)
+    check_grid_size(total, automatic=True, time=time, baseline=baseline,
+                    context="BLS default grid")
+    for p_lo, p_hi in bounds:
+        start, stop = 1.0 / p_hi, 1.0 / p_lo
+        step2, n = _arange_params(start, stop, df)
+        if n == 0:
+            continue
+        plan.segments.append(
+            _SegPlan(curve=curve, n_periods=n, durations=_segment_durations(p_lo, settings),
+                     p_max=1.0 / start, mode=2, lin_start=start, lin_step=step2,
+                     lin_num=n, lin_k0=0)
+        )
+        plan.n_periods += n
+    return plan
+
+
+def _linspace_plan(curve: int, grid: LinspaceGrid, settings: BLSSettings) -> CurvePlan:
+    """A linspace frequency grid split into log period segments from its shortest
+    period, as ``_log_segments`` splits the materialised grid."""
+    num = int(grid.num)
+    p0 = float(grid.period_at(0))
+    factor = math.log(settings.segment_factor)
+
+    def seg_index(k: int) -> int:
+        p = float(grid.period_at(k))
+        return int(np.floor(np.log(np.float64(p) / np.float64(p0)) / factor + 1e-9))
+
+    plan = CurvePlan()
+    k0 = 0
+    last = seg_index(num - 1)
+    step = (float(grid.stop) - float(grid.start)) / (num - 1)
+    while k0 < num:
+        s = seg_index(k0)
+        lo, hi = k0, num - 1
+        if seg_index(hi) == s:
+            k1 = num
+        else:
+            while hi - lo > 1:
+                mid = (lo + hi) // 2
+                if seg_index(mid) == s:
+                    lo = mid
+                else:
+                    hi = mid
+            k1 = hi
+        plan.segments.append(
+            _SegPlan(curve=curve, n_periods=k1 - k0,
+                     durations=_segment_durations(p0 * settings.segment_factor**s, settings),
+                     p_max=float(grid.period_at(k1 - 1)), mode=1,
+                     lin_start=float(grid.start), lin_step=step, lin_stop=float(grid.stop),
+                     lin_num=num, lin_k0=k0)
+        )
+        plan.n_periods += k1 - k0
+        k0 = k1
+        if s == last:
+            break
+    return plan
+
+
+def _explicit_plan(curve: int, grid: GridSpec, settings: BLSSettings) -> CurvePlan:
+    plan = CurvePlan()
+    for periods, durations in _grid_segments(grid, settings):
+        if periods.size == 0:
+            continue
+        plan.segments.append(
+            _SegPlan(curve=curve, n_periods=int(periods.size), durations=durations,
+                     p_max=float(periods.max()), mode=0,
+                     periods=np.ascontiguousarray(periods, dtype=np.float64))
+        )
+        plan.n_periods += int(periods.size)
+    return plan
+
+
+@dataclass
+class BatchResult:
+    """Per-curve outcome of :func:`bls_batch` (arrays aligned with the input curves).
+
+    ``best_period``/``best_power`` follow ``Periodogram.best_period``; ``mean``/``std``
+    are the SDE statistics of the sanitised power; ``spectra[c]`` (when requested) holds
+    the curve's seven output rows in ascending-period order.
+    """
+
+    best_period: np.ndarray
+    best_power: np.ndarray
+    best_index: np.ndarray
+    mean: np.ndarray
+    std: np.ndarray
+    n_periods: np.ndarray
+    plans: list[CurvePlan]
+    spectra: list[dict[str, np.ndarray]] | None = None
+
+
+@dataclass
+class PackedBatch:
+    """Device-resident inputs and descriptors of one chunk of curves."""
+
+    tau: Any
+    yw: Any
+    ivar: Any
+    total_points: int
+    curves: Any
+    segments: Any
+    n_segments: int
+    periods: Any
+    dur_bins: Any
+    n_out: int
+    curve_base: np.ndarray
+    curve_count: np.ndarray
+    curve_base_d: Any
+    curve_count_d: Any
+    plans: list[CurvePlan]
+    oversample: int
+    obj_flag: int
+
+
+def pack(
+    lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]],
+    grids: Sequence[LinspaceGrid | GridSpec | None],
+    settings: BLSSettings,
+) -> PackedBatch:
+    """Host preparation of a chunk: median-centred flux, ``ivar``, time origins, the
+    floors, and every curve's segment plan; inputs uploaded once."""
+    import torch
+
+    ctx = _lib.context(0)
+    if not torch.cuda.is_available():
+        raise BackendUnavailableError("BLS batch: torch sees no HIP device")
+    device = torch.device("cuda", ctx.device)
+    oversample = int(settings.bins_per_duration)
+    obj_flag = core._objective_flag(settings.objective)
+    taus, yws, ivs = [], [], []
+    curves = (_lib.Curve * len(lcs))()
+    plans: list[CurvePlan] = []
+    off = 0
+    for c, ((t, y, dy), grid) in enumerate(zip(lcs, grids)):
+        t_h, y_h, iv_h = core._host_inputs(t, y, dy)
+        if t_h.size == 0:
+            raise InsufficientDataError("BLS batch: empty light curve")
+        t_min = float(t_h.min())
+        taus.append(t_h - t_min)
+        yws.append(y_h * iv_h)
+        ivs.append(iv_h)
+        cv = curves[c]
+        cv.point_offset = off
+        cv.n_points = int(t_h.size)
+        cv.t_min = t_min
+        cv.ivar_floor = core._ivar_floor(int(t_h.size), float(iv_h.sum()))
+        off += int(t_h.size)
+        if grid is None:
+            plan = _default_plan(c, float(t_h.max() - t_min), t_h, settings)
+        elif isinstance(grid, LinspaceGrid):
+            plan = _linspace_plan(c, grid, settings)
+        else:
+            plan = _explicit_plan(c, grid, settings)
+        plans.append(plan)
+    seg_list = [sg for plan in plans for sg in plan.segments]
+    segs = (_lib.Segment * max(1, len(seg_list)))()
+    dur_parts: list[np.ndarray] = []
+    per_parts: list[np.ndarray] = []
+    out_off = p_off = d_off = 0
+    curve_base = np.zeros(len(lcs), dtype=np.int64)
+    curve_count = np.zeros(len(lcs), dtype=np.int64)
+    j = 0
+    for c, plan in enumerate(plans):
+        curve_base[c] = out_off
+        curve_count[c] = plan.n_periods
+        for sg in plan.segments:
+            bd = float(sg.durations.min()) / oversample
+            db = np.asarray(core._duration_bins(sg.durations, bd), dtype=np.int32)
+            d = segs[j]
+            d.bin_duration = bd
+            d.n_periods = sg.n_periods
+            d.out_offset = out_off
+            d.width = int(np.ceil(sg.p_max / bd)) + oversample + 1
+            d.dur_offset = d_off
+            d.n_dur = int(db.size)
+            d.curve = c
+            d.mode = sg.mode
+            d.lin_start, d.lin_step, d.lin_stop = sg.lin_start, sg.lin_step, sg.lin_stop
+            d.lin_num, d.lin_k0 = sg.lin_num, sg.lin_k0
+            if sg.mode == 0:
+                d.period_offset = p_off
+                per_parts.append(sg.periods)
+                p_off += sg.n_periods
+            dur_parts.append(db)
+            d_off += int(db.size)
+            out_off += sg.n_periods
+            j += 1
+
+    def up(a: np.ndarray, dtype) -> Any:
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if a.size == 0:
+            a = np.zeros(1, dtype=dtype)
+        return torch.from_numpy(a).to(device)
+
+    return PackedBatch(
+        tau=up(np.concatenate(taus), np.float64),
+        yw=up(np.concatenate(yws), np.float64),
+        ivar=up(np.concatenate(ivs), np.float64),
+        total_points=off,
+        curves=curves,
+        segments=segs,
+        n_segments=len(seg_list),
+        periods=up(np.concatenate(per_parts) if per_parts else np.zeros(1), np.float64),
+        dur_bins=up(np.concatenate(dur_parts) if dur_parts else np.zeros(1), np.int32),
+        n_out=out_off,
+        curve_base=curve_base,
+        curve_count=curve_count,
+        curve_base_d=up(curve_base, np.int64),
+        curve_count_d=up(curve_count, np.int64),
+        plans=plans,
+        oversample=oversample,
+        obj_flag=obj_flag,
+    )
+
+
+def _vp(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def run_packed(pb: PackedBatch, *, out=None, summary=None):
+    """Enqueue the batched search and the per-curve summaries on torch's current
+    stream; returns ``(out, summary)`` device tensors (not synchronised)."""
+    import torch
+
+    ctx = _lib.context(0)
+    device = torch.device("cuda", ctx.device)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    if out is None:
+        out = torch.empty((_lib.N_OUTPUTS, max(1, pb.n_out)), dtype=torch.float64, device=device)
+    if summary is None:
+        summary = torch.empty((len(pb.plans), 4), dtype=torch.float64, device=device)
+    with ctx.lock:
+        st = ctx.lib.cpb_bls_search_batch(
+            ctx.handle, stream, _vp(pb.tau), _vp(pb.yw), _vp(pb.ivar), pb.total_points,
+            pb.curves, len(pb.plans), _vp(pb.periods), _vp(pb.dur_bins), pb.segments,
+            pb.n_segments, pb.oversample, pb.obj_flag, _vp(out), pb.n_out,
+        )
+        ctx.check(st, "batch search")
+        st = ctx.lib.cpb_bls_summarize(
+            ctx.handle, stream, _vp(out), _vp(pb.curve_base_d), _vp(pb.curve_count_d),
+            len(pb.plans), _vp(summary),
+        )
+        ctx.check(st, "summarize")
+    return out, summary
+
+
+def finish(pb: PackedBatch, out, summary, *, spectra: bool = False) -> BatchResult:
+    """Bring the per-curve summaries (and, on request, the spectra) to the host."""
+    s = summary.cpu().numpy()
+    best_index = s[:, 2].astype(np.int64)
+    best_period = np.array(
+        [pb.plans[c].period_at(int(best_index[c])) if pb.curve_count[c] else np.nan
+         for c in range(len(pb.plans))]
+    )
+    result = BatchResult(
+        best_period=best_period,
+        best_power=s[:, 3].copy(),
+        best_index=best_index,
+        mean=s[:, 0].copy(),
+        std=s[:, 1].copy(),
+        n_periods=pb.curve_count.copy(),
+        plans=pb.plans,
+    )
+    if spectra:
+        host = out.cpu().numpy()
+        result.spectra = []
+        for c in range(len(pb.plans)):
+            sl = slice(int(pb.curve_base[c]), int(pb.curve_base[c] + pb.curve_count[c]))
+            result.spectra.append(
+                {name: host[i, sl].copy() for i, name in enumerate(core._OUTPUT_NAMES)}
+            )
+    return result
+
+
+def bls_batch(
+    lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]],
+    grids: Sequence[LinspaceGrid | GridSpec | None] | LinspaceGrid | GridSpec | None = None,
+    settings: BLSSettings | None = None,
+    *,
+    spectra: bool = False,
+) -> BatchResult:
+    """BLS of many light curves ``(t, flux, dy)`` (flux domain) in one launch sequence.
+
+    ``grids`` is one grid for every curve or one per curve (``None`` = default grid).
+    """
+    settings = settings or BLSSettings()
+    if grids is None or isinstance(grids, (LinspaceGrid, GridSpec)):
+        grids = [grids] * len(lcs)
+    if len(grids) != len(lcs):
+        raise ValueError("grids must be one grid or one per light curve")
+    if not lcs:
+        empty = np.zeros(0)
+        return BatchResult(empty, empty, empty.astype(np.int64), empty, empty,
+                           empty.astype(np.int64), [], [] if spectra else None)
+    pb = pack(lcs, grids, settings)
+    out, summary = run_packed(pb)
+    return finish(pb, out, summary, spectra=spectra)
+
+
+__all__ = ["BatchResult", "LinspaceGrid", "bls_batch", "pack", "run_packed", "finish"]

@@ -54,8 +54,8 @@ struct CurveParams {
     int64_t point_offset;
     int64_t y_tot;    // sum of fixed-point y*ivar
     int64_t w_tot;    // sum of fixed-point ivar
-    double y_unit;    // float64 value of one fixed-point unit of y*ivar: 2^(ey-59)
-    double w_unit;    // same for ivar
+    int32_t y_exp;    // one fixed-point unit of y*ivar is 2^y_exp (ey - 59)
+    int32_t w_exp;    // same for ivar
     double t_min;
     double ivar_floor;
     float ybar;       // y_tot / w_tot
@@ -169,15 +169,14 @@ __device__ __forceinline__ double segment_period(const cpb_segment& sg,
                                                  const double* __restrict__ periods,
                                                  int64_t k) {
     if (sg.mode == 0) return periods[sg.period_offset + k];
-    const int64_t kk = sg.lin_k0 + k;
-    const int64_t i = sg.lin_num - 1 - kk;
+    const int64_t i = sg.lin_num - 1 - (sg.lin_k0 + k);
     double f;
-    if (i == sg.lin_num - 1) {
+    if (sg.mode == 1 && i == sg.lin_num - 1)
         f = sg.lin_stop;
-    } else {
-        const double step = (sg.lin_stop - sg.lin_start) / (double)(sg.lin_num - 1);
-        f = (double)i * step + sg.lin_start;
-    }
+    else if (sg.mode == 2 && i == 0)
+        f = sg.lin_start;
+    else
+        f = (double)i * sg.lin_step + sg.lin_start;
     return 1.0 / f;
 }
 
@@ -195,11 +194,11 @@ __device__ __forceinline__ bool exact_box(const int2* __restrict__ H,
     const uint2 l0 = L[n], l1 = L[n + d];
     const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
     const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
-    const double iv_in = (double)win_q * cv.w_unit;
-    const double iv_out = (double)(cv.w_tot - win_q) * cv.w_unit;
+    const double iv_in = ldexp((double)win_q, cv.w_exp);
+    const double iv_out = ldexp((double)(cv.w_tot - win_q), cv.w_exp);
     if (iv_in < cv.ivar_floor || iv_out < cv.ivar_floor) return false;
-    const double y_in = (double)yin_q * cv.y_unit;
-    const double y_rest = (double)(cv.y_tot - yin_q) * cv.y_unit;
+    const double y_in = ldexp((double)yin_q, cv.y_exp);
+    const double y_rest = ldexp((double)(cv.y_tot - yin_q), cv.y_exp);
     const double yin = y_in / iv_in;
     const double yout = y_rest / iv_out;
     if (yout < yin) return false;
@@ -272,15 +271,14 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
     int ey = 0, ew = 0;
     if (finite && sa > 0.0) (void)frexp(sa, &ey);
     if (finite && sw > 0.0) (void)frexp(sw, &ew);
-    const double scale_y = ldexp(1.0, kFixBits - ey);
-    const double scale_w = ldexp(1.0, kFixBits - ew);
 
     int64_t ty = 0, tw = 0;
     for (int64_t i = tid; i < n; i += kPrepThreads) {
         int64_t a = 0, b = 0;
         if (finite) {
-            a = llrint(yw[off + i] * scale_y);
-            b = llrint(wv[off + i] * scale_w);
+            // ldexp scales exactly, whatever the data's exponent range.
+            a = llrint(ldexp(yw[off + i], kFixBits - ey));
+            b = llrint(ldexp(wv[off + i], kFixBits - ew));
         }
         yq[off + i] = a;
         wq[off + i] = b;
@@ -303,8 +301,8 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
         p.point_offset = off;
         p.y_tot = ty;
         p.w_tot = tw;
-        p.y_unit = ldexp(1.0, ey - kFixBits);
-        p.w_unit = ldexp(1.0, ew - kFixBits);
+        p.y_exp = ey - kFixBits;
+        p.w_exp = ew - kFixBits;
         p.t_min = c.t_min;
         p.ivar_floor = c.ivar_floor;
         p.ybar = (float)ybar;
@@ -583,10 +581,10 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                     const uint2 l0 = L[n], l1 = L[n + d];
                     const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
                     const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
-                    const double iv_in = (double)win_q * cv.w_unit;
-                    const double iv_out = (double)(cv.w_tot - win_q) * cv.w_unit;
-                    const double y_in = (double)yin_q * cv.y_unit;
-                    const double y_rest = (double)(cv.y_tot - yin_q) * cv.y_unit;
+                    const double iv_in = ldexp((double)win_q, cv.w_exp);
+                    const double iv_out = ldexp((double)(cv.w_tot - win_q), cv.w_exp);
+                    const double y_in = ldexp((double)yin_q, cv.y_exp);
+                    const double y_rest = ldexp((double)(cv.y_tot - yin_q), cv.y_exp);
                     const double yin = y_in / iv_in;
                     const double yout = y_rest / iv_out;
                     const double depth = yout - yin;
@@ -606,6 +604,73 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
             if constexpr (WIDE) __threadfence();
             __syncthreads();
         }
+    }
+}
+
+// Per-curve summary of a search's power row, for batched runs that keep spectra on
+// the device: with p = power with NaN and +-inf set to 0 (`_assemble`,
+// src/cuperiod/methods/bls.py:280-282), summary = (mean(p), std(p) (two passes, ddof 0),
+// period-order index of the best period, its p). The best period is the first maximum
+// in ascending frequency, i.e. the last maximum in the ascending-period order the
+// engine writes; the first maximum of a spectrum is always a peak candidate of
+// `Periodogram.best_periods` (src/cuperiod/core/result.py:369-406).
+__device__ __forceinline__ double sanitize(double v) { return isfinite(v) ? v : 0.0; }
+
+__global__ __launch_bounds__(256) void curve_summary_kernel(
+    const double* __restrict__ power, const int64_t* __restrict__ base,
+    const int64_t* __restrict__ count, double* __restrict__ summary) {
+    __shared__ double red_d[4];
+    __shared__ int64_t red_i[4];
+    const int64_t b = base[blockIdx.x];
+    const int64_t n = count[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+    double s = 0.0;
+    for (int64_t i = tid; i < n; i += 256) s += sanitize(power[b + i]);
+    s = wave_sum_f64(s);
+    if (lane == 0) red_d[wave] = s;
+    __syncthreads();
+    const double mean = n > 0 ? (red_d[0] + red_d[1] + red_d[2] + red_d[3]) / (double)n : 0.0;
+    __syncthreads();
+    double q = 0.0, bv = -INFINITY;
+    int64_t bk = -1;
+    for (int64_t i = tid; i < n; i += 256) {
+        const double v = sanitize(power[b + i]);
+        q += (v - mean) * (v - mean);
+        if (v > bv || (v == bv && i > bk)) {
+            bv = v;
+            bk = i;
+        }
+    }
+    q = wave_sum_f64(q);
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(bv, off, kWave);
+        const int64_t ok = shfl_i64(bk, lane ^ off);
+        if (ov > bv || (ov == bv && ok > bk)) {
+            bv = ov;
+            bk = ok;
+        }
+    }
+    __shared__ double red_v[4];
+    if (lane == 0) {
+        red_d[wave] = q;
+        red_v[wave] = bv;
+        red_i[wave] = bk;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double var = n > 0 ? (red_d[0] + red_d[1] + red_d[2] + red_d[3]) / (double)n : 0.0;
+        double v = red_v[0];
+        int64_t k = red_i[0];
+        for (int w = 1; w < 4; ++w)
+            if (red_v[w] > v || (red_v[w] == v && red_i[w] > k)) {
+                v = red_v[w];
+                k = red_i[w];
+            }
+        summary[4 * blockIdx.x + 0] = mean;
+        summary[4 * blockIdx.x + 1] = sqrt(var);
+        summary[4 * blockIdx.x + 2] = (double)k;
+        summary[4 * blockIdx.x + 3] = n > 0 ? v : 0.0;
     }
 }
 
@@ -641,6 +706,10 @@ struct cpb_context {
     size_t h_tables_bytes = 0;
     hipEvent_t copied = nullptr;
     bool copy_pending = false;
+    // Optional timing of the main kernel (cpb_set_timing / cpb_main_kernel_ms).
+    bool timing = false;
+    bool timed = false;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     std::vector<WorkItem> h_items;
 };
 
@@ -762,9 +831,10 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
             return fail(ctx, CPB_E_INVALID, "bin_duration must be positive");
         if (sg.mode == 0 && sg.n_periods > 0 && !periods)
             return fail(ctx, CPB_E_INVALID, "explicit periods needed");
-        if (sg.mode == 1 && (sg.lin_num < 2 || sg.lin_k0 < 0 ||
+        if (sg.mode < 0 || sg.mode > 2) return fail(ctx, CPB_E_INVALID, "unknown segment mode");
+        if (sg.mode != 0 && (sg.lin_num < 1 || sg.lin_k0 < 0 ||
                              sg.lin_k0 + sg.n_periods > sg.lin_num))
-            return fail(ctx, CPB_E_INVALID, "bad linspace segment");
+            return fail(ctx, CPB_E_INVALID, "bad parametric segment");
         if (sg.n_dur > 0 && sg.n_periods > 0) {
             dmax = std::max(dmax, (int)sg.n_dur);
             wmax = std::max(wmax, (int)sg.width);
@@ -873,6 +943,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     const auto* cp = static_cast<const CurveParams*>(ctx->curves.ptr);
     const auto* yqp = static_cast<const int64_t*>(ctx->yq.ptr);
     const auto* wqp = static_cast<const int64_t*>(ctx->wq.ptr);
+    if (ctx->timing) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
     hipError_t e;
     if (obj_flag == 0) {
         e = wide ? dispatch_d<0, true>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
@@ -886,6 +957,10 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
                                         periods, dur_bins, oversample, wl, wide_ptr, out, n_out);
     }
     if (e != hipSuccess) return hip_fail(ctx, e, "bls_main_kernel launch");
+    if (ctx->timing) {
+        CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
+        ctx->timed = true;
+    }
     return CPB_OK;
 }
 
@@ -936,6 +1011,8 @@ void cpb_destroy(cpb_context* ctx) {
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->h_tables) (void)hipHostFree(ctx->h_tables);
     if (ctx->copied) (void)hipEventDestroy(ctx->copied);
+    if (ctx->t0) (void)hipEventDestroy(ctx->t0);
+    if (ctx->t1) (void)hipEventDestroy(ctx->t1);
     delete ctx;
 }
 
@@ -985,6 +1062,21 @@ int cpb_bls_search_batch(cpb_context* ctx, void* stream, const double* tau,
                       oversample, obj_flag, out, n_out);
 }
 
+int cpb_bls_summarize(cpb_context* ctx, void* stream, const double* power,
+                      const int64_t* curve_base, const int64_t* curve_count,
+                      int32_t n_curves, double* summary) {
+    if (!ctx) return CPB_E_INVALID;
+    if (n_curves < 0 || (n_curves > 0 && (!power || !curve_base || !curve_count || !summary)))
+        return fail(ctx, CPB_E_INVALID, "summarize: bad arguments");
+    if (n_curves == 0) return CPB_OK;
+    CPB_HIP(ctx, hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(curve_summary_kernel, dim3(n_curves), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), power, curve_base, curve_count,
+                       summary);
+    CPB_HIP(ctx, hipGetLastError());
+    return CPB_OK;
+}
+
 int cpb_linspace_grid(double start, double stop, int64_t num, double* out) {
     if (num < 0 || (num > 0 && !out)) return CPB_E_INVALID;
     if (num == 0) return CPB_OK;
@@ -995,6 +1087,29 @@ int cpb_linspace_grid(double start, double stop, int64_t num, double* out) {
     const double step = (stop - start) / (double)(num - 1);
     for (int64_t i = 0; i < num - 1; ++i) out[i] = (double)i * step + start;
     out[num - 1] = stop;
+    return CPB_OK;
+}
+
+int cpb_set_timing(cpb_context* ctx, int enable) {
+    if (!ctx) return CPB_E_INVALID;
+    CPB_HIP(ctx, hipSetDevice(ctx->device));
+    if (enable && !ctx->t0) {
+        CPB_HIP(ctx, hipEventCreate(&ctx->t0));
+        CPB_HIP(ctx, hipEventCreate(&ctx->t1));
+    }
+    ctx->timing = enable != 0;
+    ctx->timed = false;
+    return CPB_OK;
+}
+
+int cpb_main_kernel_ms(cpb_context* ctx, double* ms) {
+    if (!ctx || !ms) return CPB_E_INVALID;
+    if (!ctx->timed) return fail(ctx, CPB_E_INVALID, "no timed launch");
+    CPB_HIP(ctx, hipSetDevice(ctx->device));
+    CPB_HIP(ctx, hipEventSynchronize(ctx->t1));
+    float f = 0.0f;
+    CPB_HIP(ctx, hipEventElapsedTime(&f, ctx->t0, ctx->t1));
+    *ms = (double)f;
     return CPB_OK;
 }
 

@@ -1,0 +1,128 @@
+"""Seeded synthetic survey workloads used by the benchmark and the tests.
+
+Five configurations (the benchmark's `configs`), each generated with
+``np.random.default_rng(20261203 + k)`` for configuration ``k``:
+
+1. one curve, N = 2,000 over 100 d, box transit P = 3.7 d, depth 2e-3, duration 0.1 d,
+   phase 0.3; 50,000 frequencies linear in [0.02, 2] /d; durations {0.05, 0.1, 0.15,
+   0.2} d; oversample 10;
+2. one curve, N = 20,000 over 1,000 d with 30% removed in seasonal gaps,
+   sigma 2e-3, P = 11.3 d, depth 1.5e-3, duration 0.2 d; 1,000,000 frequencies in
+   [0.001, 5] /d; 8 durations log-spaced in [0.04, 0.4] d;
+3. 4,096 curves, N log-uniform in [100, 5,000], baselines uniform in [50, 2,000] d,
+   sigma log-uniform in [5e-4, 5e-3]; even-index curves carry a box transit (P
+   log-uniform in [0.5, 20] d, depth log-uniform in [5e-4, 1e-2], duration/period
+   uniform in [0.01, 0.05]); 200,000 frequencies in [1/baseline, 2] /d per curve;
+   6 period-scaled durations per log segment;
+4. like 3 with 16,384 curves (the learned-proposer configuration);
+5. 256 curves, N = 100,000 over 10,000 d offset by 2.45e6 d, P = 0.7 d, depth 1e-3,
+   duration 0.05 d; 4,000,000 frequencies in [1e-4, 10] /d; 16 durations.
+
+A curve is ``(t, flux, dy)`` with flux in the flux domain (a transit is a dip) and a
+constant ``dy`` per curve (uniform weights).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SEED = 20261203
+
+
+def _box_dip(t: np.ndarray, period: float, t0: float, duration: float) -> np.ndarray:
+    """Points inside a box of ``duration`` centred on ``t0 + k*period``."""
+    ph = np.mod((t - t0) / period + 0.5, 1.0) - 0.5
+    return np.abs(ph) < 0.5 * duration / period
+
+
+@dataclass
+class Workload:
+    """Light curves plus how to search them."""
+
+    name: str
+    curves: list
+    grids: list
+    n_durations: int | None = None
+    durations: np.ndarray | None = None
+    oversample: int = 10
+    truth: list = field(default_factory=list)
+
+
+def config1() -> Workload:
+    rng = np.random.default_rng(SEED + 1)
+    n = 2000
+    t = np.sort(rng.uniform(0.0, 100.0, n))
+    y = 1.0 + rng.normal(0.0, 1e-3, n)
+    y[_box_dip(t, 3.7, 0.3 * 3.7, 0.1)] -= 2e-3
+    from cuperiod_amd.batch.engine import LinspaceGrid
+
+    return Workload("config1", [(t, y, np.full(n, 1e-3))], [LinspaceGrid(0.02, 2.0, 50000)],
+                    durations=np.array([0.05, 0.1, 0.15, 0.2]), truth=[3.7])
+
+
+def config2() -> Workload:
+    rng = np.random.default_rng(SEED + 2)
+    t = np.sort(rng.uniform(0.0, 1000.0, 20000))
+    t = t[np.mod(t / 365.25, 1.0) >= 0.3]
+    n = t.size
+    y = 1.0 + rng.normal(0.0, 2e-3, n)
+    y[_box_dip(t, 11.3, 0.0, 0.2)] -= 1.5e-3
+    from cuperiod_amd.batch.engine import LinspaceGrid
+
+    return Workload("config2", [(t, y, np.full(n, 2e-3))], [LinspaceGrid(0.001, 5.0, 1_000_000)],
+                    durations=np.geomspace(0.04, 0.4, 8), truth=[11.3])
+
+
+def survey_curves(n_curves: int, seed: int, n_freq: int = 200_000):
+    """Configurations 3/4: uneven survey curves with per-curve linspace grids."""
+    from cuperiod_amd.batch.engine import LinspaceGrid
+
+    rng = np.random.default_rng(seed)
+    n_pts = np.round(np.exp(rng.uniform(np.log(100), np.log(5000), n_curves))).astype(int)
+    base = rng.uniform(50.0, 2000.0, n_curves)
+    sig = np.exp(rng.uniform(np.log(5e-4), np.log(5e-3), n_curves))
+    curves, grids, truth = [], [], []
+    for c in range(n_curves):
+        t = np.sort(rng.uniform(0.0, base[c], n_pts[c]))
+        y = 1.0 + rng.normal(0.0, sig[c], n_pts[c])
+        p = None
+        if c % 2 == 0:
+            p = float(np.exp(rng.uniform(np.log(0.5), np.log(20.0))))
+            depth = float(np.exp(rng.uniform(np.log(5e-4), np.log(1e-2))))
+            q = float(rng.uniform(0.01, 0.05))
+            t0 = float(rng.uniform(0.0, p))
+            y[_box_dip(t, p, t0, q * p)] -= depth
+        curves.append((t, y, np.full(n_pts[c], sig[c])))
+        grids.append(LinspaceGrid(1.0 / base[c], 2.0, n_freq))
+        truth.append(p)
+    return curves, grids, truth
+
+
+def config3(n_curves: int = 4096, n_freq: int = 200_000) -> Workload:
+    curves, grids, truth = survey_curves(n_curves, SEED + 3, n_freq)
+    return Workload("config3", curves, grids, n_durations=6, truth=truth)
+
+
+def config4(n_curves: int = 16384, n_freq: int = 200_000) -> Workload:
+    curves, grids, truth = survey_curves(n_curves, SEED + 4, n_freq)
+    return Workload("config4", curves, grids, n_durations=6, truth=truth)
+
+
+def config5(n_curves: int = 256, n_freq: int = 4_000_000, n_points: int = 100_000) -> Workload:
+    from cuperiod_amd.batch.engine import LinspaceGrid
+
+    rng = np.random.default_rng(SEED + 5)
+    curves, grids = [], []
+    for _ in range(n_curves):
+        t = np.sort(rng.uniform(0.0, 1e4, n_points)) + 2.45e6
+        y = 1.0 + rng.normal(0.0, 1e-3, n_points)
+        y[_box_dip(t, 0.7, 2.45e6 + 0.1, 0.05)] -= 1e-3
+        curves.append((t, y, np.full(n_points, 1e-3)))
+        grids.append(LinspaceGrid(1e-4, 10.0, n_freq))
+    return Workload("config5", curves, grids, n_durations=16, truth=[0.7] * n_curves)
+
+
+__all__ = ["SEED", "Workload", "config1", "config2", "config3", "config4", "config5",
+           "survey_curves"]

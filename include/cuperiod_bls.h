@@ -31,9 +31,14 @@ extern "C" {
 typedef struct cpb_context cpb_context;
 
 /* One duration-homogeneous period block of a light curve: the `_SearchSpec` of the
- * reference (src/cuperiod/methods/_bls_core.py:687-706). Periods are either explicit
- * (`periods[period_offset + k]`) or the reciprocals of an np.linspace frequency grid
- * taken in ascending-period order (mode 1; see cpb_linspace_grid). */
+ * reference (src/cuperiod/methods/_bls_core.py:687-706). Periods are explicit
+ * (mode 0: `periods[period_offset + k]`) or the reciprocals of a parametric frequency
+ * grid f_0..f_{num-1}, taken in ascending-period order, period k = 1 / f_{num-1-(k0+k)}:
+ *   mode 1, np.linspace(start, stop, num): f_i = i*step + start, f_{num-1} = stop,
+ *           step = (stop - start)/(num - 1) (see cpb_linspace_grid);
+ *   mode 2, np.arange(start, ., step'):   f_i = start + i*step', with step' =
+ *           (start + step) - start as numpy forms it (the BLS default-grid segments,
+ *           src/cuperiod/methods/bls.py:96-104). */
 typedef struct cpb_segment {
     double bin_duration;    /* min(durations) / oversample */
     int64_t period_offset;  /* mode 0: index of the first period in `periods` */
@@ -43,12 +48,13 @@ typedef struct cpb_segment {
     int32_t dur_offset;     /* first box width of this segment in `dur_bins` */
     int32_t n_dur;          /* box widths (bins) of this segment */
     int32_t curve;          /* light-curve index (0 for single-curve calls) */
-    int32_t mode;           /* 0: explicit periods; 1: reciprocal linspace */
+    int32_t mode;           /* 0: explicit; 1: linspace; 2: arange (frequencies) */
     int32_t reserved;
-    double lin_start;       /* mode 1: np.linspace(start, stop, num) frequencies */
-    double lin_stop;
-    int64_t lin_num;
-    int64_t lin_k0;         /* mode 1: ascending-period index of the first period */
+    double lin_start;       /* modes 1-2: first frequency */
+    double lin_step;        /* modes 1-2: frequency step as numpy applies it */
+    double lin_stop;        /* mode 1: last frequency (np.linspace sets it exactly) */
+    int64_t lin_num;        /* modes 1-2: frequencies in the grid */
+    int64_t lin_k0;         /* modes 1-2: ascending-period index of the first period */
 } cpb_segment;
 
 /* Per light curve inputs of a batched search. */
@@ -114,9 +120,30 @@ int cpb_bls_search_batch(cpb_context* ctx, void* stream, const double* tau,
                          int32_t oversample, int32_t obj_flag, double* out,
                          int64_t n_out);
 
+/*
+ * Per-curve summary of power rows left on the device by a batched search (the SDE
+ * statistics and best period of `_assemble` / `Periodogram.best_period`,
+ * src/cuperiod/methods/bls.py:274-301, src/cuperiod/core/result.py:285-438):
+ * for curve c, the powers power[curve_base[c] .. + curve_count[c]) in ascending-period
+ * order, with NaN/+-inf read as 0, give
+ *   summary[4c+0] = mean, summary[4c+1] = std (ddof 0),
+ *   summary[4c+2] = index (within the curve) of the best period, summary[4c+3] = its power.
+ * curve_base, curve_count: DEVICE int64[n_curves]; summary: DEVICE float64[4*n_curves].
+ */
+int cpb_bls_summarize(cpb_context* ctx, void* stream, const double* power,
+                      const int64_t* curve_base, const int64_t* curve_count,
+                      int32_t n_curves, double* summary);
+
 /* Frequencies of np.linspace(start, stop, num) by numpy's recipe (value_i = i*step +
  * start, last = stop), computed on the HOST: the values mode-1 segments use. */
 int cpb_linspace_grid(double start, double stop, int64_t num, double* out);
+
+/* Enable (1) or disable (0) HIP-event timing of the main search kernel of later
+ * searches on this context (the events are recorded on the search's stream). */
+int cpb_set_timing(cpb_context* ctx, int enable);
+
+/* Milliseconds of the main search kernel of the last timed search (waits for it). */
+int cpb_main_kernel_ms(cpb_context* ctx, double* ms);
 
 /* Block until all work queued on `stream` has finished. */
 int cpb_synchronize(cpb_context* ctx, void* stream);

@@ -5,10 +5,12 @@ The bar (stated in every test that uses it):
   north-star bound) of the FP64 oracle -- observed ~1e-13, since the engine's bins are
   exact and the oracle's carry FP64 rounding;
 * the chosen box (duration, transit time) identical at >= ``box_frac`` of the periods
-  (default 0.995, the reference's own GPU bar in tests/test_bls_segments.py:110-131):
-  boxes whose FP64 objectives tie to the last bits can be ordered differently by the
-  oracle's rounding;
-* the best period (argmax of power) identical, or an exact tie of the oracle's power.
+  (default 0.995, the reference's own GPU bar in tests/test_bls_segments.py:110-131),
+  and wherever it differs the two boxes tie (power equal to 1e-12): boxes holding the
+  same points (shifted across empty bins, or a bin and its wrap copy) are exact ties
+  that the oracle's FP64 prefix sums order by rounding;
+* the best period (argmax of power) identical, or an exact tie of the oracle's power;
+* powers below 1e-290 (underflowed objectives) are compared in value only.
 """
 
 from __future__ import annotations
@@ -47,12 +49,23 @@ def compare(got: dict, ref: dict, *, rtol: float = 1e-6, box_frac: float = 0.995
     same_box = (np.asarray(got["duration"]) == np.asarray(ref["duration"])) & np.isclose(
         np.asarray(got["transit_time"]), np.asarray(ref["transit_time"]), rtol=0.0, atol=1e-9
     )
-    frac = float(same_box.mean()) if same_box.size else 1.0
+    # Powers that underflowed (e.g. a log-likelihood of flux scaled by 1e-200) tie at
+    # zero or subnormal values, where either order is right; they are left out of the
+    # box-identity fraction (their power must still agree).
+    meaningful = np.isfinite(rp) & (np.abs(rp) >= 1e-290)
+    frac = float(same_box[meaningful].mean()) if meaningful.any() else 1.0
     assert frac >= box_frac, f"{name}: same box at only {frac:.4f} of periods"
+    # Wherever the boxes differ, the two boxes must tie: equal power to 1e-12.
+    diff = ~same_box & meaningful
+    if diff.any():
+        tie = rel_err(gp[diff], rp[diff])
+        assert tie.max() <= 1e-12, f"{name}: a differing box is not a tie ({tie.max():.3g})"
     if gp.size and np.isfinite(rp).any():
         gi = int(np.argmax(np.where(np.isfinite(gp), gp, -np.inf)))
         ri = int(np.argmax(np.where(np.isfinite(rp), rp, -np.inf)))
-        assert gi == ri or rp[gi] == rp[ri], f"{name}: best period index {gi} vs {ri}"
+        assert gi == ri or rp[gi] >= rp[ri] * (1 - 1e-12), (
+            f"{name}: best period index {gi} vs {ri}"
+        )
     errs = {"power": float(perr.max()) if perr.size else 0.0, "same_box": frac}
     for k in ("depth", "depth_snr", "log_likelihood"):
         e = rel_err(got[k], ref[k])[same_box]

@@ -1,0 +1,115 @@
+"""Batched engine vs single-curve engine and oracle (GPU).
+
+The engine's bins are exact (64-bit fixed point), so a curve searched inside a batch
+must give bit-identical outputs to the same curve searched alone on the same periods;
+the device-regenerated parametric grids must be numpy's grids bit for bit.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cuperiod_amd as cup
+import oracle
+from cuperiod_amd.batch.engine import LinspaceGrid, bls_batch
+from cuperiod_amd.methods import _bls_core as core
+from cuperiod_amd.methods.bls import BLSMethod, _grid_segments
+from parity import KEYS, compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _curves(n_curves, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(n_curves):
+        n = int(np.round(np.exp(rng.uniform(np.log(100), np.log(2000)))))
+        base = rng.uniform(50, 400)
+        t = np.sort(rng.uniform(0, base, n)) + 2458000.0
+        sig = np.exp(rng.uniform(np.log(5e-4), np.log(5e-3)))
+        y = 1 + sig * rng.standard_normal(n)
+        if c % 2 == 0:
+            P = np.exp(rng.uniform(np.log(0.5), np.log(20)))
+            q = rng.uniform(0.01, 0.05)
+            y[np.mod(t - rng.uniform(0, P), P) < q * P] -= np.exp(rng.uniform(np.log(5e-4), np.log(1e-2)))
+        out.append((t, y, np.full(n, sig)))
+    return out
+
+
+def test_linspace_batch_equals_single_curve_search():
+    print("case batch linspace", flush=True)
+    settings = cup.BLSSettings(n_durations=6)
+    lcs = _curves(6, seed=1)
+    grids = [LinspaceGrid(1.0 / (t.max() - t.min()), 2.0, 3000) for t, _, _ in lcs]
+    res = bls_batch(lcs, grids, settings, spectra=True)
+    for c, ((t, y, dy), g) in enumerate(zip(lcs, grids)):
+        gs = g.as_gridspec()
+        np.testing.assert_array_equal(res.plans[c].periods(), gs.period)
+        segs = _grid_segments(gs, settings)
+        assert [s.n_periods for s in res.plans[c].segments] == [p.size for p, _ in segs]
+        single = core.bls_power_segments(t, y, dy, segs, settings.bins_per_duration)
+        for i, k in enumerate(KEYS):
+            ref = np.concatenate([getattr(s, k) for s in single])
+            np.testing.assert_array_equal(res.spectra[c][k], ref, err_msg=f"{c}/{k}")
+        # Against the oracle on the same periods.
+        ref = {k: np.concatenate([oracle.bls_power(t, y, dy, p, d, 10)[k] for p, d in segs]) for k in KEYS}
+        compare(res.spectra[c], ref, name=f"curve {c}")
+        # Summary = the host assembly's SDE statistics and best period.
+        pg = BLSMethod().power(gs, cup.LightCurve.from_arrays(t, y, dy, domain="flux"), settings, "hip")
+        p = pg.power
+        assert res.mean[c] == pytest.approx(float(p.mean()), rel=1e-12, abs=1e-300)
+        assert res.std[c] == pytest.approx(float(p.std()), rel=1e-10)
+        assert res.best_period[c] == pg.best_period()
+
+
+def test_default_grid_batch_equals_single_curve_search():
+    print("case batch default grid", flush=True)
+    settings = cup.BLSSettings()
+    lcs = _curves(4, seed=2)
+    res = bls_batch(lcs, None, settings, spectra=True)
+    for c, (t, y, dy) in enumerate(lcs):
+        lc = cup.LightCurve.from_arrays(t, y, dy, domain="flux")
+        grid = BLSMethod().default_grid(lc, settings)
+        np.testing.assert_array_equal(res.plans[c].periods(), grid.period)
+        pg = BLSMethod().power(grid, lc, settings, "hip")
+        assert res.best_period[c] == pg.best_period()
+        power = np.nan_to_num(res.spectra[c]["power"], neginf=0.0)
+        np.testing.assert_array_equal(power[::-1], pg.power)
+
+
+def test_survey_best_periods_match_oracle_up_to_ties():
+    # The 200k-frequency survey batch: on each curve the GPU's best period must be the
+    # oracle's, or an exact tie of it -- the oracle's own power at the GPU's choice
+    # equals the oracle's maximum to 1e-12 (flat-topped peaks: adjacent frequencies
+    # that fold the best box's points identically; the oracle's FP64 rounding picks
+    # any sample of the plateau, exact sums pick its first).
+    from cuperiod_amd.core.result import Periodogram
+    from cuperiod_amd.workloads import config3
+
+    print("case survey best periods", flush=True)
+    wl = config3(8, 200_000)
+    settings = cup.BLSSettings(n_durations=wl.n_durations)
+    res = bls_batch(wl.curves, wl.grids, settings, spectra=True)
+    exact = 0
+    for c, ((t, y, dy), g) in enumerate(zip(wl.curves, wl.grids)):
+        segs = _grid_segments(g.as_gridspec(), settings)
+        parts = [oracle.bls_power(t, y, dy, p, d, 10) for p, d in segs]
+        ref = {k: np.concatenate([p[k] for p in parts]) for k in KEYS}
+        errs = compare(res.spectra[c], ref, name=f"survey curve {c}")
+        period = np.concatenate([p["period"] for p in parts])
+        rpow = np.nan_to_num(ref["power"], neginf=0.0)
+        pg = Periodogram.from_spectrum(method="BLS", backend="oracle", frequency=1 / period,
+                                       power=rpow, objective_sense="max", n_samples=1,
+                                       baseline=1.0)
+        ref_best = pg.best_period()
+        gi = int(res.best_index[c])
+        top = float(rpow.max())
+        print(f"  curve {c}: gpu {res.best_period[c]!r} oracle {ref_best!r} "
+              f"oracle power at gpu choice {rpow[gi]!r} max {top!r} errs {errs}", flush=True)
+        assert res.best_period[c] == period[gi]
+        if res.best_period[c] == ref_best:
+            exact += 1
+        else:
+            assert abs(rpow[gi] - top) <= 1e-12 * abs(top)
+    print(f"  identical best period on {exact}/8 curves", flush=True)

@@ -58,9 +58,11 @@ def test_adversarial_cases_match_oracle(obj_flag):
         with np.errstate(all="ignore"):
             ref = oracle.bls_power(t, y, dy, periods, durations, 3, objective=objective)
             got = _hip_power(t, y, dy, periods, durations, 3, objective)
-        # Exact-tie data (constant / quantised flux) breaks ties by FP64 rounding in
-        # the oracle; the engine's sums are exact. Power still agrees.
-        box_frac = 0.0 if name in ("quantised ties", "constant") else 0.995
+        # Exact-tie data (constant / quantised flux, five points in 800 bins) has
+        # boxes holding identical point sets at most periods; the oracle orders them
+        # by FP64 rounding, the engine's exact sums by first-wins. compare() still
+        # requires every differing box to tie in power.
+        box_frac = 0.0 if name in ("quantised ties", "constant", "five points") else 0.995
         errs = compare(got, ref, name=name, box_frac=box_frac)
         _say(f"  errors {errs}")
 

@@ -68,7 +68,7 @@ def test_library_host_entry_points():
 
 
 def test_segment_struct_layout_matches_header():
-    assert ctypes.sizeof(_lib.Segment) == 88
+    assert ctypes.sizeof(_lib.Segment) == 96
     assert _lib.Segment.lin_start.offset == 56
     assert ctypes.sizeof(_lib.Curve) == 32
 
