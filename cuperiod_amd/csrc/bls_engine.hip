@@ -184,32 +184,6 @@ __device__ __forceinline__ int64_t join64(int32_t hi, uint32_t lo) {
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)lo);
 }
 
-// Exact FP64 statistics of box (n, d) from the 64-bit prefix sums; returns false
-// for an inadmissible box (floor or not a dip), as the reference's sweep skips it.
-template <int OBJ>
-__device__ __forceinline__ bool exact_box(const int2* __restrict__ H,
-                                          const uint2* __restrict__ L, int n, int d,
-                                          const CurveParams& cv, double& obj) {
-    const int2 h0 = H[n], h1 = H[n + d];
-    const uint2 l0 = L[n], l1 = L[n + d];
-    const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
-    const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
-    const double iv_in = ldexp((double)win_q, cv.w_exp);
-    const double iv_out = ldexp((double)(cv.w_tot - win_q), cv.w_exp);
-    if (iv_in < cv.ivar_floor || iv_out < cv.ivar_floor) return false;
-    const double y_in = ldexp((double)yin_q, cv.y_exp);
-    const double y_rest = ldexp((double)(cv.y_tot - yin_q), cv.y_exp);
-    const double yin = y_in / iv_in;
-    const double yout = y_rest / iv_out;
-    if (yout < yin) return false;
-    const double depth = yout - yin;
-    if (OBJ == 0)
-        obj = depth / sqrt(1.0 / iv_in + 1.0 / iv_out);
-    else
-        obj = 0.5 * iv_in * depth * depth;
-    return true;
-}
-
 // ---------------------------------------------------------------------------------
 // Per-curve preparation: fixed-point conversion and the screen's bounds.
 
@@ -333,8 +307,148 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
 
 // ---------------------------------------------------------------------------------
 // Main kernel.
+//
+// LDS (or, in MODE 2, global scratch) of one workgroup, in 16-byte slots of int64
+// pairs: during the fold, slot_of(i) = i + i/8 holds the fixed-point sums (y, w) of
+// phase bin i; the pad slot after every 8 bins puts the 8 bins a thread scans on
+// distinct banks. After the prefix:
+//   MODE 0 (span <= T*kChunk, LDS): dense arrays H[i] = (cy_hi, cw_hi) from byte 0 and
+//          L[i] = (cy_lo, cw_lo) from byte 8*wl, written over the slots;
+//   MODE 1 (LDS) and MODE 2 (global scratch): in place, slot_of(i) = (cy_hi, cw_hi,
+//          cy_lo, cw_lo).
+// cy/cw are the inclusive prefix sums; hi/lo their 32-bit words.
 
-template <int T, int DMAX, int OBJ, bool WIDE>
+constexpr int kChunk = 8;
+
+__device__ __forceinline__ int slot_of(int i) { return i + (i >> 3); }
+
+// Cross-lane moves within a wave (GFX9 DPP): row_shr:n shifts within 16-lane rows,
+// row_bcast:15/31 carry a row's last lane into the next row(s); lanes without a
+// source read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, true);
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int64_t dpp_i64(int64_t v) {
+    const uint32_t lo = dpp_u32<CTRL, ROW_MASK>((uint32_t)(uint64_t)v);
+    const uint32_t hi = dpp_u32<CTRL, ROW_MASK>((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active).
+__device__ __forceinline__ int64_t wave_incl_scan_dpp(int64_t v) {
+    v += dpp_i64<0x111, 0xF>(v);
+    v += dpp_i64<0x112, 0xF>(v);
+    v += dpp_i64<0x114, 0xF>(v);
+    v += dpp_i64<0x118, 0xF>(v);
+    v += dpp_i64<0x142, 0xA>(v);
+    v += dpp_i64<0x143, 0xC>(v);
+    return v;
+}
+
+// Maximum over the wave of non-negative int32 values, returned uniformly.
+__device__ __forceinline__ int wave_max_nonneg_dpp(int v) {
+    v = max(v, (int)dpp_u32<0x111, 0xF>((uint32_t)v));
+    v = max(v, (int)dpp_u32<0x112, 0xF>((uint32_t)v));
+    v = max(v, (int)dpp_u32<0x114, 0xF>((uint32_t)v));
+    v = max(v, (int)dpp_u32<0x118, 0xF>((uint32_t)v));
+    v = max(v, (int)dpp_u32<0x142, 0xA>((uint32_t)v));
+    v = max(v, (int)dpp_u32<0x143, 0xC>((uint32_t)v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// First-wins best (objective, key) of a wave, uniform: the lanes holding a candidate
+// are few, so they are visited one by one.
+__device__ __forceinline__ void wave_argbest_ballot(double& obj, uint32_t& key) {
+    unsigned long long m = __ballot(obj > -INFINITY);
+    double wb = -INFINITY;
+    uint32_t wk = 0xFFFFFFFFu;
+    while (m) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const double v = readlane_f64(obj, l);
+        const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key, l);
+        if (better(v, k, wb, wk)) {
+            wb = v;
+            wk = k;
+        }
+    }
+    obj = wb;
+    key = wk;
+}
+
+template <int MODE>
+struct BinView {
+    // MODE 0: dense H/L; MODE 1/2: slots.
+    const char* base;
+    int wl;
+    __device__ __forceinline__ int2 hi(int n) const {
+        if constexpr (MODE == 0)
+            return reinterpret_cast<const int2*>(base)[n];
+        else
+            return *reinterpret_cast<const int2*>(base + 16 * slot_of(n));
+    }
+    __device__ __forceinline__ uint2 lo(int n) const {
+        if constexpr (MODE == 0)
+            return reinterpret_cast<const uint2*>(base + 8 * (size_t)wl)[n];
+        else
+            return *reinterpret_cast<const uint2*>(base + 16 * slot_of(n) + 8);
+    }
+};
+
+// Exact FP64 statistics of box (n, d) from the 64-bit prefix sums; false for an
+// inadmissible box (floor, or not a dip), as the reference's sweep skips it.
+template <int OBJ, int MODE>
+__device__ __forceinline__ bool exact_box(const BinView<MODE>& bv, int n, int d,
+                                          const CurveParams& cv, double& obj) {
+    const int2 h0 = bv.hi(n), h1 = bv.hi(n + d);
+    const uint2 l0 = bv.lo(n), l1 = bv.lo(n + d);
+    const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
+    const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
+    const double iv_in = ldexp((double)win_q, cv.w_exp);
+    const double iv_out = ldexp((double)(cv.w_tot - win_q), cv.w_exp);
+    if (iv_in < cv.ivar_floor || iv_out < cv.ivar_floor) return false;
+    const double y_in = ldexp((double)yin_q, cv.y_exp);
+    const double y_rest = ldexp((double)(cv.y_tot - yin_q), cv.y_exp);
+    const double yin = y_in / iv_in;
+    const double yout = y_rest / iv_out;
+    if (yout < yin) return false;
+    const double depth = yout - yin;
+    if (OBJ == 0)
+        obj = depth / sqrt(1.0 / iv_in + 1.0 / iv_out);
+    else
+        obj = 0.5 * iv_in * depth * depth;
+    return true;
+}
+
+// FP32 screen value of box (n, d): -t|t| / den with t = Y - ybar*W (a dip has t <= 0),
+// den = W*Wout (snr) or W*Wout^2 (likelihood), from the prefix high words; `cert` is
+// false when either side holds at most `e_unc` high-word units of weight.
+template <int OBJ>
+__device__ __forceinline__ float screen_value(int2 h0, int2 h1, int s_hi, int e_unc,
+                                              float ybar, bool& cert, bool& unc) {
+    const int dw = h1.y - h0.y;
+    const int dwo = s_hi - dw;
+    const float Y = (float)(h1.x - h0.x);
+    const float W = (float)dw;
+    const float Wo = (float)dwo;
+    const float t = fmaf(-ybar, W, Y);
+    const float den = OBJ == 0 ? W * Wo : W * Wo * Wo;
+    cert = min(dw, dwo) > e_unc;
+    unc = OBJ == 1 && (dw > e_unc) & (dwo <= e_unc) & (dwo > -2);
+    return (-t) * fabsf(t) * __builtin_amdgcn_rcpf(den);
+}
+
+template <int T, int DMAX, int OBJ, int MODE>
 __global__ __launch_bounds__(T) void bls_main_kernel(
     const double* __restrict__ tau, const int64_t* __restrict__ yq,
     const int64_t* __restrict__ wq, const CurveParams* __restrict__ curves,
@@ -344,9 +458,10 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
     int64_t* __restrict__ wide_scratch, double* __restrict__ out, int64_t n_out) {
     extern __shared__ int64_t dyn_lds[];
     constexpr int NW = T / kWave;
+    constexpr bool GLOBAL = MODE == 2;
     __shared__ int32_t s_dur[DMAX];
     __shared__ int64_t s_red_i[2 * NW];
-    __shared__ float s_red_f[NW];
+    __shared__ int s_red_v[NW];
     __shared__ double s_red_d[NW];
     __shared__ uint32_t s_red_k[NW];
 
@@ -354,17 +469,13 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
     const int lane = tid % kWave;
     const int wave = tid / kWave;
 
-    int64_t* HY;
-    int64_t* HW;
-    if constexpr (WIDE) {
-        HY = wide_scratch + (int64_t)blockIdx.x * 2 * (int64_t)wl;
-        HW = HY + wl;
-    } else {
-        HY = dyn_lds;
-        HW = dyn_lds + wl;
-    }
-    const int2* H = reinterpret_cast<const int2*>(HY);
-    const uint2* L = reinterpret_cast<const uint2*>(HW);
+    const int n_slots = slot_of(wl - 1) + 1;
+    int64_t* S;  // slots: S[2*s] = y, S[2*s + 1] = w
+    if constexpr (GLOBAL)
+        S = wide_scratch + (int64_t)blockIdx.x * 2 * (int64_t)n_slots;
+    else
+        S = dyn_lds;
+    const BinView<MODE> bv{reinterpret_cast<const char*>(S), wl};
 
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const WorkItem it = items[item];
@@ -397,128 +508,155 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                 continue;
             }
 
-            // (a) zero bins [0, n_bins], then fold every point into them.
-            for (int i = tid; i <= n_bins; i += T) {
-                HY[i] = 0;
-                HW[i] = 0;
+            // (a) zero the slots of bins [0, n_bins], then fold every point.
+            {
+                longlong2* S2 = reinterpret_cast<longlong2*>(S);
+                const int last = slot_of(n_bins);
+                for (int i = tid; i <= last; i += T) S2[i] = make_longlong2(0, 0);
             }
-            if constexpr (WIDE) __threadfence();
+            if constexpr (GLOBAL) __threadfence();
             __syncthreads();
+#pragma unroll 2
             for (int j = tid; j < npts; j += T) {
                 const double x = tau[poff + j];
-                const int ind = fold_bin(x, P, invP, bd, inv_bd, width);
-                atomicAdd(reinterpret_cast<unsigned long long*>(HY + ind),
-                          (unsigned long long)yq[poff + j]);
-                atomicAdd(reinterpret_cast<unsigned long long*>(HW + ind),
-                          (unsigned long long)wq[poff + j]);
+                const int64_t a = yq[poff + j];
+                const int64_t b = wq[poff + j];
+                const int s = slot_of(fold_bin(x, P, invP, bd, inv_bd, width));
+                atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s), (unsigned long long)a);
+                atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s + 1),
+                          (unsigned long long)b);
             }
-            if constexpr (WIDE) __threadfence();
+            if constexpr (GLOBAL) __threadfence();
             __syncthreads();
             // (a') wrap copy, in order (a copy may read a bin an earlier one wrote).
             if (tid == 0) {
+                longlong2* S2 = reinterpret_cast<longlong2*>(S);
                 for (int kk = 0; kk < oversample; ++kk) {
                     const int dst = n_bins - oversample + kk;
-                    if (dst >= 0 && dst < width) {
-                        HY[dst] = HY[1 + kk];
-                        HW[dst] = HW[1 + kk];
-                    }
+                    if (dst >= 0 && dst < width) S2[slot_of(dst)] = S2[slot_of(1 + kk)];
                 }
             }
-            if constexpr (WIDE) __threadfence();
+            if constexpr (GLOBAL) __threadfence();
             __syncthreads();
 
-            // (b1) inclusive prefix over [0, n_bins]: each wave scans a contiguous
-            // quarter with cross-lane shuffles, then adds the earlier waves' totals.
+            // (b1) inclusive prefix over [0, n_bins]: every thread sums a contiguous
+            // chunk, one DPP scan per wave and one exchange between waves give each
+            // chunk its offset.
             {
                 const int span = n_bins + 1;
-                const int per = (span + NW - 1) / NW;
-                const int lo = wave * per;
-                const int hi = min(span, lo + per);
-                int64_t cy = 0, cw = 0;
-                for (int base = lo; base < hi; base += kWave) {
-                    const int i = base + lane;
-                    int64_t vy = 0, vw = 0;
-                    if (i < hi) {
-                        vy = HY[i];
-                        vw = HW[i];
+                const int c = (span + T - 1) / T;
+                const int base = tid * c;
+                const longlong2* S2 = reinterpret_cast<const longlong2*>(S);
+                int64_t sy = 0, sw = 0;
+                int64_t ry[kChunk], rw[kChunk];
+                if (MODE == 0) {
+#pragma unroll
+                    for (int kk = 0; kk < kChunk; ++kk) {
+                        const int i = base + kk;
+                        if (kk < c && i < span) {
+                            const longlong2 v = S2[slot_of(i)];
+                            sy += v.x;
+                            sw += v.y;
+                        }
+                        ry[kk] = sy;
+                        rw[kk] = sw;
                     }
-                    vy = wave_scan_incl_i64(vy, lane) + cy;
-                    vw = wave_scan_incl_i64(vw, lane) + cw;
-                    if (i < hi) {
-                        HY[i] = vy;
-                        HW[i] = vw;
+                } else {
+                    for (int kk = 0; kk < c; ++kk) {
+                        const int i = base + kk;
+                        if (i < span) {
+                            const longlong2 v = S2[slot_of(i)];
+                            sy += v.x;
+                            sw += v.y;
+                        }
                     }
-                    cy = shfl_i64(vy, kWave - 1);
-                    cw = shfl_i64(vw, kWave - 1);
                 }
-                if (lane == 0) {
-                    s_red_i[2 * wave] = cy;
-                    s_red_i[2 * wave + 1] = cw;
+                const int64_t iy = wave_incl_scan_dpp(sy);
+                const int64_t iw = wave_incl_scan_dpp(sw);
+                if (lane == kWave - 1) {
+                    s_red_i[2 * wave] = iy;
+                    s_red_i[2 * wave + 1] = iw;
                 }
-                if constexpr (WIDE) __threadfence();
+                if constexpr (GLOBAL) __threadfence();
                 __syncthreads();
-                int64_t oy = 0, ow = 0;
+                int64_t oy = iy - sy, ow = iw - sw;
                 for (int w = 0; w < wave; ++w) {
                     oy += s_red_i[2 * w];
                     ow += s_red_i[2 * w + 1];
                 }
-                int2* Hw = reinterpret_cast<int2*>(HY);
-                uint2* Lw = reinterpret_cast<uint2*>(HW);
-                for (int i = lo + lane; i < hi; i += kWave) {
-                    const uint64_t vy = (uint64_t)(HY[i] + oy);
-                    const uint64_t vw = (uint64_t)(HW[i] + ow);
-                    Hw[i] = make_int2((int32_t)(vy >> 32), (int32_t)(vw >> 32));
-                    Lw[i] = make_uint2((uint32_t)vy, (uint32_t)vw);
+                if (MODE == 0) {
+                    int2* Hw = reinterpret_cast<int2*>(S);
+                    uint2* Lw = reinterpret_cast<uint2*>(reinterpret_cast<char*>(S) + 8 * (size_t)wl);
+#pragma unroll
+                    for (int kk = 0; kk < kChunk; ++kk) {
+                        const int i = base + kk;
+                        if (kk < c && i < span) {
+                            const uint64_t vy = (uint64_t)(ry[kk] + oy);
+                            const uint64_t vw = (uint64_t)(rw[kk] + ow);
+                            Hw[i] = make_int2((int32_t)(vy >> 32), (int32_t)(vw >> 32));
+                            Lw[i] = make_uint2((uint32_t)vy, (uint32_t)vw);
+                        }
+                    }
+                } else {
+                    int4* S4 = reinterpret_cast<int4*>(S);
+                    for (int kk = 0; kk < c; ++kk) {
+                        const int i = base + kk;
+                        if (i < span) {
+                            const int s = slot_of(i);
+                            const longlong2 v = reinterpret_cast<const longlong2*>(S4)[s];
+                            oy += v.x;
+                            ow += v.y;
+                            S4[s] = make_int4((int32_t)((uint64_t)oy >> 32),
+                                              (int32_t)((uint64_t)ow >> 32),
+                                              (int32_t)(uint32_t)oy, (int32_t)(uint32_t)ow);
+                        }
+                    }
                 }
             }
-            if constexpr (WIDE) __threadfence();
+            if constexpr (GLOBAL) __threadfence();
             __syncthreads();
 
-            // (b2) FP32 screen: best value per (thread, duration).
+            // (b2) FP32 screen: best value (as non-negative float bits) per (thread,
+            // duration); boxes a screen value cannot rank are flagged for (b3).
             const float ybar = cv.ybar;
             const int s_hi = cv.s_hi;
             const int e_unc = cv.e_unc;
-            float cm[DMAX];
+            int cm[DMAX];
             uint32_t unc_bits = 0;
-            float vmax = -INFINITY;
+            int vmax = 0;
 #pragma unroll
             for (int di = 0; di < DMAX; ++di) {
-                cm[di] = -INFINITY;
+                cm[di] = 0;
                 if (di < n_dur) {
                     const int d = s_dur[di];
                     if (d >= 1 && d < n_bins) {
                         const int m = n_bins - d + 1;
-                        float vm = -INFINITY;
+                        int vm = 0;
                         bool uf = false;
+#pragma unroll 2
                         for (int n = tid; n < m; n += T) {
-                            const int2 h0 = H[n], h1 = H[n + d];
-                            const int dw = h1.y - h0.y;
-                            const int dwo = s_hi - dw;
-                            const float Y = (float)(h1.x - h0.x);
-                            const float W = (float)dw;
-                            const float Wo = (float)dwo;
-                            const float t = fmaf(-ybar, W, Y);
-                            const float den = OBJ == 0 ? W * Wo : W * Wo * Wo;
-                            float v = (-t) * fabsf(t) * __builtin_amdgcn_rcpf(den);
-                            const bool cert = (dw > e_unc) & (dwo > e_unc);
-                            v = cert ? v : -INFINITY;
-                            vm = fmaxf(vm, v);
-                            if (OBJ == 1) uf |= (dw > e_unc) & (dwo <= e_unc) & (dwo > -2);
+                            bool cert, unc;
+                            const float v = screen_value<OBJ>(bv.hi(n), bv.hi(n + d), s_hi,
+                                                              e_unc, ybar, cert, unc);
+                            vm = max(vm, cert ? __float_as_int(v) : 0);
+                            uf |= unc;
                         }
                         cm[di] = vm;
                         if (uf) unc_bits |= 1u << di;
                     }
                 }
-                vmax = fmaxf(vmax, cm[di]);
+                vmax = max(vmax, cm[di]);
             }
-            vmax = wave_max_f32(vmax);
-            if (lane == 0) s_red_f[wave] = vmax;
+            vmax = wave_max_nonneg_dpp(vmax);
+            if (lane == 0) s_red_v[wave] = vmax;
             __syncthreads();
-            float V = s_red_f[0];
+            int Vb = s_red_v[0];
 #pragma unroll
-            for (int w = 1; w < NW; ++w) V = fmaxf(V, s_red_f[w]);
+            for (int w = 1; w < NW; ++w) Vb = max(Vb, s_red_v[w]);
+            const float V = __int_as_float(Vb);
             const bool force = (cv.flags & 1) || !(V >= cv.vdeg) || !(V >= cv.vbnd);
             const float thr = V * kKeep;
+            const int thr_bits = __float_as_int(thr);
 
             // (b3) exact FP64 re-evaluation of what the screen kept, first wins.
             double best = -INFINITY;
@@ -527,25 +665,17 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
             for (int di = 0; di < DMAX; ++di) {
                 if (di < n_dur) {
                     const int d = s_dur[di];
-                    const bool want = force || (cm[di] >= thr) || ((unc_bits >> di) & 1u);
+                    const bool want =
+                        force || (cm[di] >= thr_bits) || ((unc_bits >> di) & 1u);
                     if (d >= 1 && d < n_bins && want) {
                         const int m = n_bins - d + 1;
                         for (int n = tid; n < m; n += T) {
-                            const int2 h0 = H[n], h1 = H[n + d];
-                            const int dw = h1.y - h0.y;
-                            const int dwo = s_hi - dw;
-                            const float Y = (float)(h1.x - h0.x);
-                            const float W = (float)dw;
-                            const float Wo = (float)dwo;
-                            const float t = fmaf(-ybar, W, Y);
-                            const float den = OBJ == 0 ? W * Wo : W * Wo * Wo;
-                            const float v = (-t) * fabsf(t) * __builtin_amdgcn_rcpf(den);
-                            const bool cert = (dw > e_unc) & (dwo > e_unc);
-                            const bool unc =
-                                OBJ == 1 && (dw > e_unc) & (dwo <= e_unc) & (dwo > -2);
+                            bool cert, unc;
+                            const float v = screen_value<OBJ>(bv.hi(n), bv.hi(n + d), s_hi,
+                                                              e_unc, ybar, cert, unc);
                             if (force || (cert && v >= thr) || unc) {
                                 double obj;
-                                if (exact_box<OBJ>(H, L, n, d, cv, obj) && obj > best) {
+                                if (exact_box<OBJ, MODE>(bv, n, d, cv, obj) && obj > best) {
                                     best = obj;
                                     bkey = ((uint32_t)di << 24) | (uint32_t)n;
                                 }
@@ -554,7 +684,7 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                     }
                 }
             }
-            wave_argbest(best, bkey);
+            wave_argbest_ballot(best, bkey);
             if (lane == 0) {
                 s_red_d[wave] = best;
                 s_red_k[wave] = bkey;
@@ -577,8 +707,8 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                 } else {
                     const int n = (int)(bk & 0xFFFFFFu);
                     const int d = s_dur[bk >> 24];
-                    const int2 h0 = H[n], h1 = H[n + d];
-                    const uint2 l0 = L[n], l1 = L[n + d];
+                    const int2 h0 = bv.hi(n), h1 = bv.hi(n + d);
+                    const uint2 l0 = bv.lo(n), l1 = bv.lo(n + d);
                     const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
                     const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
                     const double iv_in = ldexp((double)win_q, cv.w_exp);
@@ -601,7 +731,7 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                     out[6 * n_out + col] = ll;
                 }
             }
-            if constexpr (WIDE) __threadfence();
+            if constexpr (GLOBAL) __threadfence();
             __syncthreads();
         }
     }
@@ -749,49 +879,59 @@ int ensure(cpb_context* ctx, DevBuf& b, size_t bytes) {
     return CPB_OK;
 }
 
-template <int T, int DMAX, int OBJ, bool WIDE>
-hipError_t launch_main(dim3 grid, size_t lds, hipStream_t st, const double* tau,
-                       const int64_t* yq, const int64_t* wq, const CurveParams* curves,
-                       const cpb_segment* segs, const WorkItem* items, int64_t n_items,
-                       const double* periods, const int32_t* dur, int32_t os, int32_t wl,
-                       int64_t* wide, double* out, int64_t n_out) {
-    auto fn = bls_main_kernel<T, DMAX, OBJ, WIDE>;
-    if (!WIDE && lds > 65536) {
+struct LaunchArgs {
+    dim3 grid;
+    size_t lds;
+    hipStream_t st;
+    const double* tau;
+    const int64_t* yq;
+    const int64_t* wq;
+    const CurveParams* curves;
+    const cpb_segment* segs;
+    const WorkItem* items;
+    int64_t n_items;
+    const double* periods;
+    const int32_t* dur;
+    int32_t os;
+    int32_t wl;
+    int64_t* wide;
+    double* out;
+    int64_t n_out;
+};
+
+template <int T, int DMAX, int OBJ, int MODE>
+hipError_t launch_main(const LaunchArgs& a) {
+    auto fn = bls_main_kernel<T, DMAX, OBJ, MODE>;
+    if (MODE != 2 && a.lds > 65536) {
         // Opt in to more than 64 KiB of dynamic LDS where the runtime asks for it; a
         // refusal surfaces at launch.
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)a.lds);
     }
-    hipLaunchKernelGGL(fn, grid, dim3(T), lds, st, tau, yq, wq, curves, segs, items,
-                       n_items, periods, dur, os, wl, wide, out, n_out);
+    hipLaunchKernelGGL(fn, a.grid, dim3(T), a.lds, a.st, a.tau, a.yq, a.wq, a.curves,
+                       a.segs, a.items, a.n_items, a.periods, a.dur, a.os, a.wl, a.wide,
+                       a.out, a.n_out);
     return hipGetLastError();
 }
 
-template <int DMAX, int OBJ, bool WIDE>
-hipError_t dispatch_t(dim3 grid, size_t lds, hipStream_t st, const double* tau,
-                      const int64_t* yq, const int64_t* wq, const CurveParams* curves,
-                      const cpb_segment* segs, const WorkItem* items, int64_t n_items,
-                      const double* periods, const int32_t* dur, int32_t os, int32_t wl,
-                      int64_t* wide, double* out, int64_t n_out) {
-    return launch_main<256, DMAX, OBJ, WIDE>(grid, lds, st, tau, yq, wq, curves, segs,
-                                             items, n_items, periods, dur, os, wl, wide,
-                                             out, n_out);
+template <int OBJ, int MODE>
+hipError_t dispatch_d(int dmax, const LaunchArgs& a) {
+    if (dmax <= 8) return launch_main<256, 8, OBJ, MODE>(a);
+    if (dmax <= 16) return launch_main<256, 16, OBJ, MODE>(a);
+    return launch_main<256, 32, OBJ, MODE>(a);
 }
 
-template <int OBJ, bool WIDE>
-hipError_t dispatch_d(int dmax, dim3 grid, size_t lds, hipStream_t st, const double* tau,
-                      const int64_t* yq, const int64_t* wq, const CurveParams* curves,
-                      const cpb_segment* segs, const WorkItem* items, int64_t n_items,
-                      const double* periods, const int32_t* dur, int32_t os, int32_t wl,
-                      int64_t* wide, double* out, int64_t n_out) {
-#define CPB_D(DM)                                                                     \
-    return dispatch_t<DM, OBJ, WIDE>(grid, lds, st, tau, yq, wq, curves, segs, items, \
-                                     n_items, periods, dur, os, wl, wide, out, n_out)
-    if (dmax <= 8) CPB_D(8);
-    if (dmax <= 16) CPB_D(16);
-    CPB_D(32);
-#undef CPB_D
+template <int OBJ>
+hipError_t dispatch_mode(int mode, int dmax, const LaunchArgs& a) {
+    if (mode == 0) return dispatch_d<OBJ, 0>(dmax, a);
+    if (mode == 1) return dispatch_d<OBJ, 1>(dmax, a);
+    return dispatch_d<OBJ, 2>(dmax, a);
 }
+
+// Workgroup size and prefix chunk of the main kernel: MODE 0 keeps a thread's chunk
+// of at most 8 bins in registers, so it serves phase spans of up to 256*8 bins.
+constexpr int kThreads = 256;
+constexpr int kMode0Span = kThreads * 8;
 
 constexpr int kMaxDur = 32;
 
@@ -926,36 +1066,40 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
 
     const int wl = (wmax + 63) / 64 * 64;
     const size_t static_lds = 2048;
-    const size_t lds_need = (size_t)16 * wl;
-    const bool wide = lds_need + static_lds > (size_t)ctx->lds_limit;
+    const int n_slots = wl + (wl - 1) / 8 + 1;
+    const size_t lds_need = (size_t)16 * n_slots;
+    const bool fits = lds_need + static_lds <= (size_t)ctx->lds_limit;
+    const int mode = fits ? (wl <= kMode0Span ? 0 : 1) : 2;
     int64_t* wide_ptr = nullptr;
-    dim3 grid;
-    size_t lds = 0;
-    if (wide) {
+    LaunchArgs la;
+    la.lds = 0;
+    if (mode == 2) {
         const int64_t g = std::min<int64_t>(n_items, (int64_t)ctx->n_cu * 4);
-        if ((rc = ensure(ctx, ctx->wide, (size_t)g * 16 * (size_t)wl))) return rc;
+        if ((rc = ensure(ctx, ctx->wide, (size_t)g * lds_need))) return rc;
         wide_ptr = static_cast<int64_t*>(ctx->wide.ptr);
-        grid = dim3((unsigned)g);
+        la.grid = dim3((unsigned)g);
     } else {
-        grid = dim3((unsigned)std::min<int64_t>(n_items, (int64_t)1 << 30));
-        lds = lds_need;
+        la.grid = dim3((unsigned)std::min<int64_t>(n_items, (int64_t)1 << 30));
+        la.lds = lds_need;
     }
-    const auto* cp = static_cast<const CurveParams*>(ctx->curves.ptr);
-    const auto* yqp = static_cast<const int64_t*>(ctx->yq.ptr);
-    const auto* wqp = static_cast<const int64_t*>(ctx->wq.ptr);
+    la.st = st;
+    la.tau = tau;
+    la.yq = static_cast<const int64_t*>(ctx->yq.ptr);
+    la.wq = static_cast<const int64_t*>(ctx->wq.ptr);
+    la.curves = static_cast<const CurveParams*>(ctx->curves.ptr);
+    la.segs = sp;
+    la.items = ip;
+    la.n_items = n_items;
+    la.periods = periods;
+    la.dur = dur_bins;
+    la.os = oversample;
+    la.wl = wl;
+    la.wide = wide_ptr;
+    la.out = out;
+    la.n_out = n_out;
     if (ctx->timing) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
-    hipError_t e;
-    if (obj_flag == 0) {
-        e = wide ? dispatch_d<0, true>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
-                                       periods, dur_bins, oversample, wl, wide_ptr, out, n_out)
-                 : dispatch_d<0, false>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
-                                        periods, dur_bins, oversample, wl, wide_ptr, out, n_out);
-    } else {
-        e = wide ? dispatch_d<1, true>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
-                                       periods, dur_bins, oversample, wl, wide_ptr, out, n_out)
-                 : dispatch_d<1, false>(dmax, grid, lds, st, tau, yqp, wqp, cp, sp, ip, n_items,
-                                        periods, dur_bins, oversample, wl, wide_ptr, out, n_out);
-    }
+    const hipError_t e = obj_flag == 0 ? dispatch_mode<0>(mode, dmax, la)
+                                       : dispatch_mode<1>(mode, dmax, la);
     if (e != hipSuccess) return hip_fail(ctx, e, "bls_main_kernel launch");
     if (ctx->timing) {
         CPB_HIP(ctx, hipEventRecord(ctx->t1, st));

@@ -136,6 +136,24 @@ def test_wide_histogram_path_matches_oracle():
     _say(f"  errors {errs}")
 
 
+def test_mid_span_histogram_path_matches_oracle():
+    # ~7,000 phase bins per period: above the register-chunk prefix (2,048 bins), still
+    # within one workgroup's LDS on a 160 KiB-LDS device (slot layout in place).
+    _say("case mid-span histograms")
+    rng = np.random.default_rng(4)
+    t = np.sort(rng.uniform(0.0, 300.0, 3000))
+    flux = 1.0 + 1e-3 * rng.standard_normal(t.size)
+    flux[np.mod(t - 1.0, 29.0) < 0.2] -= 3e-3
+    err = np.full(t.size, 1e-3)
+    periods = np.linspace(25.0, 35.0, 150)
+    durations = np.array([0.05, 0.1, 0.2])
+    for objective in ("snr", "likelihood"):
+        got = _hip_power(t, flux, err, periods, durations, 10, objective)
+        ref = oracle.bls_power(t, flux, err, periods, durations, 10, objective=objective)
+        errs = compare(got, ref, name=f"mid-span/{objective}")
+        _say(f"  errors {errs}")
+
+
 def test_segments_equal_per_segment_calls():
     # tests/test_bls_segments.py:42-61: the segmented call equals one call per
     # segment; an empty segment comes back empty, in place.
