@@ -131,18 +131,6 @@ __device__ __forceinline__ bool better(double a, uint32_t ka, double b, uint32_t
     return a > b || (a == b && ka < kb);
 }
 
-__device__ __forceinline__ void wave_argbest(double& obj, uint32_t& key) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const double o = __shfl_xor(obj, off, kWave);
-        const uint32_t k = __shfl_xor(key, off, kWave);
-        if (better(o, k, obj, key)) {
-            obj = o;
-            key = k;
-        }
-    }
-}
-
 // Phase bin of one point: `int(w/bd) + 1` with `w = x - P*floor(x/P)`, clipped to
 // [0, width-1], bit for bit as the FP64 divisions give it. The quotients are formed
 // with reciprocals (|error| <= 2^-52 |q|); when a quotient lies within 2^-49 |q| of
@@ -366,24 +354,10 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// First-wins best (objective, key) of a wave, uniform: the lanes holding a candidate
-// are few, so they are visited one by one.
-__device__ __forceinline__ void wave_argbest_ballot(double& obj, uint32_t& key) {
-    unsigned long long m = __ballot(obj > -INFINITY);
-    double wb = -INFINITY;
-    uint32_t wk = 0xFFFFFFFFu;
-    while (m) {
-        const int l = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        const double v = readlane_f64(obj, l);
-        const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key, l);
-        if (better(v, k, wb, wk)) {
-            wb = v;
-            wk = k;
-        }
-    }
-    obj = wb;
-    key = wk;
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 template <int MODE>
@@ -405,29 +379,69 @@ struct BinView {
     }
 };
 
-// Exact FP64 statistics of box (n, d) from the 64-bit prefix sums; false for an
-// inadmissible box (floor, or not a dip), as the reference's sweep skips it.
+// Exact ranking key of box (n, d) from the 64-bit prefix sums, in the curve's fixed-point
+// units: with Yi, Wi (Yo, Wo) the in-box (out-of-box) sums and num = Yo*Wi - Yi*Wo =
+// depth*Wi*Wo, the objective is a fixed positive multiple of sqrt(num^2/den) with
+// den = Wi*Wo*(Wi+Wo) (SNR) or Wi*Wo^2 (log-likelihood), so boxes compare by
+// num_a^2*den_b vs num_b^2*den_a without a division. Boxes with equal sums get equal
+// keys, so exact ties stay ties for the first-wins rule. Returns false for an
+// inadmissible box: a side below the floor, or not a dip (num < 0, i.e. yout < yin).
+struct BoxKey {
+    double num2;
+    double den;
+    int64_t yin;
+    int64_t win;
+};
+
 template <int OBJ, int MODE>
-__device__ __forceinline__ bool exact_box(const BinView<MODE>& bv, int n, int d,
-                                          const CurveParams& cv, double& obj) {
+__device__ __forceinline__ bool exact_key(const BinView<MODE>& bv, int n, int d,
+                                          const CurveParams& cv, BoxKey& k) {
     const int2 h0 = bv.hi(n), h1 = bv.hi(n + d);
     const uint2 l0 = bv.lo(n), l1 = bv.lo(n + d);
     const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
     const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
-    const double iv_in = ldexp((double)win_q, cv.w_exp);
-    const double iv_out = ldexp((double)(cv.w_tot - win_q), cv.w_exp);
-    if (iv_in < cv.ivar_floor || iv_out < cv.ivar_floor) return false;
-    const double y_in = ldexp((double)yin_q, cv.y_exp);
-    const double y_rest = ldexp((double)(cv.y_tot - yin_q), cv.y_exp);
-    const double yin = y_in / iv_in;
-    const double yout = y_rest / iv_out;
-    if (yout < yin) return false;
-    const double depth = yout - yin;
-    if (OBJ == 0)
-        obj = depth / sqrt(1.0 / iv_in + 1.0 / iv_out);
-    else
-        obj = 0.5 * iv_in * depth * depth;
+    const int64_t wout_q = cv.w_tot - win_q;
+    const double Wi = (double)win_q, Wo = (double)wout_q;
+    if (ldexp(Wi, cv.w_exp) < cv.ivar_floor || ldexp(Wo, cv.w_exp) < cv.ivar_floor)
+        return false;
+    const double Yi = (double)yin_q, Yo = (double)(cv.y_tot - yin_q);
+    const double num = fma(Yo, Wi, -(Yi * Wo));
+    if (num < 0.0) return false;
+    k.num2 = num * num;
+    k.den = OBJ == 0 ? Wi * Wo * (double)cv.w_tot : Wi * Wo * Wo;
+    k.yin = yin_q;
+    k.win = win_q;
     return true;
+}
+
+__device__ __forceinline__ bool key_better(double an, double ad, uint32_t ak, double bn,
+                                           double bd, uint32_t bk) {
+    const double l = an * bd, r = bn * ad;
+    return l > r || (l == r && ak < bk);
+}
+
+// First-wins best box of a wave, uniform: the lanes holding a candidate are few, so
+// they are visited one by one.
+__device__ __forceinline__ void wave_best_ballot(BoxKey& k, uint32_t& key) {
+    unsigned long long m = __ballot(k.num2 >= 0.0);
+    BoxKey w{-1.0, 1.0, 0, 0};
+    uint32_t wk = 0xFFFFFFFFu;
+    while (m) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const double n2 = readlane_f64(k.num2, l);
+        const double dn = readlane_f64(k.den, l);
+        const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)key, l);
+        if (key_better(n2, dn, kk, w.num2, w.den, wk)) {
+            w.num2 = n2;
+            w.den = dn;
+            w.yin = readlane_i64(k.yin, l);
+            w.win = readlane_i64(k.win, l);
+            wk = kk;
+        }
+    }
+    k = w;
+    key = wk;
 }
 
 // FP32 screen value of box (n, d): -t|t| / den with t = Y - ybar*W (a dip has t <= 0),
@@ -462,7 +476,7 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
     __shared__ int32_t s_dur[DMAX];
     __shared__ int64_t s_red_i[2 * NW];
     __shared__ int s_red_v[NW];
-    __shared__ double s_red_d[NW];
+    __shared__ BoxKey s_red_b[NW];
     __shared__ uint32_t s_red_k[NW];
 
     const int tid = threadIdx.x;
@@ -528,16 +542,23 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
             }
             if constexpr (GLOBAL) __threadfence();
             __syncthreads();
-            // (a') wrap copy, in order (a copy may read a bin an earlier one wrote).
-            if (tid == 0) {
-                longlong2* S2 = reinterpret_cast<longlong2*>(S);
-                for (int kk = 0; kk < oversample; ++kk) {
-                    const int dst = n_bins - oversample + kk;
-                    if (dst >= 0 && dst < width) S2[slot_of(dst)] = S2[slot_of(1 + kk)];
+            // (a') wrap copy of `oversample` bins, bin[n_bins-os+k] = bin[1+k]. When the
+            // copied and the overwritten bins are disjoint, MODE 0's prefix reads the
+            // source bin in place of each overwritten one; otherwise the copies run in
+            // order here (a copy may read a bin an earlier one wrote).
+            const int wrap_lo = n_bins - oversample;
+            const bool fused_wrap = MODE == 0 && wrap_lo > oversample;
+            if (!fused_wrap) {
+                if (tid == 0) {
+                    longlong2* S2 = reinterpret_cast<longlong2*>(S);
+                    for (int kk = 0; kk < oversample; ++kk) {
+                        const int dst = n_bins - oversample + kk;
+                        if (dst >= 0 && dst < width) S2[slot_of(dst)] = S2[slot_of(1 + kk)];
+                    }
                 }
+                if constexpr (GLOBAL) __threadfence();
+                __syncthreads();
             }
-            if constexpr (GLOBAL) __threadfence();
-            __syncthreads();
 
             // (b1) inclusive prefix over [0, n_bins]: every thread sums a contiguous
             // chunk, one DPP scan per wave and one exchange between waves give each
@@ -554,7 +575,10 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                     for (int kk = 0; kk < kChunk; ++kk) {
                         const int i = base + kk;
                         if (kk < c && i < span) {
-                            const longlong2 v = S2[slot_of(i)];
+                            const int src = (fused_wrap && i >= wrap_lo && i < n_bins)
+                                                ? 1 + (i - wrap_lo)
+                                                : i;
+                            const longlong2 v = S2[slot_of(src)];
                             sy += v.x;
                             sw += v.y;
                         }
@@ -658,8 +682,8 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
             const float thr = V * kKeep;
             const int thr_bits = __float_as_int(thr);
 
-            // (b3) exact FP64 re-evaluation of what the screen kept, first wins.
-            double best = -INFINITY;
+            // (b3) exact re-evaluation of what the screen kept, first wins.
+            BoxKey best{-1.0, 1.0, 0, 0};
             uint32_t bkey = 0xFFFFFFFFu;
 #pragma unroll
             for (int di = 0; di < DMAX; ++di) {
@@ -674,9 +698,10 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                             const float v = screen_value<OBJ>(bv.hi(n), bv.hi(n + d), s_hi,
                                                               e_unc, ybar, cert, unc);
                             if (force || (cert && v >= thr) || unc) {
-                                double obj;
-                                if (exact_box<OBJ, MODE>(bv, n, d, cv, obj) && obj > best) {
-                                    best = obj;
+                                BoxKey kb;
+                                if (exact_key<OBJ, MODE>(bv, n, d, cv, kb) &&
+                                    kb.num2 * best.den > best.num2 * kb.den) {
+                                    best = kb;
                                     bkey = ((uint32_t)di << 24) | (uint32_t)n;
                                 }
                             }
@@ -684,37 +709,36 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                     }
                 }
             }
-            wave_argbest_ballot(best, bkey);
+            wave_best_ballot(best, bkey);
             if (lane == 0) {
-                s_red_d[wave] = best;
+                s_red_b[wave] = best;
                 s_red_k[wave] = bkey;
             }
             __syncthreads();
 
-            // (c) finalize in FP64 and write the seven outputs.
+            // (c) finalize in FP64 from the winner's exact sums and write the seven
+            // outputs. Nothing below reads the bins, so the next period's fold can start
+            // while thread 0 finishes.
             if (tid == 0) {
-                double b = s_red_d[0];
+                BoxKey b = s_red_b[0];
                 uint32_t bk = s_red_k[0];
                 for (int w = 1; w < NW; ++w) {
-                    if (better(s_red_d[w], s_red_k[w], b, bk)) {
-                        b = s_red_d[w];
+                    const BoxKey o = s_red_b[w];
+                    if (key_better(o.num2, o.den, s_red_k[w], b.num2, b.den, bk)) {
+                        b = o;
                         bk = s_red_k[w];
                     }
                 }
-                if (!(b > -INFINITY)) {
+                if (!(b.num2 >= 0.0)) {
                     out[col] = -INFINITY;
                     for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
                 } else {
                     const int n = (int)(bk & 0xFFFFFFu);
                     const int d = s_dur[bk >> 24];
-                    const int2 h0 = bv.hi(n), h1 = bv.hi(n + d);
-                    const uint2 l0 = bv.lo(n), l1 = bv.lo(n + d);
-                    const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
-                    const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
-                    const double iv_in = ldexp((double)win_q, cv.w_exp);
-                    const double iv_out = ldexp((double)(cv.w_tot - win_q), cv.w_exp);
-                    const double y_in = ldexp((double)yin_q, cv.y_exp);
-                    const double y_rest = ldexp((double)(cv.y_tot - yin_q), cv.y_exp);
+                    const double iv_in = ldexp((double)b.win, cv.w_exp);
+                    const double iv_out = ldexp((double)(cv.w_tot - b.win), cv.w_exp);
+                    const double y_in = ldexp((double)b.yin, cv.y_exp);
+                    const double y_rest = ldexp((double)(cv.y_tot - b.yin), cv.y_exp);
                     const double yin = y_in / iv_in;
                     const double yout = y_rest / iv_out;
                     const double depth = yout - yin;
@@ -722,17 +746,22 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
                     const double dsnr = depth / derr;
                     const double ll = 0.5 * iv_in * depth * depth;
                     const double dur = (double)d * bd;
+                    // (n*bd + dur/2) lies in [0, 2P): fmod by one exact subtraction.
+                    const double a = (double)n * bd + 0.5 * dur;
+                    const double ph = a < P ? a : (a < 2.0 * P ? a - P : fmod(a, P));
                     out[col] = OBJ == 0 ? dsnr : ll;
                     out[1 * n_out + col] = depth;
                     out[2 * n_out + col] = derr;
                     out[3 * n_out + col] = dsnr;
                     out[4 * n_out + col] = dur;
-                    out[5 * n_out + col] = fmod((double)n * bd + 0.5 * dur, P) + cv.t_min;
+                    out[5 * n_out + col] = ph + cv.t_min;
                     out[6 * n_out + col] = ll;
                 }
             }
-            if constexpr (GLOBAL) __threadfence();
-            __syncthreads();
+            if constexpr (GLOBAL) {
+                __threadfence();
+                __syncthreads();
+            }
         }
     }
 }
