@@ -307,6 +307,8 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
 // cy/cw are the inclusive prefix sums; hi/lo their 32-bit words.
 
 constexpr int kChunk = 8;
+// Periods per work item (the item's period table lives in LDS).
+constexpr int kMaxTile = 32;
 
 __device__ __forceinline__ int slot_of(int i) { return i + (i >> 3); }
 
@@ -474,6 +476,9 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
     constexpr int NW = T / kWave;
     constexpr bool GLOBAL = MODE == 2;
     __shared__ int32_t s_dur[DMAX];
+    __shared__ double s_per[kMaxTile];
+    __shared__ double s_invp[kMaxTile];
+    __shared__ int32_t s_nb[kMaxTile];
     __shared__ int64_t s_red_i[2 * NW];
     __shared__ int s_red_v[NW];
     __shared__ BoxKey s_red_b[NW];
@@ -496,22 +501,30 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
         const cpb_segment sg = segs[it.seg];
         const CurveParams cv = curves[sg.curve];
         const int n_dur = sg.n_dur;
-        __syncthreads();
-        if (tid < n_dur) s_dur[tid] = dur_bins[sg.dur_offset + tid];
-        __syncthreads();
         const double bd = sg.bin_duration;
         const double inv_bd = 1.0 / bd;
         const int width = sg.width;
         const int64_t poff = cv.point_offset;
         const int npts = cv.n_points;
+        __syncthreads();
+        if (tid < n_dur) s_dur[tid] = dur_bins[sg.dur_offset + tid];
+        // The item's periods, their reciprocals and bin counts, one lane per period.
+        if (tid >= kWave && tid < kWave + it.count) {
+            const int q = tid - kWave;
+            const double P = segment_period(sg, periods, it.begin + q);
+            int64_t nb = (int64_t)ceil(P / bd) + oversample;
+            if (nb > width - 1) nb = width - 1;
+            s_per[q] = P;
+            s_invp[q] = 1.0 / P;
+            s_nb[q] = (int32_t)nb;
+        }
+        __syncthreads();
 
         for (int q = 0; q < it.count; ++q) {
             const int64_t k = it.begin + q;
-            const double P = segment_period(sg, periods, k);
-            const double invP = 1.0 / P;
-            int64_t nb = (int64_t)ceil(P / bd) + oversample;
-            if (nb > width - 1) nb = width - 1;
-            const int n_bins = (int)nb;
+            const double P = s_per[q];
+            const double invP = s_invp[q];
+            const int n_bins = s_nb[q];
             const int64_t col = sg.out_offset + k;
 
             if (cv.flags & 2) {
@@ -646,31 +659,36 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
             const int s_hi = cv.s_hi;
             const int e_unc = cv.e_unc;
             int cm[DMAX];
+            int dd[DMAX];
             uint32_t unc_bits = 0;
-            int vmax = 0;
+            uint32_t dok = 0;  // durations with 1 <= d < n_bins (uniform)
 #pragma unroll
             for (int di = 0; di < DMAX; ++di) {
                 cm[di] = 0;
-                if (di < n_dur) {
-                    const int d = s_dur[di];
-                    if (d >= 1 && d < n_bins) {
-                        const int m = n_bins - d + 1;
-                        int vm = 0;
-                        bool uf = false;
-#pragma unroll 2
-                        for (int n = tid; n < m; n += T) {
+                dd[di] = di < n_dur ? s_dur[di] : 0;
+                if (di < n_dur && dd[di] >= 1 && dd[di] < n_bins) dok |= 1u << di;
+            }
+            // Start-outer, duration-inner: one read of H[n] per start, and the
+            // durations' boxes are independent work for the scheduler.
+            for (int n = tid; n < n_bins; n += T) {
+                const int2 h0 = bv.hi(n);
+#pragma unroll
+                for (int di = 0; di < DMAX; ++di) {
+                    if ((dok >> di) & 1u) {
+                        const int d = dd[di];
+                        if (n + d <= n_bins) {
                             bool cert, unc;
-                            const float v = screen_value<OBJ>(bv.hi(n), bv.hi(n + d), s_hi,
-                                                              e_unc, ybar, cert, unc);
-                            vm = max(vm, cert ? __float_as_int(v) : 0);
-                            uf |= unc;
+                            const float v = screen_value<OBJ>(h0, bv.hi(n + d), s_hi, e_unc,
+                                                              ybar, cert, unc);
+                            cm[di] = max(cm[di], cert ? __float_as_int(v) : 0);
+                            if (OBJ == 1 && unc) unc_bits |= 1u << di;
                         }
-                        cm[di] = vm;
-                        if (uf) unc_bits |= 1u << di;
                     }
                 }
-                vmax = max(vmax, cm[di]);
             }
+            int vmax = 0;
+#pragma unroll
+            for (int di = 0; di < DMAX; ++di) vmax = max(vmax, cm[di]);
             vmax = wave_max_nonneg_dpp(vmax);
             if (lane == 0) s_red_v[wave] = vmax;
             __syncthreads();
@@ -1029,7 +1047,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         if (segments[s].n_dur > 0) live += segments[s].n_periods;
     const int64_t target_items = (int64_t)ctx->n_cu * 64;
     int64_t tile = live / std::max<int64_t>(1, target_items);
-    tile = std::max<int64_t>(1, std::min<int64_t>(tile, 32));
+    tile = std::max<int64_t>(1, std::min<int64_t>(tile, kMaxTile));
     ctx->h_items.clear();
     std::vector<cpb_segment> empties;
     for (int32_t s = 0; s < n_segments; ++s) {
