@@ -2,7 +2,7 @@
 
 Workload (default): the 4,096-curve survey batch -- N log-uniform in [100, 5,000],
 baselines 50-2,000 d, 200,000 frequencies per curve in [1/baseline, 2] /d, 6
-period-scaled durations per log segment (``cuperiod_amd.workloads.config3``).
+period-scaled durations per log segment (``cuperiod_amd.workloads.survey_batch``).
 
 A step is one batched search of every curve (fold, box scan, argmax, per-curve
 summary with the best period) with the inputs already resident in HBM. W untimed warm-up
@@ -127,10 +127,10 @@ def main() -> int:
     from cuperiod_amd import _lib
     from cuperiod_amd.batch import engine
     from cuperiod_amd.core.config import BLSSettings
-    from cuperiod_amd.workloads import config3
+    from cuperiod_amd.workloads import survey_batch
 
     log(f"building workload: {args.curves} curves x {args.freqs} frequencies")
-    wl = config3(args.curves, args.freqs)
+    wl = survey_batch(args.curves, args.freqs)
     settings = BLSSettings(n_durations=wl.n_durations)
     ctx = _lib.context(0)
     t0 = time.perf_counter()

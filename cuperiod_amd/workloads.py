@@ -1,21 +1,21 @@
 """Seeded synthetic survey workloads used by the benchmark and the tests.
 
-Five configurations (the benchmark's `configs`), each generated with
-``np.random.default_rng(20261203 + k)`` for configuration ``k``:
+Five workloads, each generated with ``np.random.default_rng(20261203 + k)`` for the k-th
+of the list below:
 
-1. one curve, N = 2,000 over 100 d, box transit P = 3.7 d, depth 2e-3, duration 0.1 d,
+1. ``small_curve``: one curve, N = 2,000 over 100 d, box transit P = 3.7 d, depth 2e-3, duration 0.1 d,
    phase 0.3; 50,000 frequencies linear in [0.02, 2] /d; durations {0.05, 0.1, 0.15,
    0.2} d; oversample 10;
-2. one curve, N = 20,000 over 1,000 d with 30% removed in seasonal gaps,
+2. ``gapped_curve``: one curve, N = 20,000 over 1,000 d with 30% removed in seasonal gaps,
    sigma 2e-3, P = 11.3 d, depth 1.5e-3, duration 0.2 d; 1,000,000 frequencies in
    [0.001, 5] /d; 8 durations log-spaced in [0.04, 0.4] d;
-3. 4,096 curves, N log-uniform in [100, 5,000], baselines uniform in [50, 2,000] d,
+3. ``survey_batch``: 4,096 curves, N log-uniform in [100, 5,000], baselines uniform in [50, 2,000] d,
    sigma log-uniform in [5e-4, 5e-3]; even-index curves carry a box transit (P
    log-uniform in [0.5, 20] d, depth log-uniform in [5e-4, 1e-2], duration/period
    uniform in [0.01, 0.05]); 200,000 frequencies in [1/baseline, 2] /d per curve;
    6 period-scaled durations per log segment;
-4. like 3 with 16,384 curves (the learned-proposer configuration);
-5. 256 curves, N = 100,000 over 10,000 d offset by 2.45e6 d, P = 0.7 d, depth 1e-3,
+4. ``proposer_batch``: like 3 with 16,384 curves (the learned proposer's workload);
+5. ``long_baseline_batch``: 256 curves, N = 100,000 over 10,000 d offset by 2.45e6 d, P = 0.7 d, depth 1e-3,
    duration 0.05 d; 4,000,000 frequencies in [1e-4, 10] /d; 16 durations.
 
 A curve is ``(t, flux, dy)`` with flux in the flux domain (a transit is a dip) and a
@@ -50,7 +50,7 @@ class Workload:
     truth: list = field(default_factory=list)
 
 
-def config1() -> Workload:
+def small_curve() -> Workload:
     rng = np.random.default_rng(SEED + 1)
     n = 2000
     t = np.sort(rng.uniform(0.0, 100.0, n))
@@ -58,11 +58,11 @@ def config1() -> Workload:
     y[_box_dip(t, 3.7, 0.3 * 3.7, 0.1)] -= 2e-3
     from cuperiod_amd.batch.engine import LinspaceGrid
 
-    return Workload("config1", [(t, y, np.full(n, 1e-3))], [LinspaceGrid(0.02, 2.0, 50000)],
+    return Workload("small_curve", [(t, y, np.full(n, 1e-3))], [LinspaceGrid(0.02, 2.0, 50000)],
                     durations=np.array([0.05, 0.1, 0.15, 0.2]), truth=[3.7])
 
 
-def config2() -> Workload:
+def gapped_curve() -> Workload:
     rng = np.random.default_rng(SEED + 2)
     t = np.sort(rng.uniform(0.0, 1000.0, 20000))
     t = t[np.mod(t / 365.25, 1.0) >= 0.3]
@@ -71,7 +71,7 @@ def config2() -> Workload:
     y[_box_dip(t, 11.3, 0.0, 0.2)] -= 1.5e-3
     from cuperiod_amd.batch.engine import LinspaceGrid
 
-    return Workload("config2", [(t, y, np.full(n, 2e-3))], [LinspaceGrid(0.001, 5.0, 1_000_000)],
+    return Workload("gapped_curve", [(t, y, np.full(n, 2e-3))], [LinspaceGrid(0.001, 5.0, 1_000_000)],
                     durations=np.geomspace(0.04, 0.4, 8), truth=[11.3])
 
 
@@ -100,17 +100,17 @@ def survey_curves(n_curves: int, seed: int, n_freq: int = 200_000):
     return curves, grids, truth
 
 
-def config3(n_curves: int = 4096, n_freq: int = 200_000) -> Workload:
+def survey_batch(n_curves: int = 4096, n_freq: int = 200_000) -> Workload:
     curves, grids, truth = survey_curves(n_curves, SEED + 3, n_freq)
-    return Workload("config3", curves, grids, n_durations=6, truth=truth)
+    return Workload("survey_batch", curves, grids, n_durations=6, truth=truth)
 
 
-def config4(n_curves: int = 16384, n_freq: int = 200_000) -> Workload:
+def proposer_batch(n_curves: int = 16384, n_freq: int = 200_000) -> Workload:
     curves, grids, truth = survey_curves(n_curves, SEED + 4, n_freq)
-    return Workload("config4", curves, grids, n_durations=6, truth=truth)
+    return Workload("proposer_batch", curves, grids, n_durations=6, truth=truth)
 
 
-def config5(n_curves: int = 256, n_freq: int = 4_000_000, n_points: int = 100_000) -> Workload:
+def long_baseline_batch(n_curves: int = 256, n_freq: int = 4_000_000, n_points: int = 100_000) -> Workload:
     from cuperiod_amd.batch.engine import LinspaceGrid
 
     rng = np.random.default_rng(SEED + 5)
@@ -121,8 +121,8 @@ def config5(n_curves: int = 256, n_freq: int = 4_000_000, n_points: int = 100_00
         y[_box_dip(t, 0.7, 2.45e6 + 0.1, 0.05)] -= 1e-3
         curves.append((t, y, np.full(n_points, 1e-3)))
         grids.append(LinspaceGrid(1e-4, 10.0, n_freq))
-    return Workload("config5", curves, grids, n_durations=16, truth=[0.7] * n_curves)
+    return Workload("long_baseline_batch", curves, grids, n_durations=16, truth=[0.7] * n_curves)
 
 
-__all__ = ["SEED", "Workload", "config1", "config2", "config3", "config4", "config5",
-           "survey_curves"]
+__all__ = ["SEED", "Workload", "gapped_curve", "long_baseline_batch", "proposer_batch",
+           "small_curve", "survey_batch", "survey_curves"]

@@ -2,7 +2,7 @@
 
 The bar (stated in every test that uses it):
 * power: identical -inf pattern; every finite power within ``rtol`` (default 1e-6, the
-  north-star bound) of the FP64 oracle -- observed ~1e-13, since the engine's bins are
+  bound the engine is held to) of the FP64 oracle -- observed ~1e-13, since the engine's bins are
   exact and the oracle's carry FP64 rounding;
 * the chosen box (duration, transit time) identical at >= ``box_frac`` of the periods
   (default 0.995, the reference's own GPU bar in tests/test_bls_segments.py:110-131),

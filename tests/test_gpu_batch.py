@@ -85,10 +85,10 @@ def test_survey_best_periods_match_oracle_up_to_ties():
     # that fold the best box's points identically; the oracle's FP64 rounding picks
     # any sample of the plateau, exact sums pick its first).
     from cuperiod_amd.core.result import Periodogram
-    from cuperiod_amd.workloads import config3
+    from cuperiod_amd.workloads import survey_batch
 
     print("case survey best periods", flush=True)
-    wl = config3(8, 200_000)
+    wl = survey_batch(8, 200_000)
     settings = cup.BLSSettings(n_durations=wl.n_durations)
     res = bls_batch(wl.curves, wl.grids, settings, spectra=True)
     exact = 0
