@@ -119,7 +119,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the batch "
-              "shards by curve with no exchange; see DESIGN.md)", file=sys.stderr)
+              "shards by curve with no exchange between GPUs)", file=sys.stderr)
         return 2
 
     import torch

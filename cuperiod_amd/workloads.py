@@ -76,7 +76,7 @@ def gapped_curve() -> Workload:
 
 
 def survey_curves(n_curves: int, seed: int, n_freq: int = 200_000):
-    """Configurations 3/4: uneven survey curves with per-curve linspace grids."""
+    """Uneven survey curves with per-curve linspace grids (survey and proposer batches)."""
     from cuperiod_amd.batch.engine import LinspaceGrid
 
     rng = np.random.default_rng(seed)
