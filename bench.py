@@ -36,6 +36,7 @@ METRIC = ("light curves/s, 200k-freq BLS, 1 GPU: learned+refine vs plain, best-p
 PEAK_FP32 = 157.3e12
 PEAK_LDS = 256 * 128 * 2.4e9
 PEAK_HBM = 8.0e12
+COPY_BYTES = 256 << 20
 
 
 def log(msg: str) -> None:
@@ -116,6 +117,8 @@ def main() -> int:
     ap.add_argument("--freqs", type=int, default=200_000)
     ap.add_argument("--cpu-curves", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one plain 256 MiB device copy (the HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the batch "
@@ -133,6 +136,14 @@ def main() -> int:
     wl = survey_batch(args.curves, args.freqs)
     settings = BLSSettings(n_durations=wl.n_durations)
     ctx = _lib.context(0)
+    if args.calibrate_copy:
+        src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {COPY_BYTES >> 20} MiB done")
+        del src, dst
     t0 = time.perf_counter()
     pb = engine.pack(wl.curves, wl.grids, settings)
     log(f"packed in {time.perf_counter() - t0:.1f} s: {pb.n_segments} segments, "

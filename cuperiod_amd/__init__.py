@@ -7,6 +7,7 @@ for gfx950 behind the C ABI of ``include/cuperiod_bls.h`` (backend name ``"hip"`
 """
 
 from cuperiod_amd.api import best_periods, periodogram, to_input
+from cuperiod_amd.batch.runner import BatchSummary, batch_periodograms
 from cuperiod_amd.core.columns import Domain
 from cuperiod_amd.core.config import BLSSettings
 from cuperiod_amd.core.errors import (
@@ -31,6 +32,7 @@ __all__ = [
     "BLSPower",
     "BLSSettings",
     "BackendUnavailableError",
+    "BatchSummary",
     "CuPeriodError",
     "DeviceError",
     "Domain",
@@ -43,6 +45,7 @@ __all__ = [
     "Peak",
     "Periodogram",
     "UnknownMethodError",
+    "batch_periodograms",
     "best_periods",
     "bls_power",
     "bls_power_segments",

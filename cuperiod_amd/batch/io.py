@@ -1,0 +1,287 @@
+"""Batch inputs, result rows and sinks.
+
+The row format and sink behaviour of cuPeriod's batch driver
+(src/cuperiod/batch/io.py:58-131, 395-482, 507-667): one row per light curve and
+method with ``key, method, source, backend, n_samples, baseline, best_period`` and, for
+ranks ``k = 1..n_best``, ``period_k, power_k`` and the method's declared per-peak extras
+``{extra}_k`` (NaN past the last peak); BLS peaks are chosen alias-diverse. Parquet and
+CSV files are written to an exclusively created temporary file and renamed into place;
+CSV text cells that a spreadsheet would evaluate are quote-prefixed.
+"""
+
+from __future__ import annotations
+
+import csv
+import numbers
+import os
+import secrets
+import warnings
+from collections import Counter
+from collections.abc import Iterable, Mapping, Sequence
+from pathlib import Path
+from typing import Any
+
+from cuperiod_amd.core.lightcurve import LightCurve
+from cuperiod_amd.core.result import Periodogram
+
+#: Methods whose secondary peaks skip harmonics and aliases of the chosen ones.
+ALIAS_DIVERSE_METHODS: frozenset[str] = frozenset({"BLS", "TLS"})
+
+#: Per-peak extras each method declares, in column order.
+PEAK_EXTRA_COLUMNS: Mapping[str, tuple[str, ...]] = {
+    "BLS": ("depth", "depth_snr", "duration", "t0", "sde"),
+    "TLS": ("sr", "depth", "duration", "t0"),
+    "GLS": ("fap",),
+}
+
+_CSV_FORMULA_PREFIXES = ("=", "+", "-", "@", "\t", "\r")
+
+InputItem = tuple[str, Any]
+
+
+def _canonical_method(name: str) -> str:
+    return "".join(ch for ch in str(name) if ch.isalnum()).upper()
+
+
+def alias_diverse_for(method: str) -> bool:
+    """Whether ``method``'s batch peaks are selected alias-diverse."""
+    return _canonical_method(method) in ALIAS_DIVERSE_METHODS
+
+
+def peak_extra_columns(methods: Sequence[str]) -> tuple[str, ...]:
+    """Ordered union of the per-peak extras ``methods`` declare."""
+    seen: dict[str, None] = {}
+    for name in methods:
+        for extra in PEAK_EXTRA_COLUMNS.get(_canonical_method(name), ()):
+            seen.setdefault(extra, None)
+    return tuple(seen)
+
+
+def source_label(source: Any) -> str:
+    """Provenance of an input: an absolute path, or ``""`` for an in-memory curve."""
+    if isinstance(source, (str, os.PathLike)):
+        return str(Path(source).resolve())
+    return ""
+
+
+def _is_key(value: object) -> bool:
+    return isinstance(value, str) or (
+        isinstance(value, numbers.Integral) and not isinstance(value, bool)
+    )
+
+
+def resolve_inputs(inputs: Any) -> list[InputItem]:
+    """``[(key, LightCurve)]`` from an iterable of light curves or ``(key, lc)`` pairs.
+
+    A bare light curve is keyed by ``meta["id"]`` or its position. Keys must be strings
+    or integers; repeated keys are renamed ``key#2``, ``key#3``, ... with a warning.
+    File, glob and table inputs are not read by this package: load them into
+    :class:`LightCurve` objects first.
+    """
+    if isinstance(inputs, (str, os.PathLike)):
+        raise TypeError(
+            "batch_periodograms: file and glob inputs are not read by this package; pass "
+            "LightCurve objects or (key, LightCurve) pairs"
+        )
+    if isinstance(inputs, LightCurve):
+        inputs = [inputs]
+    items: list[InputItem] = []
+    for i, entry in enumerate(inputs):
+        if isinstance(entry, tuple) and len(entry) == 2 and isinstance(entry[1], LightCurve):
+            key, lc = entry
+            if not _is_key(key):
+                raise TypeError(
+                    f"batch key at index {i} must be a str or an integer, got "
+                    f"{type(key).__name__}"
+                )
+            items.append((str(key), lc))
+        elif isinstance(entry, LightCurve):
+            key = str(entry.meta.get("id", i)) if entry.meta else str(i)
+            items.append((key, entry))
+        elif isinstance(entry, tuple) and len(entry) == 2 and isinstance(
+            entry[1], (str, os.PathLike)
+        ):
+            raise TypeError(
+                f"batch entry at index {i} names a file; this package reads light curves "
+                "from memory only"
+            )
+        else:
+            raise TypeError(f"unsupported batch entry at index {i}: {type(entry)!r}")
+    return _unique_keys(items)
+
+
+def _unique_keys(items: list[InputItem]) -> list[InputItem]:
+    """Repeated keys (compared as a CSV sink stores them) renamed ``key#n``."""
+    stored = [str(csv_safe(key)) for key, _ in items]
+    if all(n == 1 for n in Counter(stored).values()):
+        return items
+    taken = set(stored)
+    seen: Counter[str] = Counter()
+    out: list[InputItem] = []
+    renamed: list[str] = []
+    for (key, src), form in zip(items, stored):
+        seen[form] += 1
+        if seen[form] == 1:
+            out.append((key, src))
+            continue
+        n = seen[form]
+        while str(csv_safe(f"{key}#{n}")) in taken:
+            n += 1
+        new = f"{key}#{n}"
+        taken.add(str(csv_safe(new)))
+        out.append((new, src))
+        renamed.append(new)
+    warnings.warn(
+        f"{len(renamed)} batch keys occur more than once; the repeats were renamed "
+        f"({', '.join(repr(k) for k in renamed[:3])}) so each light curve keeps its own "
+        "result rows and resume entry.",
+        UserWarning,
+        stacklevel=3,
+    )
+    return out
+
+
+def periodogram_to_row(
+    key: str,
+    pg: Periodogram,
+    *,
+    n_best: int,
+    store_raw: bool,
+    extra_keys: Sequence[str] | None = None,
+    alias_diverse: bool | None = None,
+    source: str | None = None,
+) -> dict[str, Any]:
+    """One result row: identity columns, ``best_period``, then per-rank peaks."""
+    row: dict[str, Any] = {"key": key, "method": pg.method}
+    if source is not None:
+        row["source"] = source
+    row["backend"] = pg.backend
+    row["n_samples"] = pg.n_samples
+    row["baseline"] = pg.baseline
+    diverse = alias_diverse_for(pg.method) if alias_diverse is None else alias_diverse
+    peaks = pg.best_periods(max(int(n_best), 1), alias_diverse=diverse)
+    row["best_period"] = peaks[0].period if peaks else float("nan")
+    declared = tuple(extra_keys) if extra_keys is not None else ()
+    for rank in range(1, n_best + 1):
+        peak = peaks[rank - 1] if rank <= len(peaks) else None
+        row[f"period_{rank}"] = peak.period if peak else float("nan")
+        row[f"power_{rank}"] = peak.power if peak else float("nan")
+        extra = peak.extra if peak is not None else {}
+        for name in declared:
+            value = extra.get(name)
+            row[f"{name}_{rank}"] = float("nan") if value is None else float(value)
+        for name, value in extra.items():
+            if name not in declared:
+                row[f"{name}_{rank}"] = value
+    if store_raw:
+        freq, power = pg.downsample()
+        row["pgram_frequency"] = freq.tolist()
+        row["pgram_power"] = power.tolist()
+    return row
+
+
+def csv_safe(value: Any) -> Any:
+    """Quote-prefix a text cell starting with ``= + - @ TAB CR`` unless it is a number."""
+    if isinstance(value, str) and value and value[0] in _CSV_FORMULA_PREFIXES:
+        try:
+            float(value)
+        except ValueError:
+            return "'" + value
+    return value
+
+
+def _union_columns(rows: Sequence[Mapping[str, Any]]) -> list[str]:
+    seen: dict[str, None] = {}
+    for row in rows:
+        for name in row:
+            seen.setdefault(name, None)
+    return list(seen)
+
+
+def _temp_path(path: Path) -> Path:
+    return path.with_name(f".{path.name}.{os.getpid()}.{secrets.token_hex(4)}.tmp")
+
+
+def _move_into_place(tmp: Path, path: Path) -> None:
+    try:
+        os.replace(tmp, path)
+    except OSError as exc:
+        raise type(exc)(
+            f"could not replace {path} ({exc}); the complete result is kept in {tmp}"
+        ) from exc
+
+
+def _write_parquet(rows: Sequence[Mapping[str, Any]], path: Path) -> None:
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+
+    columns = _union_columns(rows)
+    table = pa.table({col: [row.get(col) for row in rows] for col in columns})
+    tmp = _temp_path(path)
+    try:
+        with tmp.open("xb") as fh:
+            pq.write_table(table, fh)
+    except BaseException:
+        tmp.unlink(missing_ok=True)
+        raise
+    _move_into_place(tmp, path)
+
+
+def _write_csv(rows: Sequence[Mapping[str, Any]], path: Path) -> None:
+    columns = _union_columns(rows)
+    lists = [c for c in columns if any(isinstance(r.get(c), list) for r in rows)]
+    if lists:
+        raise ValueError(
+            f"CSV sink cannot store array-valued columns {lists} (e.g. from store_raw); "
+            "use a .parquet sink."
+        )
+    tmp = _temp_path(path)
+    try:
+        with tmp.open("x", newline="", encoding="utf-8") as fh:
+            writer = csv.DictWriter(fh, fieldnames=columns, extrasaction="ignore")
+            writer.writeheader()
+            writer.writerows({c: csv_safe(v) for c, v in row.items()} for row in rows)
+    except BaseException:
+        tmp.unlink(missing_ok=True)
+        raise
+    _move_into_place(tmp, path)
+
+
+def write_rows(rows: Sequence[Mapping[str, Any]], path: Path) -> None:
+    """Write rows to ``path`` (``.parquet``/``.pq`` or ``.csv``) atomically."""
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    suffix = path.suffix.lower()
+    if suffix in {".parquet", ".pq"}:
+        _write_parquet(rows, path)
+    elif suffix == ".csv":
+        _write_csv(rows, path)
+    else:
+        raise ValueError(f"unsupported sink format {path.suffix!r}; use .parquet or .csv")
+
+
+def read_rows(path: Path) -> list[dict[str, Any]]:
+    """All rows of a ``.parquet`` or ``.csv`` file."""
+    path = Path(path)
+    if path.suffix.lower() in {".parquet", ".pq"}:
+        import pyarrow.parquet as pq
+
+        with path.open("rb") as fh:
+            return pq.read_table(fh).to_pylist()
+    with path.open(newline="", encoding="utf-8") as fh:
+        return [dict(row) for row in csv.DictReader(fh)]
+
+
+__all__ = [
+    "ALIAS_DIVERSE_METHODS",
+    "PEAK_EXTRA_COLUMNS",
+    "InputItem",
+    "alias_diverse_for",
+    "csv_safe",
+    "peak_extra_columns",
+    "periodogram_to_row",
+    "read_rows",
+    "resolve_inputs",
+    "source_label",
+    "write_rows",
+]

@@ -1,0 +1,665 @@
+"""``batch_periodograms``: BLS over many light curves, one launch per chunk.
+
+The driver of cuPeriod's batch API (src/cuperiod/batch/runner.py:151-185, 734-1032)
+with the same arguments, ``BatchSummary``, row format, sinks and resume rules, but a
+different execution model. cuPeriod's ``_process_chunk`` (``:214-250``) calls
+``method.power`` once per light curve, each with its own uploads, launches and a
+``(7, F)`` download; a GPU pool of processes hides part of that. Here every chunk of
+light curves is packed into one set of device arrays and searched by ONE batched engine
+call (:mod:`cuperiod_amd.batch.engine`), in this process; the next chunk is packed on
+the host while the device searches the current one. Each curve's rows are the rows that
+``periodogram(lc, "BLS")`` gives, bit for bit: the device regenerates each curve's
+default grid with numpy's recipe and the spectra are assembled exactly as the single
+curve path assembles them.
+
+Sinks (``:734-761``): ``None`` returns rows in memory; a ``.parquet``/``.csv`` file is
+rewritten at the end (merged with its rows when resuming); a directory gets one
+``part-NNNNN.parquet`` per chunk and a ``_manifest.json`` recording the request, so a
+resumed run that would write different rows is refused (``:546-693``). Resuming skips
+every ``(key, method)`` pair the sink already holds.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import json
+import re
+import time
+import warnings
+from collections.abc import Callable, Mapping, Sequence
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+
+from cuperiod_amd.batch.io import (
+    InputItem,
+    csv_safe,
+    peak_extra_columns,
+    periodogram_to_row,
+    read_rows,
+    resolve_inputs,
+    source_label,
+    write_rows,
+)
+from cuperiod_amd.core.columns import Domain, coerce_domain
+from cuperiod_amd.core.errors import InsufficientDataError
+from cuperiod_amd.core.grid import GridSpec
+from cuperiod_amd.core.lightcurve import LightCurve, raise_if_unusable
+from cuperiod_amd.core.result import Periodogram, peak_defaults_from_settings
+from cuperiod_amd.methods.base import (
+    get_method,
+    normalize_backend,
+    requested_backend,
+    select_settings,
+)
+
+#: Light curves per chunk (one engine call, one part file of a directory sink).
+DEFAULT_CHUNK_SIZE = 256
+
+_PART_RE = re.compile(r"^part-(\d{5,})\.parquet$")
+_STALE_PART_RE = re.compile(r"^part-\d{5,}\.parquet\.tmp$")
+_PART_TEMP_RE = re.compile(r"^\.part-\d{5,}\.parquet\.\d+\.[0-9a-f]+\.tmp$")
+
+#: Settings fields that choose hardware or batch sizes, not results: left out of a
+#: directory sink's fingerprint so a run can resume elsewhere.
+_RUNTIME_SETTINGS_FIELDS = frozenset(
+    {"backend", "device", "precision", "batch_periods", "accumulation"}
+)
+_FINGERPRINT_FIELDS = ("n_best", "store_raw", "columns", "domain", "band_column", "grid")
+
+
+@dataclass
+class BatchSummary:
+    """Outcome of :func:`batch_periodograms`.
+
+    ``n_done`` rows produced by this run, ``n_failed`` light curves that raised (their
+    ``(key, message)`` in ``errors``, input order), ``n_skipped`` light curves already
+    complete in a resumed sink; ``rows`` (input order) when ``sink is None``;
+    ``n_workers`` processes used (1: this one; 0: nothing to compute).
+    """
+
+    n_inputs: int
+    n_done: int
+    n_failed: int
+    n_skipped: int
+    methods: tuple[str, ...]
+    device: str
+    sink: str | None
+    errors: list[tuple[str, str]] = field(default_factory=list)
+    rows: list[dict[str, Any]] | None = None
+    n_workers: int | None = None
+
+
+@dataclass(frozen=True)
+class _ChunkConfig:
+    methods: tuple[str, ...]
+    backends: Mapping[str, str]
+    settings_map: Mapping[str, Any]
+    n_best: int
+    store_raw: bool
+    grid: GridSpec | None
+    extra_keys: tuple[str, ...]
+
+
+# --- backend choice ------------------------------------------------------------------
+
+
+def _worker_backend(device: str, backend: str | None, settings: Any) -> str:
+    """The backend selector for ``device``: ``"cpu"`` asks for a CPU backend (which
+    this package does not have, so resolving it raises), ``"gpu"`` maps ``"auto"`` to
+    ``"gpu"`` and refuses an explicit CPU backend."""
+    if device == "cpu":
+        explicit = requested_backend(backend, settings)
+        return "cpu" if explicit in ("auto", "gpu") else explicit
+    explicit = "auto" if backend is None else normalize_backend(backend)
+    if explicit == "auto":
+        pinned = requested_backend(None, settings)
+        return "gpu" if pinned == "auto" else pinned
+    if explicit in ("gpu", "hip"):
+        return explicit
+    raise ValueError(
+        f"backend {explicit!r} runs on the CPU, but device='gpu' asks for GPU workers; "
+        "use a GPU backend ('auto', 'gpu' or 'hip')"
+    )
+
+
+# --- sinks -----------------------------------------------------------------------------
+
+
+def _classify_sink(sink: str | Path | None) -> tuple[str, Path, Path]:
+    """``("memory"|"file"|"dir", directory, path)``; a file sink's directory is made."""
+    if sink is None:
+        return "memory", Path(), Path()
+    path = Path(sink)
+    suffix = path.suffix.lower()
+    if suffix in {".parquet", ".pq", ".csv"}:
+        if path.is_dir():
+            raise ValueError(
+                f"sink {str(sink)!r} is an existing directory, but its {suffix} suffix "
+                "asks for a single-file sink; pass a file path, or a directory path "
+                "without an extension for a resumable multi-part sink."
+            )
+        path.parent.mkdir(parents=True, exist_ok=True)
+        return "file", path.parent, path
+    if suffix and not path.is_dir():
+        raise ValueError(
+            f"unsupported sink {str(sink)!r}: a file sink must end in .parquet or .csv; "
+            "pass a directory (no extension) for a resumable multi-part sink."
+        )
+    return "dir", path, path
+
+
+def _part_path(sink_dir: Path, idx: int) -> Path:
+    return sink_dir / f"part-{idx:05d}.parquet"
+
+
+def _sink_key(value: Any, *, csv_sink: bool) -> str | None:
+    if value is None:
+        return None
+    text = str(value)
+    return str(csv_safe(text)) if csv_sink else text
+
+
+def _corrupt_errors() -> tuple[type[BaseException], ...]:
+    import csv
+
+    import pyarrow as pa
+
+    return (pa.ArrowInvalid, UnicodeDecodeError, csv.Error)
+
+
+def _quarantine(path: Path, *, hidden: bool) -> Path:
+    """Move an unreadable sink file aside (never delete it)."""
+    stamp = time.strftime("%Y%m%dT%H%M%S")
+    prefix = "." if hidden else ""
+    target = path.with_name(f"{prefix}{path.name}.corrupt-{stamp}")
+    n = 1
+    while target.exists():
+        target = path.with_name(f"{prefix}{path.name}.corrupt-{stamp}-{n}")
+        n += 1
+    path.replace(target)
+    return target
+
+
+def _done_pairs(rows: Sequence[Mapping[str, Any]], *, csv_sink: bool) -> dict:
+    done: dict[tuple[str | None, str | None], str] = {}
+    for row in rows:
+        ident = (_sink_key(row.get("key"), csv_sink=csv_sink),
+                 _sink_key(row.get("method"), csv_sink=csv_sink))
+        done.setdefault(ident, str(row.get("source") or ""))
+    return done
+
+
+def _file_sink_done(path: Path) -> dict:
+    """``(key, method)`` pairs an existing file sink holds; an unreadable file is
+    moved aside with a warning and counts as empty."""
+    if not path.exists():
+        return {}
+    try:
+        rows = read_rows(path)
+    except _corrupt_errors() as exc:
+        moved = _quarantine(path, hidden=False)
+        warnings.warn(
+            f"existing sink {path} is not a readable {path.suffix} file ({exc}); it was "
+            f"moved to {moved} and a new file will be written",
+            UserWarning,
+            stacklevel=3,
+        )
+        return {}
+    return _done_pairs(rows, csv_sink=path.suffix.lower() == ".csv")
+
+
+def _scan_parts(sink_dir: Path) -> tuple[dict, int]:
+    """Pairs a directory sink's parts hold, and the next free part index."""
+    done: dict = {}
+    next_index = 0
+    if not sink_dir.is_dir():
+        return done, next_index
+    for entry in sorted(sink_dir.iterdir()):
+        if _STALE_PART_RE.match(entry.name):
+            entry.unlink(missing_ok=True)
+            continue
+        match = _PART_RE.match(entry.name)
+        if match is None:
+            continue
+        next_index = max(next_index, int(match.group(1)) + 1)
+        try:
+            rows = read_rows(entry)
+        except _corrupt_errors() as exc:
+            moved = _quarantine(entry, hidden=True)
+            warnings.warn(
+                f"part file {entry} is unreadable ({exc}); it was moved to {moved.name} "
+                "and its light curves will be computed again",
+                UserWarning,
+                stacklevel=3,
+            )
+            continue
+        for ident, src in _done_pairs(rows, csv_sink=False).items():
+            done.setdefault(ident, src)
+    return done, next_index
+
+
+def _purge_parts(sink_dir: Path) -> None:
+    if not sink_dir.is_dir():
+        return
+    for entry in sink_dir.iterdir():
+        name = entry.name
+        if (_PART_RE.match(name) or _STALE_PART_RE.match(name)
+                or _PART_TEMP_RE.match(name)) and entry.is_file():
+            entry.unlink()
+
+
+def _grid_fingerprint(grid: GridSpec | None) -> dict[str, Any] | None:
+    if grid is None:
+        return None
+    digest = hashlib.sha256(np.ascontiguousarray(grid.values).tobytes()).hexdigest()[:16]
+    return {"kind": str(grid.kind), "size": int(grid.values.size), "sha256": digest}
+
+
+def _run_fingerprint(methods, settings_map, *, n_best, store_raw, columns, domain,
+                     band_column, grid, extra_keys) -> dict[str, Any]:
+    raw = {
+        "n_best": int(n_best),
+        "store_raw": bool(store_raw),
+        "columns": None if columns is None else str(columns),
+        "domain": domain.value if domain is not None else None,
+        "band_column": band_column,
+        "grid": _grid_fingerprint(grid),
+        "extra_keys": list(extra_keys),
+        "settings": {
+            name: settings_map[name].model_dump(mode="json",
+                                                exclude=set(_RUNTIME_SETTINGS_FIELDS))
+            for name in methods
+        },
+    }
+    return json.loads(json.dumps(raw))
+
+
+def _merge_config(stored: Mapping[str, Any], current: Mapping[str, Any],
+                  sink_dir: Path) -> dict[str, Any]:
+    """Refuse a resumed request whose rows would differ from the directory's; a new
+    method is recorded."""
+    changed = [f for f in _FINGERPRINT_FIELDS if f in stored and stored[f] != current.get(f)]
+    extra = list(current.get("extra_keys", []))
+    old_extra = stored.get("extra_keys")
+    if isinstance(old_extra, list):
+        added = [k for k in extra if k not in old_extra]
+        if added:
+            changed.append(
+                f"new per-peak columns {added} that the existing part files lack; write "
+                "that method to a directory of its own"
+            )
+        extra = list(old_extra)
+    old_settings = stored.get("settings")
+    old_settings = old_settings if isinstance(old_settings, Mapping) else {}
+    for name, new in current.get("settings", {}).items():
+        old = old_settings.get(name)
+        if not isinstance(old, Mapping):
+            continue
+        diff = sorted(k for k in set(old) & set(new) if old[k] != new[k])
+        if diff:
+            changed.append(f"{name} settings ({', '.join(diff)})")
+    if changed:
+        raise ValueError(
+            f"directory sink {sink_dir} was written with a different request "
+            f"({'; '.join(changed)}); resuming would mix incompatible rows. Repeat the "
+            "original request, write to a fresh directory, or pass resume=False to "
+            "replace the directory's part files."
+        )
+    return {**current, "extra_keys": extra,
+            "settings": {**old_settings, **current.get("settings", {})}}
+
+
+def _dir_manifest_guard(sink_dir: Path, resume: bool, config: Mapping[str, Any]) -> dict:
+    """Check and write a directory sink's ``_manifest.json`` before any work."""
+    manifest = sink_dir / "_manifest.json"
+    stored: dict[str, Any] = {}
+    if manifest.exists():
+        try:
+            loaded = json.loads(manifest.read_text(encoding="utf-8"))
+            stored = loaded if isinstance(loaded, dict) else {}
+        except Exception:  # noqa: BLE001 - a corrupt manifest must not abort the run
+            stored = {}
+    if not resume:
+        _purge_parts(sink_dir)
+    elif sink_dir.is_dir() and any(_PART_RE.match(e.name) for e in sink_dir.iterdir()):
+        if isinstance(stored.get("config"), Mapping):
+            config = _merge_config(stored["config"], config, sink_dir)
+    sink_dir.mkdir(parents=True, exist_ok=True)
+    record = {"config": dict(config)}
+    manifest.write_text(json.dumps(record, indent=1, sort_keys=True), encoding="utf-8")
+    return record
+
+
+def _finalize_file(rows: list[dict[str, Any]], path: Path, resume: bool) -> None:
+    """Write a file sink; when resuming, keep its rows and add only new pairs."""
+    if not (resume and path.exists()):
+        write_rows(rows, path)
+        return
+    if not rows:
+        return
+    try:
+        existing = read_rows(path)
+    except FileNotFoundError:
+        existing = []
+    except _corrupt_errors() as exc:
+        moved = _quarantine(path, hidden=False)
+        warnings.warn(
+            f"could not read existing sink {path} ({exc}); it was moved to {moved} and a "
+            "new file is written",
+            UserWarning,
+            stacklevel=3,
+        )
+        existing = []
+    if not existing:
+        write_rows(rows, path)
+        return
+    csv_sink = path.suffix.lower() == ".csv"
+    seen = set(_done_pairs(existing, csv_sink=csv_sink))
+    new = [r for r in rows if (_sink_key(r.get("key"), csv_sink=csv_sink),
+                               _sink_key(r.get("method"), csv_sink=csv_sink)) not in seen]
+    if new:
+        write_rows(existing + new, path)
+
+
+# --- the batched search ------------------------------------------------------------
+
+
+@dataclass
+class _Prepared:
+    """A light curve that passed the per-curve checks, ready to pack."""
+
+    position: int
+    key: str
+    label: str
+    finite: LightCurve
+
+
+def _prepare(position: int, key: str, lc: LightCurve, settings: Any,
+             grid: GridSpec | None) -> _Prepared:
+    """The checks ``method.power`` makes before searching one light curve
+    (src/cuperiod/methods/bls.py:371-389), raising the same errors."""
+    from cuperiod_amd.batch import engine
+    from cuperiod_amd.methods.bls import _grid_segments
+
+    raise_if_unusable(lc, "BLS")
+    single = lc.in_domain(Domain.FLUX)
+    finite = single.finite()
+    if finite.n < settings.min_detections:
+        raise InsufficientDataError(
+            f"BLS: {finite.n} finite points < min_detections {settings.min_detections}"
+        )
+    if finite.baseline <= 0.0:
+        raise InsufficientDataError("BLS: no usable time baseline")
+    if grid is None:
+        plan = engine._default_plan(0, finite.baseline, finite.time, settings)
+        if not plan.segments:
+            raise InsufficientDataError(
+                "BLS: baseline too short for the configured period range"
+            )
+    elif not _grid_segments(grid, settings):
+        raise InsufficientDataError("BLS: empty trial grid")
+    return _Prepared(position, key, source_label(lc), finite)
+
+
+def _curve_arrays(lc: LightCurve) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    err = lc.error if lc.error is not None else np.ones_like(lc.value)
+    return lc.time, lc.value, err
+
+
+def _periodogram(prep: _Prepared, periods: np.ndarray, spectrum: Mapping[str, np.ndarray],
+                 settings: Any) -> Periodogram:
+    """The single-curve assembly (``_assemble``) of one curve's batched spectrum."""
+    from cuperiod_amd.methods.bls import _assemble
+
+    chunks = {
+        "period": [periods],
+        "power": [spectrum["power"]],
+        "depth": [spectrum["depth"]],
+        "depth_snr": [spectrum["depth_snr"]],
+        "duration": [spectrum["duration"]],
+        "transit_time": [spectrum["transit_time"]],
+    }
+    pg = _assemble(chunks, prep.finite.n, prep.finite.baseline, "hip", prep.finite.meta)
+    return pg.with_peak_defaults(peak_defaults_from_settings(settings))
+
+
+def _launch(prepared: Sequence[_Prepared], settings: Any, grid: GridSpec | None) -> Any:
+    """Pack the curves and enqueue one batched search (returns without waiting)."""
+    from cuperiod_amd.batch import engine
+
+    pb = engine.pack([_curve_arrays(p.finite) for p in prepared],
+                     [grid] * len(prepared), settings)
+    out, summary = engine.run_packed(pb)
+    return pb, out, summary
+
+
+def _collect(handle: Any) -> list[tuple[np.ndarray, dict[str, np.ndarray]]]:
+    """Wait for a launch; each curve's ``(periods, spectrum)``, ascending period."""
+    from cuperiod_amd.batch import engine
+
+    pb, out, summary = handle
+    res = engine.finish(pb, out, summary, spectra=True)
+    return [(pb.plans[c].periods(), res.spectra[c]) for c in range(len(pb.plans))]
+
+
+class _ChunkRunner:
+    """Packs and launches a chunk, then turns the previous launch into rows, so the
+    host prepares chunk k+1 while the device searches chunk k."""
+
+    def __init__(self, absorb: Callable[..., None]) -> None:
+        self.absorb = absorb
+        self.pending: tuple | None = None
+
+    def submit(self, idx: int, items: Sequence[InputItem], cfg: _ChunkConfig) -> None:
+        launches = []
+        errors: dict[int, tuple[str, str]] = {}
+        for name in cfg.methods:
+            settings = cfg.settings_map[name]
+            prepared: list[_Prepared] = []
+            for pos, (key, lc) in enumerate(items):
+                if pos in errors:
+                    continue
+                try:
+                    prepared.append(_prepare(pos, key, lc, settings, cfg.grid))
+                except Exception as exc:  # noqa: BLE001 - one bad light curve
+                    errors[pos] = (key, f"{type(exc).__name__}: {exc}")
+            handle = _launch(prepared, settings, cfg.grid) if prepared else None
+            launches.append((settings, prepared, handle))
+        self._complete()
+        self.pending = (idx, cfg, launches, errors)
+
+    def _complete(self) -> None:
+        if self.pending is None:
+            return
+        idx, cfg, launches, errors = self.pending
+        self.pending = None
+        by_pos: dict[int, list[dict[str, Any]]] = {}
+        for settings, prepared, handle in launches:
+            if handle is None:
+                continue
+            for prep, (periods, spectrum) in zip(prepared, _collect(handle)):
+                if prep.position in errors:
+                    continue
+                try:
+                    pg = _periodogram(prep, periods, spectrum, settings)
+                    row = periodogram_to_row(prep.key, pg, n_best=cfg.n_best,
+                                             store_raw=cfg.store_raw,
+                                             extra_keys=cfg.extra_keys, source=prep.label)
+                except Exception as exc:  # noqa: BLE001 - recorded per light curve
+                    errors[prep.position] = (prep.key, f"{type(exc).__name__}: {exc}")
+                    by_pos.pop(prep.position, None)
+                    continue
+                by_pos.setdefault(prep.position, []).append(row)
+        rows = [r for pos in sorted(by_pos) if pos not in errors for r in by_pos[pos]]
+        errs = [errors[pos] for pos in sorted(errors)]
+        self.absorb(idx, rows, errs)
+
+    def drain(self) -> None:
+        self._complete()
+
+
+def _chunked(items: Sequence[Any], size: int) -> list[list[Any]]:
+    return [list(items[i:i + size]) for i in range(0, len(items), size)]
+
+
+def batch_periodograms(
+    inputs: Any,
+    method: str | Sequence[str] = "BLS",
+    *,
+    backend: str | None = "auto",
+    device: str = "gpu",
+    workers: int | None = None,
+    grid: GridSpec | None = None,
+    settings: Any = None,
+    columns: Any = None,
+    domain: Domain | str | None = None,
+    band_column: str | None = None,
+    sink: str | Path | None = None,
+    n_best: int = 10,
+    store_raw: bool = False,
+    resume: bool = True,
+    chunk_size: int | None = None,
+) -> BatchSummary:
+    """Periodograms of many light curves; one row per light curve and method.
+
+    ``inputs`` is an iterable of :class:`LightCurve` objects or ``(key, lc)`` pairs.
+    ``device="gpu"`` (the default here) runs the HIP engine in this process, one engine
+    call per chunk of ``chunk_size`` light curves (default 256); ``device="cpu"``
+    raises :class:`BackendUnavailableError`, since this package has no CPU backend.
+    ``workers`` is accepted for signature compatibility: the batched engine fills the
+    device from one process. ``grid``, ``settings``, ``sink``, ``n_best``,
+    ``store_raw``, ``resume`` and ``chunk_size`` mean what they mean in cuPeriod
+    (src/cuperiod/batch/runner.py:764-859). ``columns`` and ``domain`` only concern
+    file and table inputs, which this package does not read; ``band_column``
+    (multi-band input) is not supported.
+
+    Raises
+    ------
+    ValueError
+        Unknown device, a CPU backend with ``device="gpu"``, an unsupported sink, or a
+        directory sink written by a different request.
+    BackendUnavailableError
+        When the HIP backend cannot run here (checked before any work).
+    """
+    if device not in {"cpu", "gpu", "hybrid"}:
+        raise ValueError("device must be 'cpu', 'gpu', or 'hybrid'")
+    if device == "hybrid":
+        raise NotImplementedError("hybrid device is planned; use device='gpu'")
+    if grid is not None and not isinstance(grid, GridSpec):
+        raise TypeError(f"grid must be a GridSpec or None, got {type(grid).__name__}")
+    if chunk_size is not None and int(chunk_size) < 1:
+        raise ValueError(f"chunk_size must be a positive integer, got {chunk_size}")
+    if band_column is not None:
+        raise NotImplementedError("multi-band batches are not supported by this package")
+    domain = coerce_domain(domain)
+
+    names = (method,) if isinstance(method, str) else tuple(method)
+    resolved = [get_method(m) for m in names]
+    method_objs = list({m.name: m for m in resolved}.values())
+    if not method_objs:
+        raise ValueError("no methods specified")
+    methods = tuple(m.name for m in method_objs)
+    settings_map = select_settings(method_objs, settings)
+    backends = {n: _worker_backend(device, backend, settings_map[n]) for n in methods}
+    for m in method_objs:
+        m.resolve_backend_for(backends[m.name], settings_map[m.name])
+
+    sink_kind, sink_dir, sink_file = _classify_sink(sink)
+    if store_raw and sink_kind == "file" and sink_file.suffix.lower() == ".csv":
+        raise ValueError(
+            "store_raw=True produces array-valued spectrum columns that a CSV sink cannot "
+            "hold; use a .parquet sink or a directory sink."
+        )
+    items = resolve_inputs(inputs)
+    summary_sink = str(sink) if sink is not None else None
+    if not items:
+        if sink is not None:
+            warnings.warn(
+                "batch_periodograms: no inputs to process; the sink was left untouched",
+                UserWarning,
+                stacklevel=2,
+            )
+        return BatchSummary(0, 0, 0, 0, methods, device, summary_sink,
+                            rows=[] if sink_kind == "memory" else None, n_workers=0)
+
+    extra_keys = peak_extra_columns(methods)
+    cfg = _ChunkConfig(methods, backends, settings_map, int(n_best), bool(store_raw),
+                       grid, extra_keys)
+    done: dict = {}
+    next_index = 0
+    csv_sink = sink_kind == "file" and sink_file.suffix.lower() == ".csv"
+    if sink_kind == "dir":
+        fingerprint = _run_fingerprint(
+            methods, settings_map, n_best=n_best, store_raw=store_raw, columns=columns,
+            domain=domain, band_column=band_column, grid=grid, extra_keys=extra_keys,
+        )
+        record = _dir_manifest_guard(sink_dir, resume, fingerprint)
+        cfg = dataclasses.replace(cfg, extra_keys=tuple(record["config"]["extra_keys"]))
+        if resume:
+            done, next_index = _scan_parts(sink_dir)
+    elif sink_kind == "file" and resume:
+        done = _file_sink_done(sink_file)
+
+    groups: dict[tuple[str, ...], list[InputItem]] = {}
+    n_skipped = 0
+    for key, src in items:
+        lookup = _sink_key(key, csv_sink=csv_sink)
+        missing = tuple(n for n in methods if (lookup, n) not in done)
+        if missing:
+            groups.setdefault(missing, []).append((key, src))
+        else:
+            n_skipped += 1
+
+    size = int(chunk_size) if chunk_size is not None else DEFAULT_CHUNK_SIZE
+    chunks: dict[int, list[InputItem]] = {}
+    configs: dict[int, _ChunkConfig] = {}
+    idx = next_index
+    for missing, group in groups.items():
+        group_cfg = cfg if missing == methods else dataclasses.replace(cfg, methods=missing)
+        for chunk in _chunked(group, size):
+            chunks[idx] = chunk
+            configs[idx] = group_cfg
+            idx += 1
+
+    rows_by_chunk: dict[int, list[dict[str, Any]]] = {}
+    errors_by_chunk: dict[int, list[tuple[str, str]]] = {}
+    n_done = 0
+
+    def absorb(i: int, rows: list[dict[str, Any]], errs: list[tuple[str, str]]) -> None:
+        nonlocal n_done
+        n_done += len(rows)
+        errors_by_chunk[i] = errs
+        if sink_kind == "dir":
+            if rows:
+                write_rows(rows, _part_path(sink_dir, i))
+        else:
+            rows_by_chunk[i] = rows
+
+    runner = _ChunkRunner(absorb)
+    for i in sorted(chunks):
+        runner.submit(i, chunks[i], configs[i])
+    runner.drain()
+
+    all_rows = [row for i in sorted(rows_by_chunk) for row in rows_by_chunk[i]]
+    errors = [err for i in sorted(errors_by_chunk) for err in errors_by_chunk[i]]
+    if sink_kind == "file":
+        _finalize_file(all_rows, sink_file, resume)
+    return BatchSummary(
+        n_inputs=len(items),
+        n_done=n_done,
+        n_failed=len(errors),
+        n_skipped=n_skipped,
+        methods=methods,
+        device=device,
+        sink=summary_sink,
+        errors=errors,
+        rows=None if sink_kind != "memory" else all_rows,
+        n_workers=1 if chunks else 0,
+    )
+
+
+__all__ = ["DEFAULT_CHUNK_SIZE", "BatchSummary", "batch_periodograms"]
