@@ -113,6 +113,47 @@ def cpu_baseline(wl, settings, n_curves: int):
     return elapsed, threads, np.asarray(best), powers
 
 
+def learned_leg(pb, wl, settings, plain, plain_s: float, steps: int) -> dict:
+    """The learned proposer + refinement on the same batch, against the plain search:
+    speed, and agreement of the best period (identical, or the same best power, i.e. an
+    exact tie the plain search resolved to another sample)."""
+    import torch
+
+    from cuperiod_amd.learned.proposer import DEFAULT_WEIGHTS, LearnedProposer
+
+    prop = LearnedProposer()
+    t0 = time.perf_counter()
+    prep = prop.prepare(pb.curve_set, wl.grids, settings, full=pb.plans)
+    prep_s = time.perf_counter() - t0
+    prop.search(prep, settings)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = prop.search(prep, settings)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    identical = r.best_period == plain.best_period
+    same_power = r.best_power == plain.best_power
+    keep = ~r.fallback
+    return {
+        "value": round(len(pb.plans) / el, 3),
+        "unit": "light curves/s",
+        "ms_per_step": round(1000.0 * el, 3),
+        "speedup_vs_plain": round(plain_s / el, 3),
+        "best_period_match": float(np.mean(identical | same_power)),
+        "best_period_identical": float(np.mean(identical)),
+        "fallback_frac": float(np.mean(r.fallback)),
+        "match_without_fallback": float(np.mean((identical | same_power)[keep])) if keep.any() else None,
+        "fine_freqs_per_refined_curve": float(np.mean(r.n_fine[keep])) if keep.any() else None,
+        "coarse_freqs": 4096,
+        "windows": {"max": prop.max_windows, "mass": prop.window_mass,
+                    "threshold": prop.threshold, "fine_per_window": prop.fine_per_window},
+        "timings_s": {k: round(v, 4) for k, v in r.timings.items()},
+        "prepare_s": round(prep_s, 3),
+        "weights": DEFAULT_WEIGHTS.name,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +163,8 @@ def main() -> int:
     ap.add_argument("--freqs", type=int, default=200_000)
     ap.add_argument("--cpu-curves", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned proposer + refinement leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 256 MiB device copy (the HBM counter scale)")
     args = ap.parse_args()
@@ -223,6 +266,14 @@ def main() -> int:
         "learned": None,
         "cpu_baseline": None,
     }
+    from cuperiod_amd.learned.proposer import DEFAULT_WEIGHTS
+
+    if not args.no_learned and DEFAULT_WEIGHTS.exists():
+        log("learned proposer + refinement ...")
+        line["learned"] = learned_leg(pb, wl, settings, res, elapsed / args.steps,
+                                      max(1, min(args.steps, 2)))
+        log(f"learned: {line['learned']['value']} curves/s, match "
+            f"{line['learned']['best_period_match']:.4f}")
     if not args.no_cpu and args.cpu_curves > 0:
         n = min(args.cpu_curves, len(pb.plans))
         log(f"CPU oracle on {n} curves ...")

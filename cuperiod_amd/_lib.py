@@ -44,7 +44,21 @@ EXPORTED_SYMBOLS = (
     "cpb_linspace_grid",
     "cpb_set_timing",
     "cpb_main_kernel_ms",
+    "cpb_set_accumulation",
     "cpb_synchronize",
+)
+
+#: Every symbol the proposer header (include/cuperiod_proposer.h) declares.
+PROPOSER_SYMBOLS = (
+    "cpp_features",
+    "cpp_layer1_forward",
+    "cpp_conv_forward",
+    "cpp_conv_wgrad",
+    "cpp_layer1_wgrad",
+    "cpp_flip_weights",
+    "cpp_head_train",
+    "cpp_head_windows",
+    "cpp_adam",
 )
 
 
@@ -119,8 +133,26 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.cpb_set_timing.argtypes = [vp, ctypes.c_int]
     lib.cpb_main_kernel_ms.restype = ctypes.c_int
     lib.cpb_main_kernel_ms.argtypes = [vp, ctypes.POINTER(f64)]
+    lib.cpb_set_accumulation.restype = ctypes.c_int
+    lib.cpb_set_accumulation.argtypes = [vp, ctypes.c_int]
     lib.cpb_synchronize.restype = ctypes.c_int
     lib.cpb_synchronize.argtypes = [vp, vp]
+    f32 = ctypes.c_float
+    proposer = {
+        "cpp_features": [vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, vp],
+        "cpp_layer1_forward": [vp, vp, vp, vp, i32, vp, vp],
+        "cpp_conv_forward": [vp, vp, vp, vp, vp, vp, vp, i32, i32],
+        "cpp_conv_wgrad": [vp, vp, vp, i32, vp, vp],
+        "cpp_layer1_wgrad": [vp, vp, vp, i32, vp, vp],
+        "cpp_flip_weights": [vp, vp, vp],
+        "cpp_head_train": [vp, vp, vp, vp, vp, vp, i32, f32, vp, vp, vp, vp],
+        "cpp_head_windows": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp],
+        "cpp_adam": [vp, vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32],
+    }
+    for name, args in proposer.items():
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = args
 
 
 def load_library() -> ctypes.CDLL:
@@ -230,6 +262,7 @@ def context(device: int = 0) -> Context:
 
 __all__ = [
     "EXPORTED_SYMBOLS",
+    "PROPOSER_SYMBOLS",
     "LIB_PATH",
     "N_OUTPUTS",
     "Context",

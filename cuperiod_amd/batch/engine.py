@@ -257,6 +257,24 @@ class BatchResult:
 
 
 @dataclass
+class CurveSet:
+    """A chunk's light curves on the device: FP64 ``tau = t - t_min``, ``y*ivar`` and
+    ``ivar`` concatenated, plus the per-curve descriptors the engine reads."""
+
+    tau: Any
+    yw: Any
+    ivar: Any
+    total_points: int
+    curves: Any
+    curves32: Any
+    t_min: np.ndarray
+    baseline: np.ndarray
+    n_points: np.ndarray
+    point_base: np.ndarray
+    times: list[np.ndarray]
+
+
+@dataclass
 class PackedBatch:
     """Device-resident inputs and descriptors of one chunk of curves."""
 
@@ -277,62 +295,108 @@ class PackedBatch:
     plans: list[CurvePlan]
     oversample: int
     obj_flag: int
+    curve_set: CurveSet | None = None
+    accumulation: int = 0
 
 
-def pack(
-    lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]],
-    grids: Sequence[LinspaceGrid | GridSpec | None],
-    settings: BLSSettings,
-) -> PackedBatch:
-    """Host preparation of a chunk: median-centred flux, ``ivar``, time origins, the
-    floors, and every curve's segment plan; inputs uploaded once."""
+def _device():
     import torch
 
     ctx = _lib.context(0)
     if not torch.cuda.is_available():
         raise BackendUnavailableError("BLS batch: torch sees no HIP device")
-    device = torch.device("cuda", ctx.device)
-    oversample = int(settings.bins_per_duration)
-    obj_flag = core._objective_flag(settings.objective)
-    taus, yws, ivs = [], [], []
-    curves = (_lib.Curve * len(lcs))()
-    plans: list[CurvePlan] = []
+    return torch.device("cuda", ctx.device)
+
+
+def _upload(a: np.ndarray, dtype, device) -> Any:
+    import torch
+
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.size == 0:
+        a = np.zeros(1, dtype=dtype)
+    return torch.from_numpy(a).to(device)
+
+
+def pack_curves(lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]]) -> CurveSet:
+    """Host preparation (median-centred flux, ``ivar``, time origin, floor) and one
+    upload of a chunk's curves ``(t, flux, dy)``."""
+    device = _device()
+    taus, yws, ivs, times = [], [], [], []
+    curves = (_lib.Curve * max(1, len(lcs)))()
+    curves32 = (_lib.Curve * max(1, len(lcs)))()
+    n = len(lcs)
+    t_min = np.zeros(n)
+    baseline = np.zeros(n)
+    n_points = np.zeros(n, dtype=np.int64)
+    point_base = np.zeros(n, dtype=np.int64)
     off = 0
-    for c, ((t, y, dy), grid) in enumerate(zip(lcs, grids)):
+    for c, (t, y, dy) in enumerate(lcs):
         t_h, y_h, iv_h = core._host_inputs(t, y, dy)
         if t_h.size == 0:
             raise InsufficientDataError("BLS batch: empty light curve")
-        t_min = float(t_h.min())
-        taus.append(t_h - t_min)
+        t_min[c] = float(t_h.min())
+        baseline[c] = float(t_h.max() - t_min[c])
+        taus.append(t_h - t_min[c])
         yws.append(y_h * iv_h)
         ivs.append(iv_h)
+        times.append(t_h)
         cv = curves[c]
         cv.point_offset = off
         cv.n_points = int(t_h.size)
-        cv.t_min = t_min
+        cv.t_min = t_min[c]
         cv.ivar_floor = core._ivar_floor(int(t_h.size), float(iv_h.sum()))
+        c32 = curves32[c]
+        c32.point_offset, c32.n_points, c32.t_min = cv.point_offset, cv.n_points, cv.t_min
+        c32.ivar_floor = core._ivar_floor(int(t_h.size), float(iv_h.sum()), float32=True)
+        n_points[c] = t_h.size
+        point_base[c] = off
         off += int(t_h.size)
-        if grid is None:
-            plan = _default_plan(c, float(t_h.max() - t_min), t_h, settings)
-        elif isinstance(grid, LinspaceGrid):
-            plan = _linspace_plan(c, grid, settings)
-        else:
-            plan = _explicit_plan(c, grid, settings)
-        plans.append(plan)
+    return CurveSet(
+        tau=_upload(np.concatenate(taus) if taus else np.zeros(0), np.float64, device),
+        yw=_upload(np.concatenate(yws) if yws else np.zeros(0), np.float64, device),
+        ivar=_upload(np.concatenate(ivs) if ivs else np.zeros(0), np.float64, device),
+        total_points=off, curves=curves, curves32=curves32, t_min=t_min, baseline=baseline, n_points=n_points,
+        point_base=point_base, times=times,
+    )
+
+
+def plan_grid(cs: CurveSet, c: int, grid: LinspaceGrid | GridSpec | None,
+              settings: BLSSettings) -> CurvePlan:
+    """The segment plan of curve ``c`` on ``grid`` (``None``: its default grid)."""
+    if grid is None:
+        return _default_plan(c, float(cs.baseline[c]), cs.times[c], settings)
+    if isinstance(grid, LinspaceGrid):
+        return _linspace_plan(c, grid, settings)
+    return _explicit_plan(c, grid, settings)
+
+
+def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) -> PackedBatch:
+    """Segment descriptors of ``plans`` (one per curve of ``cs``, in order) over the
+    uploaded curves."""
+    device = _device()
+    oversample = int(settings.bins_per_duration)
+    obj_flag = core._objective_flag(settings.objective)
+    accumulation = core._accumulation_code(settings.accumulation, settings.precision)
+    plans = list(plans)
     seg_list = [sg for plan in plans for sg in plan.segments]
     segs = (_lib.Segment * max(1, len(seg_list)))()
     dur_parts: list[np.ndarray] = []
     per_parts: list[np.ndarray] = []
     out_off = p_off = d_off = 0
-    curve_base = np.zeros(len(lcs), dtype=np.int64)
-    curve_count = np.zeros(len(lcs), dtype=np.int64)
+    curve_base = np.zeros(len(plans), dtype=np.int64)
+    curve_count = np.zeros(len(plans), dtype=np.int64)
+    dur_cache: dict[tuple[float, bytes], np.ndarray] = {}
     j = 0
     for c, plan in enumerate(plans):
         curve_base[c] = out_off
         curve_count[c] = plan.n_periods
         for sg in plan.segments:
             bd = float(sg.durations.min()) / oversample
-            db = np.asarray(core._duration_bins(sg.durations, bd), dtype=np.int32)
+            key = (bd, sg.durations.tobytes())
+            db = dur_cache.get(key)
+            if db is None:
+                db = np.asarray(core._duration_bins(sg.durations, bd), dtype=np.int32)
+                dur_cache[key] = db
             d = segs[j]
             d.bin_duration = bd
             d.n_periods = sg.n_periods
@@ -352,32 +416,53 @@ def pack(
             d_off += int(db.size)
             out_off += sg.n_periods
             j += 1
-
-    def up(a: np.ndarray, dtype) -> Any:
-        a = np.ascontiguousarray(a, dtype=dtype)
-        if a.size == 0:
-            a = np.zeros(1, dtype=dtype)
-        return torch.from_numpy(a).to(device)
-
     return PackedBatch(
-        tau=up(np.concatenate(taus), np.float64),
-        yw=up(np.concatenate(yws), np.float64),
-        ivar=up(np.concatenate(ivs), np.float64),
-        total_points=off,
-        curves=curves,
-        segments=segs,
-        n_segments=len(seg_list),
-        periods=up(np.concatenate(per_parts) if per_parts else np.zeros(1), np.float64),
-        dur_bins=up(np.concatenate(dur_parts) if dur_parts else np.zeros(1), np.int32),
-        n_out=out_off,
-        curve_base=curve_base,
-        curve_count=curve_count,
-        curve_base_d=up(curve_base, np.int64),
-        curve_count_d=up(curve_count, np.int64),
-        plans=plans,
-        oversample=oversample,
-        obj_flag=obj_flag,
+        tau=cs.tau, yw=cs.yw, ivar=cs.ivar, total_points=cs.total_points,
+        curves=cs.curves32 if accumulation else cs.curves,
+        segments=segs, n_segments=len(seg_list),
+        periods=_upload(np.concatenate(per_parts) if per_parts else np.zeros(1), np.float64, device),
+        dur_bins=_upload(np.concatenate(dur_parts) if dur_parts else np.zeros(1), np.int32, device),
+        n_out=out_off, curve_base=curve_base, curve_count=curve_count,
+        curve_base_d=_upload(curve_base, np.int64, device),
+        curve_count_d=_upload(curve_count, np.int64, device),
+        plans=plans, oversample=oversample, obj_flag=obj_flag, curve_set=cs,
+        accumulation=accumulation,
     )
+
+
+def pack(
+    lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]],
+    grids: Sequence[LinspaceGrid | GridSpec | None],
+    settings: BLSSettings,
+) -> PackedBatch:
+    """Host preparation of a chunk: curves uploaded once, every curve's segment plan."""
+    cs = pack_curves(lcs)
+    plans = [plan_grid(cs, c, grid, settings) for c, grid in enumerate(grids)]
+    return pack_plans(cs, plans, settings)
+
+
+def window_plan(base: CurvePlan, ranges: Sequence[tuple[int, int]]) -> CurvePlan:
+    """The part of ``base`` (a plan of parametric segments) at ascending-period indices
+    in the disjoint, sorted ``ranges`` ``[k0, k1)``. Every kept period keeps its
+    segment's durations and binning, so its outputs equal the full search's."""
+    plan = CurvePlan()
+    start = 0
+    for sg in base.segments:
+        if sg.mode == 0:
+            raise ValueError("window_plan needs parametric segments")
+        lo, hi = start, start + sg.n_periods
+        for a, b in ranges:
+            a, b = max(a, lo), min(b, hi)
+            if a >= b:
+                continue
+            plan.segments.append(
+                _SegPlan(curve=sg.curve, n_periods=b - a, durations=sg.durations, p_max=sg.p_max,
+                         mode=sg.mode, lin_start=sg.lin_start, lin_step=sg.lin_step,
+                         lin_stop=sg.lin_stop, lin_num=sg.lin_num, lin_k0=sg.lin_k0 + (a - lo))
+            )
+            plan.n_periods += b - a
+        start = hi
+    return plan
 
 
 def _vp(t) -> ctypes.c_void_p:
@@ -397,6 +482,7 @@ def run_packed(pb: PackedBatch, *, out=None, summary=None):
     if summary is None:
         summary = torch.empty((len(pb.plans), 4), dtype=torch.float64, device=device)
     with ctx.lock:
+        ctx.check(ctx.lib.cpb_set_accumulation(ctx.handle, pb.accumulation), "accumulation")
         st = ctx.lib.cpb_bls_search_batch(
             ctx.handle, stream, _vp(pb.tau), _vp(pb.yw), _vp(pb.ivar), pb.total_points,
             pb.curves, len(pb.plans), _vp(pb.periods), _vp(pb.dur_bins), pb.segments,
@@ -464,4 +550,5 @@ def bls_batch(
     return finish(pb, out, summary, spectra=spectra)
 
 
-__all__ = ["BatchResult", "LinspaceGrid", "bls_batch", "pack", "run_packed", "finish"]
+__all__ = ["BatchResult", "CurvePlan", "CurveSet", "LinspaceGrid", "PackedBatch", "bls_batch",
+           "finish", "pack", "pack_curves", "pack_plans", "plan_grid", "run_packed", "window_plan"]

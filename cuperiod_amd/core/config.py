@@ -7,8 +7,9 @@ allows; an explicit constructor argument wins over the environment.
 
 One field is new: ``accumulation`` selects how the HIP fold accumulates phase bins
 (``"fixed64"``: exact 64-bit fixed point, the default; ``"kahan32"``: FP32 bins with
-TwoSum compensation; ``"fp32"``: plain FP32 with an FP32 phase, the precision
-ablation). ``backend`` accepts ``"hip"``, this engine's backend name.
+TwoSum-compensated atomic adds and double-float prefix sums; ``"fp32"``: plain FP32
+bins, the precision ablation). Both FP32 modes keep the FP64 phase fold and evaluate
+boxes in FP32. ``backend`` accepts ``"hip"``, this engine's backend name.
 """
 
 from __future__ import annotations

@@ -49,6 +49,8 @@ constexpr int kFixBits = 59;
 // re-evaluated exactly. The screen's relative error is below 1/16 whenever the best
 // value is above the curve's `vdeg` bound; below it every box is evaluated exactly.
 constexpr float kKeep = 0.875f;
+// Distinct box widths (duration bins) per segment.
+constexpr int kMaxDurations = 32;
 
 struct CurveParams {
     int64_t point_offset;
@@ -784,6 +786,261 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
     }
 }
 
+// Reduced-precision accumulation, for the precision study (cpb_set_accumulation):
+// phase bins in FP32, filled by LDS float atomics from the same FP64 fold.
+//   ACC 1, compensated: each atomic add returns the bin's previous value, from which
+//          TwoSum recovers the add's rounding error exactly; the error (and the FP32
+//          residual of the input value) is added to a second FP32 bin, and prefix sums
+//          and box differences are carried in double-float (hi, lo) pairs;
+//   ACC 2, plain FP32 bins, prefix sums and box differences.
+// The objective of every box is evaluated in FP32; the winning box is finalised in FP64
+// from its FP32 (collapsed double-float) sums. One workgroup per work item, one period
+// after another; LDS holds 2 (ACC 2) or 4 (ACC 1) FP32 arrays of the phase span.
+
+struct DF {
+    float hi, lo;
+};
+
+__device__ __forceinline__ DF two_sum(float a, float b) {
+    const float s = a + b;
+    const float bp = s - a;
+    return {s, (a - (s - bp)) + (b - bp)};
+}
+
+__device__ __forceinline__ DF df_add(DF a, DF b) {
+    const DF s = two_sum(a.hi, b.hi);
+    const float lo = s.lo + (a.lo + b.lo);
+    const float hi = s.hi + lo;
+    return {hi, lo - (hi - s.hi)};
+}
+
+__device__ __forceinline__ DF df_neg(DF a) { return {-a.hi, -a.lo}; }
+
+template <int ACC>
+struct PrecBins {
+    float* yh;
+    float* wh;
+    float* yl;  // ACC 1 only
+    float* wl;
+    __device__ __forceinline__ DF y(int i) const { return {yh[i], ACC == 1 ? yl[i] : 0.0f}; }
+    __device__ __forceinline__ DF w(int i) const { return {wh[i], ACC == 1 ? wl[i] : 0.0f}; }
+    __device__ __forceinline__ void set(int i, DF yv, DF wv) {
+        yh[i] = yv.hi;
+        wh[i] = wv.hi;
+        if (ACC == 1) {
+            yl[i] = yv.lo;
+            wl[i] = wv.lo;
+        }
+    }
+};
+
+template <int ACC>
+__device__ __forceinline__ DF acc_add(DF a, DF b) {
+    if constexpr (ACC == 1) return df_add(a, b);
+    return {a.hi + b.hi, 0.0f};
+}
+
+template <int ACC>
+__device__ __forceinline__ float acc_diff(DF a, DF b) {
+    if constexpr (ACC == 1) {
+        const DF d = df_add(a, df_neg(b));
+        return d.hi + d.lo;
+    }
+    return a.hi - b.hi;
+}
+
+template <int ACC, int OBJ>
+__global__ __launch_bounds__(256) void bls_precision_kernel(
+    const double* __restrict__ tau, const double* __restrict__ yw64,
+    const double* __restrict__ wv64, const CurveParams* __restrict__ curves,
+    const cpb_segment* __restrict__ segs, const WorkItem* __restrict__ items,
+    int64_t n_items, const double* __restrict__ periods,
+    const int32_t* __restrict__ dur_bins, int32_t oversample, int32_t wl,
+    double* __restrict__ out, int64_t n_out) {
+    constexpr int T = 256;
+    constexpr int NW = T / kWave;
+    extern __shared__ float prec_lds[];
+    __shared__ int32_t s_dur[kMaxDurations];
+    __shared__ DF s_tot[2 * T];
+    __shared__ float s_bv[NW];
+    __shared__ uint32_t s_bk[NW];
+    __shared__ float s_by[NW], s_bw[NW];
+    PrecBins<ACC> bins{prec_lds, prec_lds + wl, prec_lds + 2 * wl, prec_lds + 3 * wl};
+    const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const WorkItem it = items[item];
+        const cpb_segment sg = segs[it.seg];
+        const CurveParams cv = curves[sg.curve];
+        const int n_dur = sg.n_dur;
+        const double bd = sg.bin_duration;
+        const double inv_bd = 1.0 / bd;
+        const int width = sg.width;
+        const int64_t poff = cv.point_offset;
+        const int npts = cv.n_points;
+        const double ytot64 = ldexp((double)cv.y_tot, cv.y_exp);
+        const double wtot64 = ldexp((double)cv.w_tot, cv.w_exp);
+        const DF ytot{(float)ytot64, (float)(ytot64 - (double)(float)ytot64)};
+        const DF wtot{(float)wtot64, (float)(wtot64 - (double)(float)wtot64)};
+        const float floor32 = (float)cv.ivar_floor;
+        __syncthreads();
+        if (tid < n_dur) s_dur[tid] = dur_bins[sg.dur_offset + tid];
+        __syncthreads();
+        for (int q = 0; q < it.count; ++q) {
+            const int64_t k = it.begin + q;
+            const double P = segment_period(sg, periods, k);
+            const double invP = 1.0 / P;
+            int64_t nb64 = (int64_t)ceil(P / bd) + oversample;
+            if (nb64 > width - 1) nb64 = width - 1;
+            const int n_bins = (int)nb64;
+            const int64_t col = sg.out_offset + k;
+            if (cv.flags & 2) {
+                if (tid == 0) {
+                    out[col] = -INFINITY;
+                    for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
+                }
+                continue;
+            }
+            for (int i = tid; i <= n_bins; i += T) bins.set(i, DF{0.0f, 0.0f}, DF{0.0f, 0.0f});
+            __syncthreads();
+            for (int j = tid; j < npts; j += T) {
+                const int b = fold_bin(tau[poff + j], P, invP, bd, inv_bd, width);
+                const double a64 = yw64[poff + j], w64 = wv64[poff + j];
+                const float a = (float)a64, w = (float)w64;
+                const float ao = atomicAdd(&bins.yh[b], a);
+                const float wo = atomicAdd(&bins.wh[b], w);
+                if (ACC == 1) {
+                    atomicAdd(&bins.yl[b], two_sum(ao, a).lo + (float)(a64 - (double)a));
+                    atomicAdd(&bins.wl[b], two_sum(wo, w).lo + (float)(w64 - (double)w));
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                for (int kk = 0; kk < oversample; ++kk) {
+                    const int dst = n_bins - oversample + kk;
+                    if (dst >= 0 && dst < width) bins.set(dst, bins.y(1 + kk), bins.w(1 + kk));
+                }
+            }
+            __syncthreads();
+            // inclusive prefix over [0, n_bins]: per-thread chunks, then chunk offsets
+            const int span = n_bins + 1;
+            const int c = (span + T - 1) / T;
+            const int base = tid * c;
+            DF sy{0.0f, 0.0f}, sw{0.0f, 0.0f};
+            for (int i = base; i < base + c && i < span; ++i) {
+                sy = acc_add<ACC>(sy, bins.y(i));
+                sw = acc_add<ACC>(sw, bins.w(i));
+            }
+            s_tot[2 * tid] = sy;
+            s_tot[2 * tid + 1] = sw;
+            __syncthreads();
+            if (tid == 0) {
+                DF ry{0.0f, 0.0f}, rw{0.0f, 0.0f};
+                for (int t = 0; t < T; ++t) {
+                    const DF ty = s_tot[2 * t], tw = s_tot[2 * t + 1];
+                    s_tot[2 * t] = ry;
+                    s_tot[2 * t + 1] = rw;
+                    ry = acc_add<ACC>(ry, ty);
+                    rw = acc_add<ACC>(rw, tw);
+                }
+            }
+            __syncthreads();
+            sy = s_tot[2 * tid];
+            sw = s_tot[2 * tid + 1];
+            for (int i = base; i < base + c && i < span; ++i) {
+                sy = acc_add<ACC>(sy, bins.y(i));
+                sw = acc_add<ACC>(sw, bins.w(i));
+                bins.set(i, sy, sw);
+            }
+            __syncthreads();
+            // box sweep in FP32, first strict maximum in (duration index, start) order
+            float best = -INFINITY, by = 0.0f, bw = 0.0f;
+            uint32_t bk = 0xFFFFFFFFu;
+            for (int di = 0; di < n_dur; ++di) {
+                const int d = s_dur[di];
+                if (d < 1 || d >= n_bins) continue;
+                for (int n = tid; n <= n_bins - d; n += T) {
+                    const float win = acc_diff<ACC>(bins.w(n + d), bins.w(n));
+                    const float wout = acc_diff<ACC>(wtot, DF{win, 0.0f});
+                    if (!(win >= floor32) || !(wout >= floor32)) continue;
+                    const float yin = acc_diff<ACC>(bins.y(n + d), bins.y(n));
+                    const float yrest = acc_diff<ACC>(ytot, DF{yin, 0.0f});
+                    const float m_in = yin / win, m_out = yrest / wout;
+                    if (m_out < m_in) continue;
+                    const float depth = m_out - m_in;
+                    const float obj = OBJ == 0 ? depth / sqrtf(1.0f / win + 1.0f / wout)
+                                               : 0.5f * win * depth * depth;
+                    if (obj > best) {
+                        best = obj;
+                        bk = ((uint32_t)di << 24) | (uint32_t)n;
+                        by = yin;
+                        bw = win;
+                    }
+                }
+            }
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const float ov = __shfl_xor(best, off, kWave);
+                const uint32_t ok = __shfl_xor(bk, off, kWave);
+                const float oy = __shfl_xor(by, off, kWave);
+                const float ow = __shfl_xor(bw, off, kWave);
+                if (ov > best || (ov == best && ok < bk)) {
+                    best = ov;
+                    bk = ok;
+                    by = oy;
+                    bw = ow;
+                }
+            }
+            if (lane == 0) {
+                s_bv[wave] = best;
+                s_bk[wave] = bk;
+                s_by[wave] = by;
+                s_bw[wave] = bw;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                best = s_bv[0];
+                bk = s_bk[0];
+                by = s_by[0];
+                bw = s_bw[0];
+                for (int w = 1; w < NW; ++w) {
+                    if (s_bv[w] > best || (s_bv[w] == best && s_bk[w] < bk)) {
+                        best = s_bv[w];
+                        bk = s_bk[w];
+                        by = s_by[w];
+                        bw = s_bw[w];
+                    }
+                }
+                if (!(best > -INFINITY)) {
+                    out[col] = -INFINITY;
+                    for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
+                } else {
+                    const int n = (int)(bk & 0xFFFFFFu);
+                    const int d = s_dur[bk >> 24];
+                    const double iv_in = (double)bw;
+                    const double iv_out = wtot64 - iv_in;
+                    const double yin = (double)by / iv_in;
+                    const double yout = (ytot64 - (double)by) / iv_out;
+                    const double depth = yout - yin;
+                    const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
+                    const double dsnr = depth / derr;
+                    const double ll = 0.5 * iv_in * depth * depth;
+                    const double dur = (double)d * bd;
+                    const double a = (double)n * bd + 0.5 * dur;
+                    const double ph = a < P ? a : (a < 2.0 * P ? a - P : fmod(a, P));
+                    out[col] = OBJ == 0 ? dsnr : ll;
+                    out[1 * n_out + col] = depth;
+                    out[2 * n_out + col] = derr;
+                    out[3 * n_out + col] = dsnr;
+                    out[4 * n_out + col] = dur;
+                    out[5 * n_out + col] = ph + cv.t_min;
+                    out[6 * n_out + col] = ll;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Per-curve summary of a search's power row, for batched runs that keep spectra on
 // the device: with p = power with NaN and +-inf set to 0 (`_assemble`,
 // src/cuperiod/methods/bls.py:280-282), summary = (mean(p), std(p) (two passes, ddof 0),
@@ -888,6 +1145,8 @@ struct cpb_context {
     bool timed = false;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     std::vector<WorkItem> h_items;
+    // Bin accumulation: 0 exact 64-bit fixed point, 1 compensated FP32, 2 plain FP32.
+    int accumulation = 0;
 };
 
 namespace {
@@ -980,7 +1239,7 @@ hipError_t dispatch_mode(int mode, int dmax, const LaunchArgs& a) {
 constexpr int kThreads = 256;
 constexpr int kMode0Span = kThreads * 8;
 
-constexpr int kMaxDur = 32;
+constexpr int kMaxDur = kMaxDurations;
 
 int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double* yw,
                const double* wv, int64_t total_points, const cpb_curve* curves,
@@ -1112,6 +1371,33 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     if (n_items == 0) return CPB_OK;
 
     const int wl = (wmax + 63) / 64 * 64;
+    if (ctx->accumulation != 0) {
+        const size_t plds = (size_t)(ctx->accumulation == 1 ? 4 : 2) * wl * sizeof(float);
+        if (plds + 8192 > (size_t)ctx->lds_limit)
+            return fail(ctx, CPB_E_RESOURCE,
+                        "reduced-precision accumulation keeps the phase bins in LDS; the "
+                        "phase span is too wide");
+        const dim3 grid((unsigned)std::min<int64_t>(n_items, (int64_t)1 << 30));
+        const auto* cp = static_cast<const CurveParams*>(ctx->curves.ptr);
+        if (ctx->timing) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
+        auto launch = [&](auto fn) {
+            if (plds > 65536)
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds);
+            hipLaunchKernelGGL(fn, grid, dim3(256), plds, st, tau, yw, wv, cp, sp, ip, n_items,
+                               periods, dur_bins, oversample, wl, out, n_out);
+        };
+        if (ctx->accumulation == 1)
+            obj_flag == 0 ? launch(bls_precision_kernel<1, 0>) : launch(bls_precision_kernel<1, 1>);
+        else
+            obj_flag == 0 ? launch(bls_precision_kernel<2, 0>) : launch(bls_precision_kernel<2, 1>);
+        CPB_HIP(ctx, hipGetLastError());
+        if (ctx->timing) {
+            CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
+            ctx->timed = true;
+        }
+        return CPB_OK;
+    }
     const size_t static_lds = 2048;
     const int n_slots = wl + (wl - 1) / 8 + 1;
     const size_t lds_need = (size_t)16 * n_slots;
@@ -1301,6 +1587,13 @@ int cpb_main_kernel_ms(cpb_context* ctx, double* ms) {
     float f = 0.0f;
     CPB_HIP(ctx, hipEventElapsedTime(&f, ctx->t0, ctx->t1));
     *ms = (double)f;
+    return CPB_OK;
+}
+
+int cpb_set_accumulation(cpb_context* ctx, int mode) {
+    if (!ctx) return CPB_E_INVALID;
+    if (mode < 0 || mode > 2) return fail(ctx, CPB_E_INVALID, "accumulation mode must be 0, 1 or 2");
+    ctx->accumulation = mode;
     return CPB_OK;
 }
 

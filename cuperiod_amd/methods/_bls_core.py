@@ -221,9 +221,12 @@ def _bls_search_hip_specs(
     oversample: int,
     obj_flag: int,
     device_cache: dict[str, Any] | None = None,
+    accumulation: int = 0,
 ) -> list[dict[str, np.ndarray]]:
     """Every segment's box search for one light curve: one upload, one launch
-    sequence, one download. Each segment's result is that of a separate search."""
+    sequence, one download. Each segment's result is that of a separate search.
+    ``accumulation`` selects the bins (0 exact fixed point, 1 compensated FP32, 2 FP32;
+    the FP32 modes use the FP32 admissibility floor)."""
     ctx = _lib.context(0)
     device = _device_of(ctx)
     torch = _torch()
@@ -256,8 +259,9 @@ def _bls_search_hip_specs(
     if int(max(spec.width for spec in specs)) > (1 << 30):
         raise BackendUnavailableError("BLS: phase-bin count above 2^30 per period")
     out = torch.empty((_lib.N_OUTPUTS, total), dtype=torch.float64, device=device)
-    floor = _ivar_floor(data["n"], data["sum_ivar"])
+    floor = _ivar_floor(data["n"], data["sum_ivar"], float32=accumulation != 0)
     with ctx.lock:
+        ctx.check(ctx.lib.cpb_set_accumulation(ctx.handle, int(accumulation)), "accumulation")
         status = ctx.lib.cpb_bls_search(
             ctx.handle, _stream_handle(device), _ptr(data["tau"]), _ptr(data["yw"]),
             _ptr(data["ivar"]), data["n"], data["t_min"], floor, _ptr(periods_d),
@@ -277,6 +281,23 @@ def _bls_search_hip_specs(
 
 # --- public entry points -------------------------------------------------------------
 
+#: Bin accumulation modes (``BLSSettings.accumulation``) and their engine codes.
+ACCUMULATION_MODES = {"fixed64": 0, "kahan32": 1, "fp32": 2}
+
+
+def _accumulation_code(accumulation: str | None, precision: str) -> int:
+    """Engine code of an accumulation mode; ``precision="float32"`` (the reference's
+    float32 kernel) means plain FP32 bins when no mode is named."""
+    if accumulation is None:
+        return 2 if str(precision).lower() in ("float32", "fp32", "single") else 0
+    try:
+        return ACCUMULATION_MODES[str(accumulation)]
+    except KeyError:
+        raise ValueError(
+            f"unknown accumulation {accumulation!r}; choose one of "
+            f"{sorted(ACCUMULATION_MODES)}"
+        ) from None
+
 
 def bls_power(
     t: np.ndarray,
@@ -291,14 +312,17 @@ def bls_power(
     batch: int = DEFAULT_BATCH,
     precision: str = "auto",
     device_cache: dict[str, Any] | None = None,
+    accumulation: str | None = None,
 ) -> BLSPower:
     """BLS box search of one light curve over ``periods`` on the MI355X.
 
     Equivalent to ``BoxLeastSquares(t, y, dy).power(periods, durations,
     objective=objective, oversample=oversample)`` with cuPeriod's relative
     admissibility floor: median-centred flux, ``ivar = 1/dy**2``, box widths rounded
-    half away from zero. ``batch`` and ``precision`` are accepted for signature
-    compatibility; the HIP engine always bins in exact 64-bit fixed point.
+    half away from zero. ``batch`` is accepted for signature compatibility. The engine
+    bins in exact 64-bit fixed point unless ``accumulation`` is ``"kahan32"`` or
+    ``"fp32"`` (or ``precision="float32"``), the reduced-precision modes of the
+    precision study.
 
     Raises
     ------
@@ -318,7 +342,7 @@ def bls_power(
         return _to_power(periods_host, _empty_output(periods_host.size))
     out = _bls_search_hip_specs(
         t_host, y_host, ivar_host, [spec], oversample=oversample, obj_flag=obj_flag,
-        device_cache=device_cache,
+        device_cache=device_cache, accumulation=_accumulation_code(accumulation, precision),
     )[0]
     return _to_power(periods_host, out)
 
@@ -335,12 +359,14 @@ def bls_power_segments(
     batch: int = DEFAULT_BATCH,
     precision: str = "auto",
     device_cache: dict[str, Any] | None = None,
+    accumulation: str | None = None,
 ) -> list[BLSPower]:
     """:func:`bls_power` over the ``(periods, durations)`` segments of one curve,
     with one upload of the curve and one download of every result; equal to one
     :func:`bls_power` call per segment."""
     obj_flag = _objective_flag(objective)
     _check_backend(backend)
+    acc = _accumulation_code(accumulation, precision)
     t_host, y_host, ivar_host = _host_inputs(t, y, dy)
     specs = [
         _search_spec(periods, durations, oversample)
@@ -350,7 +376,7 @@ def bls_power_segments(
     outs = iter(
         _bls_search_hip_specs(
             t_host, y_host, ivar_host, specs, oversample=oversample, obj_flag=obj_flag,
-            device_cache=device_cache,
+            device_cache=device_cache, accumulation=acc,
         )
         if specs
         else []
@@ -366,6 +392,7 @@ def bls_power_segments(
 
 
 __all__ = [
+    "ACCUMULATION_MODES",
     "BLSPower",
     "DEFAULT_BATCH",
     "HIP_BACKEND",

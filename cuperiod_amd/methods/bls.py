@@ -162,7 +162,7 @@ def _segments_power(
     powers = bls_power_segments(
         jd, flux, err, segments, settings.bins_per_duration,
         objective=settings.objective, backend=backend, batch=settings.batch_periods,
-        precision=settings.precision, device_cache={},
+        precision=settings.precision, device_cache={}, accumulation=settings.accumulation,
     )
     return [{name: getattr(p, name) for name in _SEGMENT_FIELDS} for p in powers]
 

@@ -145,6 +145,14 @@ int cpb_set_timing(cpb_context* ctx, int enable);
 /* Milliseconds of the main search kernel of the last timed search (waits for it). */
 int cpb_main_kernel_ms(cpb_context* ctx, double* ms);
 
+/* Bin accumulation of later searches on `ctx`: 0 (default) exact 64-bit fixed point;
+ * 1 FP32 bins with compensated (TwoSum) atomic adds and double-float prefix sums;
+ * 2 plain FP32 bins and prefix sums. Modes 1 and 2 evaluate every box in FP32 and serve
+ * the precision study (the reference's float32 CUDA kernel,
+ * src/cuperiod/methods/_bls_core.py:505-656 with REAL=float, is the plain FP32 case); they
+ * need the phase bins in LDS (CPB_E_RESOURCE otherwise). */
+int cpb_set_accumulation(cpb_context* ctx, int mode);
+
 /* Block until all work queued on `stream` has finished. */
 int cpb_synchronize(cpb_context* ctx, void* stream);
 

@@ -41,18 +41,19 @@ def _flux_lc(t, flux, err):
 # --- C ABI ------------------------------------------------------------------------
 
 
-def _header_functions() -> list[str]:
-    text = (ROOT / "include" / "cuperiod_bls.h").read_text()
-    return re.findall(r"^\s*(?:int|void|const char\*)\s+(cpb_\w+)\(", text, re.M)
+def _header_functions(header: str = "cuperiod_bls.h", prefix: str = "cpb") -> list[str]:
+    text = (ROOT / "include" / header).read_text()
+    return re.findall(rf"^\s*(?:int|void|const char\*)\s+({prefix}_\w+)\(", text, re.M)
 
 
 def test_header_and_binding_declare_the_same_symbols():
     assert tuple(_header_functions()) == _lib.EXPORTED_SYMBOLS
+    assert tuple(_header_functions("cuperiod_proposer.h", "cpp")) == _lib.PROPOSER_SYMBOLS
 
 
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(_lib.LIB_PATH))
-    for name in _header_functions():
+    for name in _header_functions() + _header_functions("cuperiod_proposer.h", "cpp"):
         assert hasattr(lib, name), name
 
 
