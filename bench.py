@@ -163,6 +163,10 @@ def main() -> int:
     ap.add_argument("--freqs", type=int, default=200_000)
     ap.add_argument("--cpu-curves", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shape instead of autotuning it")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each step directly instead of replaying a captured graph")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned proposer + refinement leg")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -201,22 +205,41 @@ def main() -> int:
     device = torch.device("cuda", ctx.device)
     out = torch.empty((_lib.N_OUTPUTS, pb.n_out), dtype=torch.float64, device=device)
     summary = torch.empty((len(pb.plans), 4), dtype=torch.float64, device=device)
+    launch = {"graph": not args.no_graph, "shape": None}
+    if not args.no_tune:
+        from cuperiod_amd.batch import autotune
+
+        sample = autotune.sample_batch(pb, settings, 64)
+        tr = autotune.tune(pb, sample, use_cache=False)
+        launch.update(shape=vars(tr.shape), tune_s=round(tr.seconds, 2),
+                      tune_sample="first 64 curves, full grids",
+                      tune_outputs_identical=tr.identical, candidates_ms=tr.timings_ms)
+        log(f"autotuned launch shape {tr.shape} in {tr.seconds:.1f} s "
+            f"(outputs identical across shapes: {tr.identical})")
+        del sample
     ctx.lib.cpb_set_timing(ctx.handle, 1)
+    if args.no_graph:
+        def step():
+            engine.run_packed(pb, out=out, summary=summary)
+    else:
+        graph = engine.capture(pb, out, summary)
+        step = graph.launch
     for w in range(args.warmup):
-        engine.run_packed(pb, out=out, summary=summary)
+        step()
         torch.cuda.synchronize()
         log(f"warm-up step {w + 1}/{args.warmup} done")
     kernel_ms = []
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for s in range(args.steps):
-        engine.run_packed(pb, out=out, summary=summary)
+        step()
         import ctypes
 
         ms = ctypes.c_double()
         ctx.check(ctx.lib.cpb_main_kernel_ms(ctx.handle, ctypes.byref(ms)), "timing")
         kernel_ms.append(ms.value)
-        log(f"step {s + 1}/{args.steps}: main kernel {ms.value:.1f} ms")
+        log(f"step {s + 1}/{args.steps}: {'graph' if not args.no_graph else 'main kernel'} "
+            f"{ms.value:.1f} ms")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     res = engine.finish(pb, out, summary)
@@ -263,6 +286,7 @@ def main() -> int:
             "lds_frac": 16.0 * (folds + boxes) / (k_ms * 1e-3) / PEAK_LDS,
             "best_period_sample": [float(x) for x in res.best_period[:4]],
         },
+        "launch": launch,
         "learned": None,
         "cpu_baseline": None,
     }

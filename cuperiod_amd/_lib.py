@@ -45,6 +45,10 @@ EXPORTED_SYMBOLS = (
     "cpb_set_timing",
     "cpb_main_kernel_ms",
     "cpb_set_accumulation",
+    "cpb_set_launch",
+    "cpb_graph_create",
+    "cpb_graph_launch",
+    "cpb_graph_destroy",
     "cpb_synchronize",
 )
 
@@ -133,6 +137,17 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.cpb_set_timing.argtypes = [vp, ctypes.c_int]
     lib.cpb_main_kernel_ms.restype = ctypes.c_int
     lib.cpb_main_kernel_ms.argtypes = [vp, ctypes.POINTER(f64)]
+    lib.cpb_set_launch.restype = ctypes.c_int
+    lib.cpb_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.cpb_graph_create.restype = ctypes.c_int
+    lib.cpb_graph_create.argtypes = [
+        vp, vp, vp, vp, i64, ctypes.POINTER(Curve), i32, vp, vp,
+        ctypes.POINTER(Segment), i32, i32, i32, vp, i64, vp, vp, vp, ctypes.POINTER(vp),
+    ]
+    lib.cpb_graph_launch.restype = ctypes.c_int
+    lib.cpb_graph_launch.argtypes = [vp, vp, vp]
+    lib.cpb_graph_destroy.restype = None
+    lib.cpb_graph_destroy.argtypes = [vp]
     lib.cpb_set_accumulation.restype = ctypes.c_int
     lib.cpb_set_accumulation.argtypes = [vp, ctypes.c_int]
     lib.cpb_synchronize.restype = ctypes.c_int

@@ -497,6 +497,60 @@ def run_packed(pb: PackedBatch, *, out=None, summary=None):
     return out, summary
 
 
+class BatchGraph:
+    """A batched search and its per-curve summaries captured once into a HIP graph
+    (``cpb_graph_create``); :meth:`launch` replays it on torch's current stream. The
+    graph reads the packed inputs and writes ``out``/``summary`` in place, so ``pb``,
+    ``out`` and ``summary`` are kept alive with it."""
+
+    def __init__(self, pb: PackedBatch, out, summary) -> None:
+        self.ctx = _lib.context(0)
+        self.pb, self.out, self.summary = pb, out, summary
+        handle = ctypes.c_void_p()
+        with self.ctx.lock:
+            self.ctx.check(self.ctx.lib.cpb_set_accumulation(self.ctx.handle, pb.accumulation),
+                           "accumulation")
+            st = self.ctx.lib.cpb_graph_create(
+                self.ctx.handle, _vp(pb.tau), _vp(pb.yw), _vp(pb.ivar), pb.total_points,
+                pb.curves, len(pb.plans), _vp(pb.periods), _vp(pb.dur_bins), pb.segments,
+                pb.n_segments, pb.oversample, pb.obj_flag, _vp(out), pb.n_out,
+                _vp(pb.curve_base_d), _vp(pb.curve_count_d), _vp(summary), ctypes.byref(handle),
+            )
+            self.ctx.check(st, "graph capture")
+        self.handle = handle
+
+    def launch(self) -> None:
+        import torch
+
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.out.device).cuda_stream)
+        with self.ctx.lock:
+            self.ctx.check(self.ctx.lib.cpb_graph_launch(self.ctx.handle, self.handle, stream),
+                           "graph launch")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.ctx.lib.cpb_graph_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def capture(pb: PackedBatch, out=None, summary=None) -> BatchGraph:
+    """Capture ``pb``'s search (and summaries) into a replayable :class:`BatchGraph`."""
+    import torch
+
+    device = _device()
+    if out is None:
+        out = torch.empty((_lib.N_OUTPUTS, max(1, pb.n_out)), dtype=torch.float64, device=device)
+    if summary is None:
+        summary = torch.empty((len(pb.plans), 4), dtype=torch.float64, device=device)
+    return BatchGraph(pb, out, summary)
+
+
 def finish(pb: PackedBatch, out, summary, *, spectra: bool = False) -> BatchResult:
     """Bring the per-curve summaries (and, on request, the spectra) to the host."""
     s = summary.cpu().numpy()
@@ -550,5 +604,6 @@ def bls_batch(
     return finish(pb, out, summary, spectra=spectra)
 
 
-__all__ = ["BatchResult", "CurvePlan", "CurveSet", "LinspaceGrid", "PackedBatch", "bls_batch",
+__all__ = ["BatchGraph", "BatchResult", "CurvePlan", "CurveSet", "LinspaceGrid", "PackedBatch",
+           "bls_batch", "capture",
            "finish", "pack", "pack_curves", "pack_plans", "plan_grid", "run_packed", "window_plan"]

@@ -5,11 +5,11 @@ Every field can be overridden from the environment as ``CUPERIOD_BLS_<FIELD>`` (
 example ``CUPERIOD_BLS_N_DURATIONS=6``), as the reference's pydantic-settings model
 allows; an explicit constructor argument wins over the environment.
 
-One field is new: ``accumulation`` selects how the HIP fold accumulates phase bins
-(``"fixed64"``: exact 64-bit fixed point, the default; ``"kahan32"``: FP32 bins with
-TwoSum-compensated atomic adds and double-float prefix sums; ``"fp32"``: plain FP32
-bins, the precision ablation). Both FP32 modes keep the FP64 phase fold and evaluate
-boxes in FP32. ``backend`` accepts ``"hip"``, this engine's backend name.
+One field is new: ``accumulation`` selects the engine's precision (``"fixed64"``: FP64
+phase fold and exact 64-bit fixed-point bins, the default; ``"kahan32"``: FP64 phase fold,
+FP32 bins with TwoSum-compensated atomic adds and double-float prefix sums; ``"fp32"``:
+all FP32 -- phase fold, bins and prefix sums -- the precision ablation). Both FP32 modes
+evaluate boxes in FP32. ``backend`` accepts ``"hip"``, this engine's backend name.
 """
 
 from __future__ import annotations

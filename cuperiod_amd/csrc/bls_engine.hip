@@ -786,13 +786,16 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
     }
 }
 
-// Reduced-precision accumulation, for the precision study (cpb_set_accumulation):
-// phase bins in FP32, filled by LDS float atomics from the same FP64 fold.
-//   ACC 1, compensated: each atomic add returns the bin's previous value, from which
-//          TwoSum recovers the add's rounding error exactly; the error (and the FP32
-//          residual of the input value) is added to a second FP32 bin, and prefix sums
-//          and box differences are carried in double-float (hi, lo) pairs;
-//   ACC 2, plain FP32 bins, prefix sums and box differences.
+// Reduced precision, for the precision study (cpb_set_accumulation): phase bins in FP32,
+// filled by LDS float atomics.
+//   ACC 1, FP64 phase + compensated FP32: the FP64 fold of the main kernel; each atomic
+//          add returns the bin's previous value, from which TwoSum recovers the add's
+//          rounding error exactly; the error (and the FP32 residual of the input value)
+//          is added to a second FP32 bin, and prefix sums and box differences are carried
+//          in double-float (hi, lo) pairs;
+//   ACC 2, all FP32: the fold x - P*floor(x/P) in FP32 from FP32 tau and period (as the
+//          reference's float32 kernel does), plain FP32 bins, prefix sums and box
+//          differences.
 // The objective of every box is evaluated in FP32; the winning box is finalised in FP64
 // from its FP32 (collapsed double-float) sums. One workgroup per work item, one period
 // after another; LDS holds 2 (ACC 2) or 4 (ACC 1) FP32 arrays of the phase span.
@@ -890,6 +893,7 @@ __global__ __launch_bounds__(256) void bls_precision_kernel(
             const int64_t k = it.begin + q;
             const double P = segment_period(sg, periods, k);
             const double invP = 1.0 / P;
+            const float Pf = (float)P, bdf = (float)bd;
             int64_t nb64 = (int64_t)ceil(P / bd) + oversample;
             if (nb64 > width - 1) nb64 = width - 1;
             const int n_bins = (int)nb64;
@@ -904,7 +908,17 @@ __global__ __launch_bounds__(256) void bls_precision_kernel(
             for (int i = tid; i <= n_bins; i += T) bins.set(i, DF{0.0f, 0.0f}, DF{0.0f, 0.0f});
             __syncthreads();
             for (int j = tid; j < npts; j += T) {
-                const int b = fold_bin(tau[poff + j], P, invP, bd, inv_bd, width);
+                int b;
+                if constexpr (ACC == 2) {
+                    // all-FP32: the phase too, from FP32 tau and period
+                    const float xf = (float)tau[poff + j];
+                    const float kf = floorf(xf / Pf);
+                    const float wf = xf - Pf * kf;
+                    b = (int)(wf / bdf) + 1;
+                    b = b < 0 ? 0 : (b > width - 1 ? width - 1 : b);
+                } else {
+                    b = fold_bin(tau[poff + j], P, invP, bd, inv_bd, width);
+                }
                 const double a64 = yw64[poff + j], w64 = wv64[poff + j];
                 const float a = (float)a64, w = (float)w64;
                 const float ao = atomicAdd(&bins.yh[b], a);
@@ -1147,6 +1161,13 @@ struct cpb_context {
     std::vector<WorkItem> h_items;
     // Bin accumulation: 0 exact 64-bit fixed point, 1 compensated FP32, 2 plain FP32.
     int accumulation = 0;
+    // Launch shape (cpb_set_launch): workgroup size, periods per work item (0: chosen
+    // from the batch), workgroups per CU of a persistent grid (0: one per work item).
+    int threads = 256;
+    int tile = 0;
+    int grid_per_cu = 0;
+    // Set while a graph captures a search: no event is recorded on the stream.
+    bool capturing = false;
 };
 
 namespace {
@@ -1220,24 +1241,26 @@ hipError_t launch_main(const LaunchArgs& a) {
     return hipGetLastError();
 }
 
-template <int OBJ, int MODE>
+template <int T, int OBJ, int MODE>
 hipError_t dispatch_d(int dmax, const LaunchArgs& a) {
-    if (dmax <= 8) return launch_main<256, 8, OBJ, MODE>(a);
-    if (dmax <= 16) return launch_main<256, 16, OBJ, MODE>(a);
-    return launch_main<256, 32, OBJ, MODE>(a);
+    if (dmax <= 8) return launch_main<T, 8, OBJ, MODE>(a);
+    if (dmax <= 16) return launch_main<T, 16, OBJ, MODE>(a);
+    return launch_main<T, 32, OBJ, MODE>(a);
+}
+
+template <int T, int OBJ>
+hipError_t dispatch_mode(int mode, int dmax, const LaunchArgs& a) {
+    if (mode == 0) return dispatch_d<T, OBJ, 0>(dmax, a);
+    if (mode == 1) return dispatch_d<T, OBJ, 1>(dmax, a);
+    return dispatch_d<T, OBJ, 2>(dmax, a);
 }
 
 template <int OBJ>
-hipError_t dispatch_mode(int mode, int dmax, const LaunchArgs& a) {
-    if (mode == 0) return dispatch_d<OBJ, 0>(dmax, a);
-    if (mode == 1) return dispatch_d<OBJ, 1>(dmax, a);
-    return dispatch_d<OBJ, 2>(dmax, a);
+hipError_t dispatch_threads(int threads, int mode, int dmax, const LaunchArgs& a) {
+    if (threads == 128) return dispatch_mode<128, OBJ>(mode, dmax, a);
+    if (threads == 512) return dispatch_mode<512, OBJ>(mode, dmax, a);
+    return dispatch_mode<256, OBJ>(mode, dmax, a);
 }
-
-// Workgroup size and prefix chunk of the main kernel: MODE 0 keeps a thread's chunk
-// of at most 8 bins in registers, so it serves phase spans of up to 256*8 bins.
-constexpr int kThreads = 256;
-constexpr int kMode0Span = kThreads * 8;
 
 constexpr int kMaxDur = kMaxDurations;
 
@@ -1307,6 +1330,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     const int64_t target_items = (int64_t)ctx->n_cu * 64;
     int64_t tile = live / std::max<int64_t>(1, target_items);
     tile = std::max<int64_t>(1, std::min<int64_t>(tile, kMaxTile));
+    if (ctx->tile > 0) tile = ctx->tile;
     ctx->h_items.clear();
     std::vector<cpb_segment> empties;
     for (int32_t s = 0; s < n_segments; ++s) {
@@ -1349,8 +1373,10 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     if (n_items > 0)
         std::memcpy(hb + off_item, ctx->h_items.data(), sizeof(WorkItem) * n_items);
     CPB_HIP(ctx, hipMemcpyAsync(ctx->tables.ptr, hb, tbytes, hipMemcpyHostToDevice, st));
-    CPB_HIP(ctx, hipEventRecord(ctx->copied, st));
-    ctx->copy_pending = true;
+    if (!ctx->capturing) {
+        CPB_HIP(ctx, hipEventRecord(ctx->copied, st));
+        ctx->copy_pending = true;
+    }
     char* db = static_cast<char*>(ctx->tables.ptr);
     const auto* curve_in = reinterpret_cast<const cpb_curve*>(db);
     const auto* sp = reinterpret_cast<const cpb_segment*>(db + off_seg);
@@ -1377,9 +1403,12 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
             return fail(ctx, CPB_E_RESOURCE,
                         "reduced-precision accumulation keeps the phase bins in LDS; the "
                         "phase span is too wide");
-        const dim3 grid((unsigned)std::min<int64_t>(n_items, (int64_t)1 << 30));
+        // a launch (and a graph kernel node) holds fewer than 2^32 work-items; the
+        // kernel loops over the items its grid does not cover
+        const dim3 grid((unsigned)std::min<int64_t>(n_items, ((int64_t)1 << 31) / 256));
         const auto* cp = static_cast<const CurveParams*>(ctx->curves.ptr);
-        if (ctx->timing) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
+        const bool timed = ctx->timing && !ctx->capturing;
+        if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
         auto launch = [&](auto fn) {
             if (plds > 65536)
                 (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
@@ -1392,27 +1421,32 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         else
             obj_flag == 0 ? launch(bls_precision_kernel<2, 0>) : launch(bls_precision_kernel<2, 1>);
         CPB_HIP(ctx, hipGetLastError());
-        if (ctx->timing) {
+        if (timed) {
             CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
             ctx->timed = true;
         }
         return CPB_OK;
     }
+    const int threads = ctx->threads;
     const size_t static_lds = 2048;
     const int n_slots = wl + (wl - 1) / 8 + 1;
     const size_t lds_need = (size_t)16 * n_slots;
     const bool fits = lds_need + static_lds <= (size_t)ctx->lds_limit;
-    const int mode = fits ? (wl <= kMode0Span ? 0 : 1) : 2;
+    const int mode = fits ? (wl <= threads * kChunk ? 0 : 1) : 2;
     int64_t* wide_ptr = nullptr;
     LaunchArgs la;
     la.lds = 0;
+    const int64_t persistent = (int64_t)ctx->n_cu * ctx->grid_per_cu;
     if (mode == 2) {
-        const int64_t g = std::min<int64_t>(n_items, (int64_t)ctx->n_cu * 4);
+        const int64_t g = std::min<int64_t>(n_items, persistent > 0 ? persistent : (int64_t)ctx->n_cu * 4);
         if ((rc = ensure(ctx, ctx->wide, (size_t)g * lds_need))) return rc;
         wide_ptr = static_cast<int64_t*>(ctx->wide.ptr);
         la.grid = dim3((unsigned)g);
     } else {
-        la.grid = dim3((unsigned)std::min<int64_t>(n_items, (int64_t)1 << 30));
+        // fewer than 2^32 work-items per launch (a graph kernel node refuses more); the
+        // kernel loops over the items its grid does not cover
+        const int64_t cap = ((int64_t)1 << 31) / threads;
+        la.grid = dim3((unsigned)std::min<int64_t>(n_items, persistent > 0 ? persistent : cap));
         la.lds = lds_need;
     }
     la.st = st;
@@ -1430,11 +1464,12 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     la.wide = wide_ptr;
     la.out = out;
     la.n_out = n_out;
-    if (ctx->timing) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
-    const hipError_t e = obj_flag == 0 ? dispatch_mode<0>(mode, dmax, la)
-                                       : dispatch_mode<1>(mode, dmax, la);
+    const bool timed = ctx->timing && !ctx->capturing;
+    if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
+    const hipError_t e = obj_flag == 0 ? dispatch_threads<0>(threads, mode, dmax, la)
+                                       : dispatch_threads<1>(threads, mode, dmax, la);
     if (e != hipSuccess) return hip_fail(ctx, e, "bls_main_kernel launch");
-    if (ctx->timing) {
+    if (timed) {
         CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
         ctx->timed = true;
     }
@@ -1587,6 +1622,103 @@ int cpb_main_kernel_ms(cpb_context* ctx, double* ms) {
     float f = 0.0f;
     CPB_HIP(ctx, hipEventElapsedTime(&f, ctx->t0, ctx->t1));
     *ms = (double)f;
+    return CPB_OK;
+}
+
+int cpb_set_launch(cpb_context* ctx, int threads, int tile, int grid_per_cu) {
+    if (!ctx) return CPB_E_INVALID;
+    if (threads != 128 && threads != 256 && threads != 512)
+        return fail(ctx, CPB_E_INVALID, "threads must be 128, 256 or 512");
+    if (tile < 0 || tile > kMaxTile) return fail(ctx, CPB_E_INVALID, "tile must be in [0, 32]");
+    if (grid_per_cu < 0 || grid_per_cu > 64)
+        return fail(ctx, CPB_E_INVALID, "grid_per_cu must be in [0, 64]");
+    ctx->threads = threads;
+    ctx->tile = tile;
+    ctx->grid_per_cu = grid_per_cu;
+    return CPB_OK;
+}
+
+struct cpb_graph {
+    cpb_context* priv = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+void cpb_graph_destroy(cpb_graph* g) {
+    if (!g) return;
+    if (g->priv) (void)hipSetDevice(g->priv->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->cap) (void)hipStreamDestroy(g->cap);
+    if (g->priv) cpb_destroy(g->priv);
+    delete g;
+}
+
+int cpb_graph_create(cpb_context* ctx, const double* tau, const double* yw, const double* wv,
+                     int64_t total_points, const cpb_curve* curves, int32_t n_curves,
+                     const double* periods, const int32_t* dur_bins,
+                     const cpb_segment* segments, int32_t n_segments, int32_t oversample,
+                     int32_t obj_flag, double* out, int64_t n_out, const int64_t* curve_base,
+                     const int64_t* curve_count, double* summary, cpb_graph** graph) {
+    if (!ctx || !graph) return CPB_E_INVALID;
+    *graph = nullptr;
+    auto* g = new cpb_graph();
+    int rc = cpb_create(ctx->device, &g->priv);
+    if (rc != CPB_OK) {
+        delete g;
+        return fail(ctx, rc, "graph: cannot create its private context");
+    }
+    g->priv->accumulation = ctx->accumulation;
+    g->priv->threads = ctx->threads;
+    g->priv->tile = ctx->tile;
+    g->priv->grid_per_cu = ctx->grid_per_cu;
+    g->priv->capturing = true;
+    auto bail = [&](int status, const std::string& msg) {
+        std::string m = msg;
+        if (g->priv && !g->priv->error.empty()) m += ": " + g->priv->error;
+        cpb_graph_destroy(g);
+        return fail(ctx, status, m);
+    };
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(CPB_E_DEVICE, std::string("graph stream: ") + hipGetErrorString(e));
+    e = hipStreamBeginCapture(g->cap, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return bail(CPB_E_DEVICE, std::string("begin capture: ") + hipGetErrorString(e));
+    rc = run_search(g->priv, g->cap, tau, yw, wv, total_points, curves, n_curves, periods,
+                    dur_bins, segments, n_segments, oversample, obj_flag, out, n_out);
+    if (rc == CPB_OK && summary) {
+        if (!curve_base || !curve_count) {
+            rc = CPB_E_INVALID;
+        } else {
+            hipLaunchKernelGGL(curve_summary_kernel, dim3(n_curves), dim3(256), 0, g->cap, out,
+                               curve_base, curve_count, summary);
+            if (hipGetLastError() != hipSuccess) rc = CPB_E_DEVICE;
+        }
+    }
+    hipGraph_t captured = nullptr;
+    e = hipStreamEndCapture(g->cap, &captured);
+    g->graph = captured;
+    if (rc != CPB_OK) return bail(rc, "graph: capture of the search failed");
+    if (e != hipSuccess || !captured)
+        return bail(CPB_E_DEVICE, std::string("end capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return bail(CPB_E_DEVICE, std::string("instantiate: ") + hipGetErrorString(e));
+    g->priv->capturing = false;
+    *graph = g;
+    return CPB_OK;
+}
+
+int cpb_graph_launch(cpb_context* ctx, cpb_graph* g, void* stream) {
+    if (!ctx || !g || !g->exec) return CPB_E_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    CPB_HIP(ctx, hipSetDevice(ctx->device));
+    if (ctx->timing) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
+    CPB_HIP(ctx, hipGraphLaunch(g->exec, st));
+    if (ctx->timing) {
+        CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
+        ctx->timed = true;
+    }
     return CPB_OK;
 }
 

@@ -145,13 +145,38 @@ int cpb_set_timing(cpb_context* ctx, int enable);
 /* Milliseconds of the main search kernel of the last timed search (waits for it). */
 int cpb_main_kernel_ms(cpb_context* ctx, double* ms);
 
-/* Bin accumulation of later searches on `ctx`: 0 (default) exact 64-bit fixed point;
- * 1 FP32 bins with compensated (TwoSum) atomic adds and double-float prefix sums;
- * 2 plain FP32 bins and prefix sums. Modes 1 and 2 evaluate every box in FP32 and serve
- * the precision study (the reference's float32 CUDA kernel,
- * src/cuperiod/methods/_bls_core.py:505-656 with REAL=float, is the plain FP32 case); they
+/* Precision of later searches on `ctx`: 0 (default) FP64 phase fold and exact 64-bit
+ * fixed-point bins; 1 FP64 phase fold, FP32 bins with compensated (TwoSum) atomic adds
+ * and double-float prefix sums; 2 all FP32: phase fold, bins and prefix sums (the
+ * reference's float32 CUDA kernel, src/cuperiod/methods/_bls_core.py:505-656 with
+ * REAL=float). Modes 1 and 2 evaluate every box in FP32, serve the precision study, and
  * need the phase bins in LDS (CPB_E_RESOURCE otherwise). */
 int cpb_set_accumulation(cpb_context* ctx, int mode);
+
+/* Launch shape of later searches on `ctx` (chosen by the autotuner,
+ * cuperiod_amd/batch/autotune.py): `threads` per workgroup (128, 256 or 512), `tile`
+ * periods per work item (1..32; 0 = chosen from the batch size), and `grid_per_cu`
+ * workgroups per compute unit of a persistent grid that loops over the work items
+ * (0 = one workgroup per item). Results do not depend on the shape. */
+int cpb_set_launch(cpb_context* ctx, int threads, int tile, int grid_per_cu);
+
+/* A captured batched search: the arguments of cpb_bls_search_batch plus, when `summary`
+ * is not NULL, those of cpb_bls_summarize, recorded once into a HIP graph with buffers
+ * of its own (the context's accumulation and launch shape at capture time apply).
+ * Every cpb_graph_launch replays the whole sequence on `stream`; inputs and outputs are
+ * read and written at the captured addresses, which must stay valid. With timing on
+ * (cpb_set_timing), cpb_main_kernel_ms afterwards reports the replay's duration.
+ * Replaces the per-curve launch loop of cuPeriod's batch driver
+ * (src/cuperiod/batch/runner.py:214-250) with one launch per chunk. */
+typedef struct cpb_graph cpb_graph;
+int cpb_graph_create(cpb_context* ctx, const double* tau, const double* yw, const double* wv,
+                     int64_t total_points, const cpb_curve* curves, int32_t n_curves,
+                     const double* periods, const int32_t* dur_bins,
+                     const cpb_segment* segments, int32_t n_segments, int32_t oversample,
+                     int32_t obj_flag, double* out, int64_t n_out, const int64_t* curve_base,
+                     const int64_t* curve_count, double* summary, cpb_graph** graph);
+int cpb_graph_launch(cpb_context* ctx, cpb_graph* graph, void* stream);
+void cpb_graph_destroy(cpb_graph* graph);
 
 /* Block until all work queued on `stream` has finished. */
 int cpb_synchronize(cpb_context* ctx, void* stream);

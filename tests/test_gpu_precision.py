@@ -1,8 +1,8 @@
 """The reduced-precision accumulation modes of the precision study against the oracle.
 
-``fixed64`` (the default) is held to the parity bar; ``kahan32`` (FP32 bins with
-compensated adds) and ``fp32`` (plain FP32 bins) are the study's modes: they must find
-the injected period and stay within the stated error bounds.
+``fixed64`` (the default) is held to the parity bar; ``kahan32`` (FP64 phase, FP32 bins
+with compensated adds) and ``fp32`` (all FP32, phase included) are the study's modes: they
+must find the injected period, and the compensated mode must stay within 1e-5.
 """
 
 from __future__ import annotations
@@ -36,8 +36,10 @@ def test_accumulation_modes_against_oracle(objective):
         best = periods[int(np.argmax(np.nan_to_num(r.power, neginf=-1.0)))]
         assert abs(best - 3.7) < 0.01, (acc, best)
     assert errs["fixed64"] < 1e-6, errs
-    assert errs["kahan32"] < 1e-3, errs
-    assert errs["fp32"] < 5e-2, errs
+    assert errs["kahan32"] < 1e-5, errs
+    # the all-FP32 ablation bins some points differently (FP32 phase): only its period
+    # recovery is required; its error is what the study measures
+    assert errs["kahan32"] < errs["fp32"], errs
 
 
 def test_precision_setting_reaches_the_batch_engine():
