@@ -545,15 +545,31 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
             }
             if constexpr (GLOBAL) __threadfence();
             __syncthreads();
-#pragma unroll 2
-            for (int j = tid; j < npts; j += T) {
-                const double x = tau[poff + j];
-                const int64_t a = yq[poff + j];
-                const int64_t b = wq[poff + j];
-                const int s = slot_of(fold_bin(x, P, invP, bd, inv_bd, width));
-                atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s), (unsigned long long)a);
-                atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s + 1),
-                          (unsigned long long)b);
+            // two points per thread per pass, both loaded before either is folded
+            for (int j0 = tid; j0 < npts; j0 += 2 * T) {
+                const int j1 = j0 + T;
+                const double x0 = tau[poff + j0];
+                const int64_t a0 = yq[poff + j0];
+                const int64_t b0 = wq[poff + j0];
+                double x1 = 0.0;
+                int64_t a1 = 0, b1 = 0;
+                if (j1 < npts) {
+                    x1 = tau[poff + j1];
+                    a1 = yq[poff + j1];
+                    b1 = wq[poff + j1];
+                }
+                const int s0 = slot_of(fold_bin(x0, P, invP, bd, inv_bd, width));
+                atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0),
+                          (unsigned long long)a0);
+                atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0 + 1),
+                          (unsigned long long)b0);
+                if (j1 < npts) {
+                    const int s1 = slot_of(fold_bin(x1, P, invP, bd, inv_bd, width));
+                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s1),
+                              (unsigned long long)a1);
+                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s1 + 1),
+                              (unsigned long long)b1);
+                }
             }
             if constexpr (GLOBAL) __threadfence();
             __syncthreads();
@@ -1123,6 +1139,113 @@ __global__ __launch_bounds__(256) void curve_summary_kernel(
 }
 
 // Columns of a segment with no box width: power -inf, every other output 0.
+// Peak candidates of each curve's sanitised power row, for batch rows that do not need
+// the whole spectrum. Position x counts in ascending frequency (x = count-1-k for the
+// engine's ascending-period column k). A candidate is an interior maximum
+// (s[x] > s[x-1] and s[x] >= s[x+1]), x = 0 when s[0] >= s[1], x = n-1 when
+// s[n-1] > s[n-2] (x = 0 alone when n = 1): the rule of `Periodogram._peak_candidates`
+// (src/cuperiod/core/result.py:369-406). Sanitised power is never negative, so its bit
+// pattern orders like its value; a two-pass radix select (11 bits each) finds a threshold
+// holding at least `want` candidates, and every candidate at or above it is written (at
+// most `cap`, in no particular order) with its power, depth, depth_snr, duration and
+// transit_time. counts[c] = (all candidates, candidates at or above the threshold).
+constexpr int kPeakBins = 2048;
+
+__device__ __forceinline__ uint64_t peak_bits(const double* __restrict__ power, int64_t b,
+                                              int64_t n, int64_t x) {
+    const double v = sanitize(power[b + (n - 1 - x)]);
+    return v > 0.0 ? (uint64_t)__double_as_longlong(v) : 0ull;
+}
+
+__device__ __forceinline__ bool peak_candidate(const double* __restrict__ power, int64_t b,
+                                               int64_t n, int64_t x, uint64_t& bits) {
+    bits = peak_bits(power, b, n, x);
+    if (n == 1) return true;
+    const double s = sanitize(power[b + (n - 1 - x)]);
+    if (x == 0) return s >= sanitize(power[b + (n - 2)]);
+    const double left = sanitize(power[b + (n - x)]);
+    if (x == n - 1) return s > left;
+    const double right = sanitize(power[b + (n - 2 - x)]);
+    return s > left && s >= right;
+}
+
+__global__ __launch_bounds__(256) void peak_candidates_kernel(
+    const double* __restrict__ out, int64_t n_out, const int64_t* __restrict__ base,
+    const int64_t* __restrict__ count, int32_t want, int32_t cap,
+    int32_t* __restrict__ cand, int64_t* __restrict__ counts, double* __restrict__ values) {
+    __shared__ uint32_t hist[kPeakBins];
+    __shared__ uint64_t s_prefix;
+    __shared__ int s_level;
+    __shared__ uint32_t s_total, s_written;
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int64_t b = base[c];
+    const int64_t n = count[c];
+    const double* power = out;
+    // level 0: no threshold yet; 1: top 11 bits fixed; 2: top 22 bits fixed
+    if (tid == 0) {
+        s_prefix = 0;
+        s_level = 0;
+        s_total = 0;
+        s_written = 0;
+    }
+    __syncthreads();
+    uint32_t above = 0;  // candidates strictly above the current prefix bucket
+    for (int level = 0; level < 2; ++level) {
+        for (int i = tid; i < kPeakBins; i += 256) hist[i] = 0;
+        __syncthreads();
+        const int shift = 53 - 11 * level;
+        const uint64_t pre = s_prefix;
+        uint32_t local = 0;
+        for (int64_t x = tid; x < n; x += 256) {
+            uint64_t bits;
+            if (!peak_candidate(power, b, n, x, bits)) continue;
+            if (level == 0) ++local;
+            if (level == 0 || (bits >> (shift + 11)) == pre) atomicAdd(&hist[(bits >> shift) & (kPeakBins - 1)], 1u);
+        }
+        if (level == 0 && local) atomicAdd(&s_total, local);
+        __syncthreads();
+        const uint32_t total = s_total;
+        if (total <= (uint32_t)want) break;  // every candidate is kept
+        if (tid == 0) {
+            uint32_t cum = above;
+            int bucket = kPeakBins - 1;
+            for (; bucket > 0; --bucket) {
+                if (cum + hist[bucket] >= (uint32_t)want) break;
+                cum += hist[bucket];
+            }
+            s_prefix = (pre << 11) | (uint64_t)bucket;
+            s_level = level + 1;
+            hist[0] = cum;  // candidates above the chosen bucket
+        }
+        __syncthreads();
+        above = hist[0];
+        __syncthreads();
+    }
+    const int lvl = s_level;
+    const uint64_t thr = lvl == 0 ? 0ull : (s_prefix << (64 - 11 * lvl));
+    for (int64_t x = tid; x < n; x += 256) {
+        uint64_t bits;
+        if (!peak_candidate(power, b, n, x, bits) || bits < thr) continue;
+        const uint32_t i = atomicAdd(&s_written, 1u);
+        if (i < (uint32_t)cap) {
+            const int64_t col = b + (n - 1 - x);
+            cand[(int64_t)c * cap + i] = (int32_t)x;
+            double* v = values + ((int64_t)c * cap + i) * 5;
+            v[0] = sanitize(out[col]);
+            v[1] = out[1 * n_out + col];
+            v[2] = out[3 * n_out + col];
+            v[3] = out[4 * n_out + col];
+            v[4] = out[5 * n_out + col];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        counts[2 * c] = s_total;
+        counts[2 * c + 1] = s_written;
+    }
+}
+
 __global__ void fill_empty_kernel(double* __restrict__ out, int64_t n_out, int64_t off,
                                   int64_t count) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1585,6 +1708,23 @@ int cpb_bls_summarize(cpb_context* ctx, void* stream, const double* power,
     hipLaunchKernelGGL(curve_summary_kernel, dim3(n_curves), dim3(256), 0,
                        static_cast<hipStream_t>(stream), power, curve_base, curve_count,
                        summary);
+    CPB_HIP(ctx, hipGetLastError());
+    return CPB_OK;
+}
+
+int cpb_peak_candidates(cpb_context* ctx, void* stream, const double* out, int64_t n_out,
+                        const int64_t* curve_base, const int64_t* curve_count,
+                        int32_t n_curves, int32_t want, int32_t cap, int32_t* cand,
+                        int64_t* counts, double* values) {
+    if (!ctx) return CPB_E_INVALID;
+    if (n_curves < 0 || want < 1 || cap < want ||
+        (n_curves > 0 && (!out || !curve_base || !curve_count || !cand || !counts || !values)))
+        return fail(ctx, CPB_E_INVALID, "peak candidates: bad arguments");
+    if (n_curves == 0) return CPB_OK;
+    CPB_HIP(ctx, hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(peak_candidates_kernel, dim3(n_curves), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), out, n_out, curve_base, curve_count,
+                       want, cap, cand, counts, values);
     CPB_HIP(ctx, hipGetLastError());
     return CPB_OK;
 }

@@ -134,6 +134,22 @@ int cpb_bls_summarize(cpb_context* ctx, void* stream, const double* power,
                       const int64_t* curve_base, const int64_t* curve_count,
                       int32_t n_curves, double* summary);
 
+/* Peak candidates of each curve's power row of `out` (columns curve_base[c] ..
+ * curve_base[c] + curve_count[c], ascending period), as `Periodogram._peak_candidates`
+ * defines them on the sanitised power in ascending frequency
+ * (src/cuperiod/core/result.py:369-406): position x = curve_count[c]-1-k for column k.
+ * Every candidate at or above a per-curve threshold that keeps at least `want` of them
+ * (all when there are fewer) is written, at most `cap` per curve, in no particular order:
+ * cand[c*cap + i] = x and values[(c*cap + i)*5 + 0..4] = (sanitised power, depth,
+ * depth_snr, duration, transit_time). counts[2c] = all candidates of the curve,
+ * counts[2c+1] = candidates at or above the threshold (may exceed `cap`). Replaces the
+ * (7, F) download per curve of cuPeriod's batch rows (src/cuperiod/batch/io.py:507-571)
+ * with `cap` values per curve. */
+int cpb_peak_candidates(cpb_context* ctx, void* stream, const double* out, int64_t n_out,
+                        const int64_t* curve_base, const int64_t* curve_count,
+                        int32_t n_curves, int32_t want, int32_t cap, int32_t* cand,
+                        int64_t* counts, double* values);
+
 /* Frequencies of np.linspace(start, stop, num) by numpy's recipe (value_i = i*step +
  * start, last = stop), computed on the HOST: the values mode-1 segments use. */
 int cpb_linspace_grid(double start, double stop, int64_t num, double* out);
