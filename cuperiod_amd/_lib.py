@@ -41,6 +41,7 @@ EXPORTED_SYMBOLS = (
     "cpb_bls_search",
     "cpb_bls_search_batch",
     "cpb_bls_summarize",
+    "cpb_peak_candidates",
     "cpb_linspace_grid",
     "cpb_set_timing",
     "cpb_main_kernel_ms",
@@ -131,6 +132,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     ]
     lib.cpb_bls_summarize.restype = ctypes.c_int
     lib.cpb_bls_summarize.argtypes = [vp, vp, vp, vp, vp, i32, vp]
+    lib.cpb_peak_candidates.restype = ctypes.c_int
+    lib.cpb_peak_candidates.argtypes = [vp, vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp]
     lib.cpb_linspace_grid.restype = ctypes.c_int
     lib.cpb_linspace_grid.argtypes = [f64, f64, i64, ctypes.POINTER(f64)]
     lib.cpb_set_timing.restype = ctypes.c_int
