@@ -124,6 +124,30 @@ class CurvePlan:
             k -= sg.n_periods
         raise IndexError("period index out of range")
 
+    def periods_at(self, k: np.ndarray) -> np.ndarray:
+        """Periods of ascending indices ``k`` (an int array), by the device's recipe."""
+        k = np.asarray(k, dtype=np.int64)
+        out = np.empty(k.shape, dtype=np.float64)
+        start = 0
+        for sg in self.segments:
+            sel = (k >= start) & (k < start + sg.n_periods)
+            if sel.any():
+                kk = k[sel] - start
+                if sg.mode == 0:
+                    out[sel] = sg.periods[kk]
+                else:
+                    i = sg.lin_num - 1 - (sg.lin_k0 + kk)
+                    f = i.astype(np.float64) * sg.lin_step + sg.lin_start
+                    if sg.mode == 1:
+                        f = np.where(i == sg.lin_num - 1, sg.lin_stop, f)
+                    else:
+                        f = np.where(i == 0, sg.lin_start, f)
+                    out[sel] = 1.0 / f
+            start += sg.n_periods
+        if np.any((k < 0) | (k >= start)):
+            raise IndexError("period index out of range")
+        return out
+
     def periods(self) -> np.ndarray:
         """Every period of the curve's grid, ascending (materialised on request)."""
         parts = []
@@ -497,6 +521,50 @@ def run_packed(pb: PackedBatch, *, out=None, summary=None):
     return out, summary
 
 
+@dataclass
+class PeakCandidates:
+    """Device peak candidates of each curve (``cpb_peak_candidates``), on the host:
+    ``index[c]`` ascending-frequency positions (sorted), ``values[c]`` their (power,
+    depth, depth_snr, duration, transit_time), ``total[c]`` all candidates of the curve,
+    ``written[c]`` those at or above the curve's threshold (``> cap``: truncated)."""
+
+    index: list[np.ndarray]
+    values: list[np.ndarray]
+    total: np.ndarray
+    written: np.ndarray
+    cap: int
+
+
+def peak_candidates(pb: PackedBatch, out, *, want: int = 512, cap: int = 1024) -> PeakCandidates:
+    """Peak candidates of every curve of a finished (or enqueued) search, on the host."""
+    import torch
+
+    ctx = _lib.context(0)
+    device = out.device
+    n = len(pb.plans)
+    cand = torch.empty((max(1, n), cap), dtype=torch.int32, device=device)
+    counts = torch.empty((max(1, n), 2), dtype=torch.int64, device=device)
+    values = torch.empty((max(1, n), cap, 5), dtype=torch.float64, device=device)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    with ctx.lock:
+        st = ctx.lib.cpb_peak_candidates(
+            ctx.handle, stream, _vp(out), pb.n_out, _vp(pb.curve_base_d), _vp(pb.curve_count_d),
+            n, int(want), int(cap), _vp(cand), _vp(counts), _vp(values),
+        )
+        ctx.check(st, "peak candidates")
+    cand_h = cand.cpu().numpy()
+    counts_h = counts.cpu().numpy()
+    values_h = values.cpu().numpy()
+    index, vals = [], []
+    for c in range(n):
+        m = int(min(counts_h[c, 1], cap))
+        order = np.argsort(cand_h[c, :m], kind="stable")
+        index.append(cand_h[c, :m][order].astype(np.int64))
+        vals.append(values_h[c, :m][order])
+    return PeakCandidates(index=index, values=vals, total=counts_h[:n, 0].copy(),
+                          written=counts_h[:n, 1].copy(), cap=cap)
+
+
 class BatchGraph:
     """A batched search and its per-curve summaries captured once into a HIP graph
     (``cpb_graph_create``); :meth:`launch` replays it on torch's current stream. The
@@ -605,5 +673,5 @@ def bls_batch(
 
 
 __all__ = ["BatchGraph", "BatchResult", "CurvePlan", "CurveSet", "LinspaceGrid", "PackedBatch",
-           "bls_batch", "capture",
+           "PeakCandidates", "bls_batch", "capture", "peak_candidates",
            "finish", "pack", "pack_curves", "pack_plans", "plan_grid", "run_packed", "window_plan"]

@@ -22,7 +22,7 @@ from pathlib import Path
 from typing import Any
 
 from cuperiod_amd.core.lightcurve import LightCurve
-from cuperiod_amd.core.result import Periodogram
+from cuperiod_amd.core.result import Peak, Periodogram
 
 #: Methods whose secondary peaks skip harmonics and aliases of the chosen ones.
 ALIAS_DIVERSE_METHODS: frozenset[str] = frozenset({"BLS", "TLS"})
@@ -152,14 +152,34 @@ def periodogram_to_row(
     source: str | None = None,
 ) -> dict[str, Any]:
     """One result row: identity columns, ``best_period``, then per-rank peaks."""
-    row: dict[str, Any] = {"key": key, "method": pg.method}
-    if source is not None:
-        row["source"] = source
-    row["backend"] = pg.backend
-    row["n_samples"] = pg.n_samples
-    row["baseline"] = pg.baseline
     diverse = alias_diverse_for(pg.method) if alias_diverse is None else alias_diverse
     peaks = pg.best_periods(max(int(n_best), 1), alias_diverse=diverse)
+    raw = pg.downsample() if store_raw else None
+    return peaks_to_row(key, pg.method, pg.backend, pg.n_samples, pg.baseline, peaks,
+                        n_best=n_best, extra_keys=extra_keys, source=source, raw=raw)
+
+
+def peaks_to_row(
+    key: str,
+    method: str,
+    backend: str,
+    n_samples: int,
+    baseline: float,
+    peaks: Sequence[Peak],
+    *,
+    n_best: int,
+    extra_keys: Sequence[str] | None = None,
+    source: str | None = None,
+    raw: tuple | None = None,
+) -> dict[str, Any]:
+    """The row of :func:`periodogram_to_row` from already selected peaks (most
+    significant first) and, for ``store_raw``, the downsampled ``(frequency, power)``."""
+    row: dict[str, Any] = {"key": key, "method": method}
+    if source is not None:
+        row["source"] = source
+    row["backend"] = backend
+    row["n_samples"] = n_samples
+    row["baseline"] = baseline
     row["best_period"] = peaks[0].period if peaks else float("nan")
     declared = tuple(extra_keys) if extra_keys is not None else ()
     for rank in range(1, n_best + 1):
@@ -173,8 +193,8 @@ def periodogram_to_row(
         for name, value in extra.items():
             if name not in declared:
                 row[f"{name}_{rank}"] = value
-    if store_raw:
-        freq, power = pg.downsample()
+    if raw is not None:
+        freq, power = raw
         row["pgram_frequency"] = freq.tolist()
         row["pgram_power"] = power.tolist()
     return row
@@ -279,6 +299,7 @@ __all__ = [
     "alias_diverse_for",
     "csv_safe",
     "peak_extra_columns",
+    "peaks_to_row",
     "periodogram_to_row",
     "read_rows",
     "resolve_inputs",

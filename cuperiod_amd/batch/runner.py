@@ -8,9 +8,14 @@ different execution model. cuPeriod's ``_process_chunk`` (``:214-250``) calls
 light curves is packed into one set of device arrays and searched by ONE batched engine
 call (:mod:`cuperiod_amd.batch.engine`), in this process; the next chunk is packed on
 the host while the device searches the current one. Each curve's rows are the rows that
-``periodogram(lc, "BLS")`` gives, bit for bit: the device regenerates each curve's
-default grid with numpy's recipe and the spectra are assembled exactly as the single
-curve path assembles them.
+``periodogram(lc, "BLS")`` gives: the device regenerates each curve's default grid with
+numpy's recipe. Rows without the raw spectrum are built from device peak candidates
+(``cpb_peak_candidates``: the candidates above a per-curve threshold, a few KB per curve
+instead of the (7, F) spectrum); the selection is the host's own greedy on them, and a
+curve whose selection they cannot decide (it would read past them, or equal scores make
+its order depend on the whole spectrum) is built from its full spectrum. Their ``sde_k``
+uses the device's mean and standard deviation of the power (numpy's pairwise summation
+differs in the last bits); every other column is identical.
 
 Sinks (``:734-761``): ``None`` returns rows in memory; a ``.parquet``/``.csv`` file is
 rewritten at the end (merged with its rows when resuming); a directory gets one
@@ -36,8 +41,10 @@ import numpy as np
 
 from cuperiod_amd.batch.io import (
     InputItem,
+    alias_diverse_for,
     csv_safe,
     peak_extra_columns,
+    peaks_to_row,
     periodogram_to_row,
     read_rows,
     resolve_inputs,
@@ -48,7 +55,8 @@ from cuperiod_amd.core.columns import Domain, coerce_domain
 from cuperiod_amd.core.errors import InsufficientDataError
 from cuperiod_amd.core.grid import GridSpec
 from cuperiod_amd.core.lightcurve import LightCurve, raise_if_unusable
-from cuperiod_amd.core.result import Periodogram, peak_defaults_from_settings
+from cuperiod_amd.core.result import Peak, Periodogram, peak_defaults_from_settings
+from cuperiod_amd.methods._bls_core import _OUTPUT_NAMES
 from cuperiod_amd.methods.base import (
     get_method,
     normalize_backend,
@@ -437,13 +445,126 @@ def _launch(prepared: Sequence[_Prepared], settings: Any, grid: GridSpec | None)
     return pb, out, summary
 
 
-def _collect(handle: Any) -> list[tuple[np.ndarray, dict[str, np.ndarray]]]:
-    """Wait for a launch; each curve's ``(periods, spectrum)``, ascending period."""
+def _collect(handle: Any, *, candidates: bool = False) -> list:
+    """Wait for a launch. Per curve: ``(periods, spectrum)`` (ascending period), or,
+    with ``candidates``, a :class:`_Candidates` (the rows need no full spectrum)."""
     from cuperiod_amd.batch import engine
 
     pb, out, summary = handle
-    res = engine.finish(pb, out, summary, spectra=True)
-    return [(pb.plans[c].periods(), res.spectra[c]) for c in range(len(pb.plans))]
+    if not candidates:
+        res = engine.finish(pb, out, summary, spectra=True)
+        return [(pb.plans[c].periods(), res.spectra[c]) for c in range(len(pb.plans))]
+    pk = engine.peak_candidates(pb, out)
+    summ = summary.cpu().numpy()
+    results = []
+    for c, plan in enumerate(pb.plans):
+        n = int(pb.curve_count[c])
+        base = int(pb.curve_base[c])
+
+        def fetch(c=c, plan=plan, n=n, base=base):
+            host = out[:, base:base + n].cpu().numpy()
+            return plan.periods(), {name: host[i].copy()
+                                    for i, name in enumerate(_OUTPUT_NAMES)}
+
+        def fetch_power(n=n, base=base):
+            return out[0, base:base + n].cpu().numpy()
+
+        def fetch_extras(k, base=base):
+            import torch
+
+            cols = torch.from_numpy(np.asarray(k, dtype=np.int64) + base).to(out.device)
+            return out[[1, 3, 4, 5]][:, cols].cpu().numpy().T
+
+        x = pk.index[c]
+        results.append(_Candidates(
+            periods=plan.periods_at(n - 1 - x) if x.size else np.zeros(0),
+            values=pk.values[c], total=int(pk.total[c]), written=int(pk.written[c]),
+            cap=pk.cap, mean=float(summ[c, 0]), std=float(summ[c, 1]), fetch=fetch,
+            fetch_power=fetch_power, fetch_extras=fetch_extras,
+            all_periods=plan.periods,
+        ))
+    return results
+
+
+@dataclass
+class _Candidates:
+    """One curve's device peak candidates in ascending frequency, its SDE statistics and
+    ways to fetch its full spectrum, its power row, or outputs at given columns."""
+
+    periods: np.ndarray
+    values: np.ndarray
+    total: int
+    written: int
+    cap: int
+    mean: float
+    std: float
+    fetch: Callable[[], tuple]
+    fetch_power: Callable[[], np.ndarray] | None = None
+    fetch_extras: Callable[[np.ndarray], np.ndarray] | None = None
+    all_periods: Callable[[], np.ndarray] | None = None
+
+
+def _peaks_from(period, freq, power, extras, chosen, mean, std) -> list[Peak]:
+    peaks = []
+    for rank, i in enumerate(chosen, 1):
+        p = float(power[i])
+        e = extras[rank - 1]
+        sde = (p - mean) / std if std > 0.0 else 0.0
+        peaks.append(Peak(period=float(period[i]), frequency=float(freq[i]), power=p, rank=rank,
+                          extra={"depth": float(e[0]), "depth_snr": float(e[1]),
+                                 "duration": float(e[2]), "t0": float(e[3]), "sde": sde}))
+    return peaks
+
+
+def _candidate_row(prep: _Prepared, cand: _Candidates, settings: Any,
+                   cfg: _ChunkConfig) -> dict[str, Any] | None:
+    """The row :func:`periodogram_to_row` gives for this curve, from its device
+    candidates. When they cannot decide the selection -- it would read past them, or
+    equal scores among the visited ones make the order depend on how numpy sorts the
+    whole candidate list -- the selection runs on the curve's full power row instead (one
+    row, not seven), with the other outputs fetched at the chosen columns only; ``None``
+    when even that is unavailable. SDE uses the device's mean and standard deviation of
+    the power."""
+    count = max(int(cfg.n_best), 1)
+    diverse = alias_diverse_for("BLS")
+    defaults = peak_defaults_from_settings(settings)
+
+    def stub(freq, period, score):
+        return Periodogram(method="BLS", backend="hip", frequency=freq, period=period,
+                           power=score, objective_sense="max", n_samples=prep.finite.n,
+                           baseline=prep.finite.baseline, peak_defaults=defaults)
+
+    if cand.written <= cand.cap and cand.periods.size:
+        freq = 1.0 / cand.periods
+        period = 1.0 / freq  # what Periodogram.from_spectrum stores
+        score = cand.values[:, 0]
+        seen: list[int] = []
+        chosen = stub(freq, period, score).select_peaks(
+            freq, score, np.arange(score.size), count, alias_diverse=diverse, examined=seen)
+        visited = np.sort(score)[::-1][:min(seen[0] + 1, score.size)]
+        decided = (chosen.size == count or cand.written == cand.total) and (
+            np.unique(visited).size == visited.size)
+        if decided:
+            peaks = _peaks_from(period, freq, score, cand.values[chosen, 1:], chosen,
+                                cand.mean, cand.std)
+            return peaks_to_row(prep.key, "BLS", "hip", prep.finite.n, prep.finite.baseline,
+                                peaks, n_best=cfg.n_best, extra_keys=cfg.extra_keys,
+                                source=prep.label)
+    if cand.fetch_power is None or cand.fetch_extras is None or cand.all_periods is None:
+        return None
+    # the full candidate list, in the order the single-curve path builds it
+    periods_all = cand.all_periods()
+    n = periods_all.size
+    freq = (1.0 / periods_all)[::-1]
+    period = 1.0 / freq
+    score = np.nan_to_num(cand.fetch_power()[::-1], nan=0.0, posinf=0.0, neginf=0.0)
+    pg = stub(freq, period, score)
+    chosen = pg.select_peaks(freq, score, pg._peak_candidates(score), count,
+                             alias_diverse=diverse)
+    extras = cand.fetch_extras(n - 1 - chosen) if chosen.size else np.zeros((0, 4))
+    peaks = _peaks_from(period, freq, score, extras, chosen, cand.mean, cand.std)
+    return peaks_to_row(prep.key, "BLS", "hip", prep.finite.n, prep.finite.baseline, peaks,
+                        n_best=cfg.n_best, extra_keys=cfg.extra_keys, source=prep.label)
 
 
 class _ChunkRunner:
@@ -481,14 +602,21 @@ class _ChunkRunner:
         for settings, prepared, handle in launches:
             if handle is None:
                 continue
-            for prep, (periods, spectrum) in zip(prepared, _collect(handle)):
+            for prep, res in zip(prepared, _collect(handle, candidates=not cfg.store_raw)):
                 if prep.position in errors:
                     continue
                 try:
-                    pg = _periodogram(prep, periods, spectrum, settings)
-                    row = periodogram_to_row(prep.key, pg, n_best=cfg.n_best,
-                                             store_raw=cfg.store_raw,
-                                             extra_keys=cfg.extra_keys, source=prep.label)
+                    row = None
+                    if isinstance(res, _Candidates):
+                        row = _candidate_row(prep, res, settings, cfg)
+                        if row is None:
+                            res = res.fetch()
+                    if row is None:
+                        periods, spectrum = res
+                        pg = _periodogram(prep, periods, spectrum, settings)
+                        row = periodogram_to_row(prep.key, pg, n_best=cfg.n_best,
+                                                 store_raw=cfg.store_raw,
+                                                 extra_keys=cfg.extra_keys, source=prep.label)
                 except Exception as exc:  # noqa: BLE001 - recorded per light curve
                     errors[prep.position] = (prep.key, f"{type(exc).__name__}: {exc}")
                     by_pos.pop(prep.position, None)

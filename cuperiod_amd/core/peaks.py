@@ -61,16 +61,23 @@ def select_top_peaks(
     candidates: np.ndarray,
     n_peaks: int,
     min_separation: float,
+    *,
+    examined: list | None = None,
 ) -> np.ndarray:
-    """Greedy highest-score peaks at least ``min_separation`` apart in frequency."""
+    """Greedy highest-score peaks at least ``min_separation`` apart in frequency.
+    ``examined`` (a list) receives the number of candidates visited."""
     if n_peaks <= 0:
         return np.empty(0, dtype=np.int64)
     chosen: list[int] = []
+    seen = 0
     for idx in _descending(score, candidates):
+        seen += 1
         if all(abs(frequency[idx] - frequency[j]) >= min_separation for j in chosen):
             chosen.append(int(idx))
             if len(chosen) >= n_peaks:
                 break
+    if examined is not None:
+        examined.append(seen)
     return np.asarray(chosen, dtype=np.int64)
 
 
@@ -111,13 +118,17 @@ def select_diverse_peaks(
     harmonic_max: int,
     *,
     min_separation: float | None = None,
+    examined: list | None = None,
 ) -> np.ndarray:
     """Greedy highest-score peaks skipping any candidate alias-related (either way
-    round) to an already chosen one."""
+    round) to an already chosen one. ``examined`` (a list) receives the number of
+    candidates visited."""
     if n_peaks <= 0:
         return np.empty(0, dtype=np.int64)
     chosen: list[int] = []
+    seen = 0
     for idx in _descending(score, candidates):
+        seen += 1
         f_new = float(frequency[idx])
         if all(
             not is_alias_related(
@@ -133,6 +144,8 @@ def select_diverse_peaks(
             chosen.append(int(idx))
             if len(chosen) >= n_peaks:
                 break
+    if examined is not None:
+        examined.append(seen)
     return np.asarray(chosen, dtype=np.int64)
 
 

@@ -209,6 +209,29 @@ class Periodogram:
         candidates = self._peak_candidates(score)
         if candidates.size == 0:
             return []
+        chosen = self.select_peaks(
+            self.frequency, score, candidates, count,
+            min_separation_rayleigh=min_separation_rayleigh, alias_diverse=alias_diverse,
+            harmonic_max=harmonic_max, alias_tolerance=alias_tolerance,
+        )
+        return [self._make_peak(int(idx), rank) for rank, idx in enumerate(chosen, 1)]
+
+    def select_peaks(
+        self,
+        frequency: np.ndarray,
+        score: np.ndarray,
+        candidates: np.ndarray,
+        count: int,
+        *,
+        min_separation_rayleigh: float | None = None,
+        alias_diverse: bool = False,
+        harmonic_max: int | None = None,
+        alias_tolerance: float | None = None,
+        examined: list | None = None,
+    ) -> np.ndarray:
+        """The greedy selection of :meth:`best_periods` on given candidates, with this
+        periodogram's baseline and peak defaults (``examined`` receives the number of
+        candidates visited)."""
         separation_rayleigh = float(
             self._peak_default("min_separation_rayleigh", DEFAULT_SEPARATION_RAYLEIGH)
             if min_separation_rayleigh is None
@@ -229,16 +252,13 @@ class Periodogram:
             )
             if tolerance is None:
                 tolerance = min(separation, 1.0 / self.baseline) if resolved else 0.0
-            chosen = select_diverse_peaks(
-                self.frequency, score, candidates, count,
+            return select_diverse_peaks(
+                frequency, score, candidates, count,
                 max(float(tolerance), 1e-12), order,
-                min_separation=max(separation, 1e-12),
+                min_separation=max(separation, 1e-12), examined=examined,
             )
-        else:
-            chosen = select_top_peaks(
-                self.frequency, score, candidates, count, separation
-            )
-        return [self._make_peak(int(idx), rank) for rank, idx in enumerate(chosen, 1)]
+        return select_top_peaks(frequency, score, candidates, count, separation,
+                                examined=examined)
 
     def _peak_candidates(self, score: np.ndarray) -> np.ndarray:
         """Interior local maxima plus boundary samples exceeding their neighbour;

@@ -55,7 +55,7 @@ def fake_engine(monkeypatch):
 
     monkeypatch.setattr(base, "hip_ready", lambda: True)
     monkeypatch.setattr(runner, "_launch", launch)
-    monkeypatch.setattr(runner, "_collect", lambda h: _oracle_spectra(*h))
+    monkeypatch.setattr(runner, "_collect", lambda h, **kw: _oracle_spectra(*h))
     return calls
 
 
@@ -181,3 +181,78 @@ def test_no_device_refused_before_work(monkeypatch):
     monkeypatch.setattr(base, "hip_ready", lambda: False)
     with pytest.raises(BackendUnavailableError):
         runner.batch_periodograms([("a", _curve(50))], "BLS")
+
+
+def _host_candidates(power, want):
+    """Candidates of a power row (ascending period) by the host rule, in ascending
+    frequency, with every candidate at or above the want-th largest score."""
+    s = np.nan_to_num(power[::-1], nan=0.0, posinf=0.0, neginf=0.0)
+    n = s.size
+    x = []
+    for i in range(n):
+        if n == 1:
+            x.append(i)
+        elif i == 0:
+            if s[0] >= s[1]:
+                x.append(i)
+        elif i == n - 1:
+            if s[i] > s[i - 1]:
+                x.append(i)
+        elif s[i] > s[i - 1] and s[i] >= s[i + 1]:
+            x.append(i)
+    x = np.array(x, dtype=np.int64)
+    if x.size > want:
+        thr = np.sort(s[x])[::-1][want - 1]
+        keep = x[s[x] >= thr]
+    else:
+        keep = x
+    return keep, x.size
+
+
+def _as_candidates(periods, spectrum, want, cap=None):
+    keep, total = _host_candidates(spectrum["power"], want)
+    n = periods.size
+    k = n - 1 - keep
+    p = np.nan_to_num(spectrum["power"], nan=0.0, posinf=0.0, neginf=0.0)
+    values = np.stack([p[k], spectrum["depth"][k], spectrum["depth_snr"][k],
+                       spectrum["duration"][k], spectrum["transit_time"][k]], axis=1)
+    return runner._Candidates(
+        periods=periods[k], values=values, total=int(total), written=int(keep.size),
+        cap=cap or max(1, keep.size), mean=float(p.mean()), std=float(p.std()),
+        fetch=lambda: (periods, spectrum), fetch_power=lambda: spectrum["power"].copy(),
+        fetch_extras=lambda kk: np.stack([spectrum[q][kk] for q in
+                                          ("depth", "depth_snr", "duration", "transit_time")],
+                                         axis=1),
+        all_periods=lambda: periods)
+
+
+@pytest.mark.parametrize("seed", [1, 3, 7, 11])
+def test_rows_from_candidates_equal_full_rows(seed):
+    """Rows from device-style candidates equal the full path's rows, whether the
+    candidates decide the selection or the power row has to (ties, too few, truncated)."""
+    settings = BLSSettings()
+    lc = _curve(seed, n=300, baseline=40.0)
+    prep = runner._prepare(0, "k", lc, settings, None)
+    (periods, spectrum), = _oracle_spectra([prep], settings, None)
+    cfg = runner._ChunkConfig(("BLS",), {"BLS": "hip"}, {"BLS": settings}, 10, False, None,
+                              bio.peak_extra_columns(["BLS"]))
+    full = _expected_row("k", lc)
+    for want, cap_delta in ((512, 0), (3, 0), (512, -1)):
+        cand = _as_candidates(periods, spectrum, want)
+        if cap_delta:
+            cand.cap = cand.written + cap_delta
+        _same(runner._candidate_row(prep, cand, settings, cfg), full)
+
+
+def test_candidates_decide_distinct_scores_without_the_power_row():
+    settings = BLSSettings()
+    lc = _curve(3, n=2000, baseline=40.0)
+    prep = runner._prepare(0, "k", lc, settings, None)
+    (periods, spectrum), = _oracle_spectra([prep], settings, None)
+    cfg = runner._ChunkConfig(("BLS",), {"BLS": "hip"}, {"BLS": settings}, 10, False, None,
+                              bio.peak_extra_columns(["BLS"]))
+    cand = _as_candidates(periods, spectrum, 512)
+    cand.fetch_power = None  # the power row must not be needed
+    row = runner._candidate_row(prep, cand, settings, cfg)
+    assert row is not None
+    _same(row, _expected_row("k", lc))

@@ -31,12 +31,15 @@ def _curves():
     return out
 
 
-def _same(a, b):
+def _same(a, b, sde_rtol=0.0):
+    """Equal rows; ``sde_k`` within ``sde_rtol`` (device vs numpy mean/std)."""
     assert a.keys() == b.keys()
     for k in a:
         x, y = a[k], b[k]
         if isinstance(x, float) and math.isnan(x):
             assert isinstance(y, float) and math.isnan(y), k
+        elif k.startswith("sde_") and sde_rtol:
+            assert abs(x - y) <= sde_rtol * max(1.0, abs(y)), (k, x, y)
         else:
             assert x == y, (k, x, y)
 
@@ -54,7 +57,54 @@ def test_rows_equal_single_curve_path():
         pg = periodogram(lc, "BLS")
         expected = bio.periodogram_to_row(key, pg, n_best=10, store_raw=False,
                                           extra_keys=extra, source="")
-        _same(next(rows), expected)
+        _same(next(rows), expected, sde_rtol=1e-12)
+
+
+def test_raw_rows_equal_single_curve_path():
+    items = [it for it in _curves() if it[0] != "short"][:4]
+    res = batch_periodograms(items, "BLS", store_raw=True)
+    extra = bio.peak_extra_columns(["BLS"])
+    for row, (key, lc) in zip(res.rows, items):
+        pg = periodogram(lc, "BLS")
+        _same(row, bio.periodogram_to_row(key, pg, n_best=10, store_raw=True,
+                                          extra_keys=extra, source=""))
+
+
+def test_device_candidates_follow_the_host_rule():
+    from cuperiod_amd.batch import engine
+    from cuperiod_amd.core.config import BLSSettings as S
+    from cuperiod_amd.workloads import survey_curves
+
+    curves, grids, _ = survey_curves(6, seed=515, n_freq=40_000)
+    settings = S(n_durations=6)
+    pb = engine.pack(curves, grids, settings)
+    out, summary = engine.run_packed(pb)
+    res = engine.finish(pb, out, summary, spectra=True)
+    for want in (5, 64, 512):
+        pk = engine.peak_candidates(pb, out, want=want, cap=2 * want)
+        for c in range(len(pb.plans)):
+            s = np.nan_to_num(res.spectra[c]["power"][::-1], nan=0.0, posinf=0.0, neginf=0.0)
+            inner = np.flatnonzero((s[1:-1] > s[:-2]) & (s[1:-1] >= s[2:])) + 1
+            cands = list(inner)
+            if s[0] >= s[1]:
+                cands.append(0)
+            if s[-1] > s[-2]:
+                cands.append(s.size - 1)
+            cands = np.sort(np.array(cands))
+            assert pk.total[c] == cands.size
+            got = pk.index[c]
+            assert got.size == min(pk.written[c], 2 * want)
+            assert pk.written[c] >= min(want, cands.size)
+            thr = s[got].min()
+            if pk.written[c] <= 2 * want:
+                np.testing.assert_array_equal(got, cands[s[cands] >= thr])
+            else:
+                assert np.all(np.isin(got, cands))
+            k = s.size - 1 - got
+            np.testing.assert_array_equal(pk.values[c][:, 0], s[got])
+            np.testing.assert_array_equal(pk.values[c][:, 1], res.spectra[c]["depth"][k])
+            np.testing.assert_array_equal(pk.values[c][:, 4],
+                                          res.spectra[c]["transit_time"][k])
 
 
 def test_best_periods_match_oracle():
