@@ -402,48 +402,65 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
     obj_flag = core._objective_flag(settings.objective)
     accumulation = core._accumulation_code(settings.accumulation, settings.precision)
     plans = list(plans)
-    seg_list = [sg for plan in plans for sg in plan.segments]
-    segs = (_lib.Segment * max(1, len(seg_list)))()
+    # descriptor pieces: a segment whose periods need different phase-bin layouts is cut
+    # where the width crosses the engine's span limits (outputs do not depend on it)
+    pieces: list[tuple[int, _SegPlan, int, int, int]] = []
+    for c, plan in enumerate(plans):
+        for sg in plan.segments:
+            bd = float(sg.durations.min()) / oversample
+            width = int(np.ceil(sg.p_max / bd)) + oversample + 1
+            if width <= core._SPAN_LIMITS[0] or sg.n_periods < 2:
+                pieces.append((c, sg, 0, sg.n_periods, width))
+                continue
+            sub = CurvePlan(segments=[sg], n_periods=sg.n_periods)
+            per = sg.periods if sg.mode == 0 else sub.periods_at(np.arange(sg.n_periods))
+            for a, b, w in core._width_ranges(per, bd, oversample):
+                pieces.append((c, sg, a, b, w))
+    segs = (_lib.Segment * max(1, len(pieces)))()
     dur_parts: list[np.ndarray] = []
     per_parts: list[np.ndarray] = []
     out_off = p_off = d_off = 0
     curve_base = np.zeros(len(plans), dtype=np.int64)
-    curve_count = np.zeros(len(plans), dtype=np.int64)
-    dur_cache: dict[tuple[float, bytes], np.ndarray] = {}
-    j = 0
-    for c, plan in enumerate(plans):
-        curve_base[c] = out_off
-        curve_count[c] = plan.n_periods
-        for sg in plan.segments:
-            bd = float(sg.durations.min()) / oversample
-            key = (bd, sg.durations.tobytes())
-            db = dur_cache.get(key)
-            if db is None:
-                db = np.asarray(core._duration_bins(sg.durations, bd), dtype=np.int32)
-                dur_cache[key] = db
-            d = segs[j]
-            d.bin_duration = bd
-            d.n_periods = sg.n_periods
-            d.out_offset = out_off
-            d.width = int(np.ceil(sg.p_max / bd)) + oversample + 1
-            d.dur_offset = d_off
-            d.n_dur = int(db.size)
-            d.curve = c
-            d.mode = sg.mode
-            d.lin_start, d.lin_step, d.lin_stop = sg.lin_start, sg.lin_step, sg.lin_stop
-            d.lin_num, d.lin_k0 = sg.lin_num, sg.lin_k0
-            if sg.mode == 0:
-                d.period_offset = p_off
-                per_parts.append(sg.periods)
-                p_off += sg.n_periods
+    curve_count = np.array([plan.n_periods for plan in plans], dtype=np.int64)
+    dur_cache: dict[tuple[float, bytes], tuple[np.ndarray, int]] = {}
+    last_curve = -1
+    for j, (c, sg, a, b, width) in enumerate(pieces):
+        if c != last_curve:
+            for cc in range(last_curve + 1, c + 1):
+                curve_base[cc] = out_off
+            last_curve = c
+        bd = float(sg.durations.min()) / oversample
+        key = (bd, sg.durations.tobytes())
+        hit = dur_cache.get(key)
+        if hit is None:
+            db = np.asarray(core._duration_bins(sg.durations, bd), dtype=np.int32)
+            hit = (db, d_off)
+            dur_cache[key] = hit
             dur_parts.append(db)
             d_off += int(db.size)
-            out_off += sg.n_periods
-            j += 1
+        db, db_off = hit
+        d = segs[j]
+        d.bin_duration = bd
+        d.n_periods = b - a
+        d.out_offset = out_off
+        d.width = int(width)
+        d.dur_offset = db_off
+        d.n_dur = int(db.size)
+        d.curve = c
+        d.mode = sg.mode
+        d.lin_start, d.lin_step, d.lin_stop = sg.lin_start, sg.lin_step, sg.lin_stop
+        d.lin_num, d.lin_k0 = sg.lin_num, sg.lin_k0 + a
+        if sg.mode == 0:
+            d.period_offset = p_off
+            per_parts.append(sg.periods[a:b])
+            p_off += b - a
+        out_off += b - a
+    for cc in range(last_curve + 1, len(plans)):
+        curve_base[cc] = out_off
     return PackedBatch(
         tau=cs.tau, yw=cs.yw, ivar=cs.ivar, total_points=cs.total_points,
         curves=cs.curves32 if accumulation else cs.curves,
-        segments=segs, n_segments=len(seg_list),
+        segments=segs, n_segments=len(pieces),
         periods=_upload(np.concatenate(per_parts) if per_parts else np.zeros(1), np.float64, device),
         dur_bins=_upload(np.concatenate(dur_parts) if dur_parts else np.zeros(1), np.int32, device),
         n_out=out_off, curve_base=curve_base, curve_count=curve_count,

@@ -1454,7 +1454,23 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     int64_t tile = live / std::max<int64_t>(1, target_items);
     tile = std::max<int64_t>(1, std::min<int64_t>(tile, kMaxTile));
     if (ctx->tile > 0) tile = ctx->tile;
-    ctx->h_items.clear();
+    // Work items grouped by the phase-bin layout their segment's width needs (MODE 0:
+    // register-chunk prefix in LDS; 1: in-place slots in LDS; 2: global scratch), one
+    // launch per layout, so a wide segment does not push narrow ones out of LDS. (The
+    // reduced-precision kernel takes every item in one launch.)
+    const int threads = ctx->threads;
+    auto layout_of = [&](int width) {
+        if (ctx->accumulation != 0) return 0;
+        const int w = (width + 63) / 64 * 64;
+        const size_t need = (size_t)16 * (w + (w - 1) / 8 + 1);
+        if (need + 2048 > (size_t)ctx->lds_limit) return 2;
+        return w <= threads * kChunk ? 0 : 1;
+    };
+    struct Layout {
+        int64_t begin = 0, count = 0;
+        int wmax = 2, dmax = 1;
+    } layouts[3];
+    std::vector<WorkItem> by_layout[3];
     std::vector<cpb_segment> empties;
     for (int32_t s = 0; s < n_segments; ++s) {
         const cpb_segment& sg = segments[s];
@@ -1463,13 +1479,22 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
             empties.push_back(sg);
             continue;
         }
+        const int L = layout_of(sg.width);
+        layouts[L].wmax = std::max(layouts[L].wmax, (int)sg.width);
+        layouts[L].dmax = std::max(layouts[L].dmax, (int)sg.n_dur);
         for (int64_t b = 0; b < sg.n_periods; b += tile) {
             WorkItem w;
             w.seg = s;
             w.begin = b;
             w.count = (int32_t)std::min<int64_t>(tile, sg.n_periods - b);
-            ctx->h_items.push_back(w);
+            by_layout[L].push_back(w);
         }
+    }
+    ctx->h_items.clear();
+    for (int L = 0; L < 3; ++L) {
+        layouts[L].begin = (int64_t)ctx->h_items.size();
+        layouts[L].count = (int64_t)by_layout[L].size();
+        ctx->h_items.insert(ctx->h_items.end(), by_layout[L].begin(), by_layout[L].end());
     }
     const int64_t n_items = (int64_t)ctx->h_items.size();
 
@@ -1550,48 +1575,53 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         }
         return CPB_OK;
     }
-    const int threads = ctx->threads;
     const size_t static_lds = 2048;
-    const int n_slots = wl + (wl - 1) / 8 + 1;
-    const size_t lds_need = (size_t)16 * n_slots;
-    const bool fits = lds_need + static_lds <= (size_t)ctx->lds_limit;
-    const int mode = fits ? (wl <= threads * kChunk ? 0 : 1) : 2;
-    int64_t* wide_ptr = nullptr;
-    LaunchArgs la;
-    la.lds = 0;
     const int64_t persistent = (int64_t)ctx->n_cu * ctx->grid_per_cu;
-    if (mode == 2) {
-        const int64_t g = std::min<int64_t>(n_items, persistent > 0 ? persistent : (int64_t)ctx->n_cu * 4);
-        if ((rc = ensure(ctx, ctx->wide, (size_t)g * lds_need))) return rc;
-        wide_ptr = static_cast<int64_t*>(ctx->wide.ptr);
-        la.grid = dim3((unsigned)g);
-    } else {
-        // fewer than 2^32 work-items per launch (a graph kernel node refuses more); the
-        // kernel loops over the items its grid does not cover
-        const int64_t cap = ((int64_t)1 << 31) / threads;
-        la.grid = dim3((unsigned)std::min<int64_t>(n_items, persistent > 0 ? persistent : cap));
-        la.lds = lds_need;
-    }
-    la.st = st;
-    la.tau = tau;
-    la.yq = static_cast<const int64_t*>(ctx->yq.ptr);
-    la.wq = static_cast<const int64_t*>(ctx->wq.ptr);
-    la.curves = static_cast<const CurveParams*>(ctx->curves.ptr);
-    la.segs = sp;
-    la.items = ip;
-    la.n_items = n_items;
-    la.periods = periods;
-    la.dur = dur_bins;
-    la.os = oversample;
-    la.wl = wl;
-    la.wide = wide_ptr;
-    la.out = out;
-    la.n_out = n_out;
     const bool timed = ctx->timing && !ctx->capturing;
     if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
-    const hipError_t e = obj_flag == 0 ? dispatch_threads<0>(threads, mode, dmax, la)
-                                       : dispatch_threads<1>(threads, mode, dmax, la);
-    if (e != hipSuccess) return hip_fail(ctx, e, "bls_main_kernel launch");
+    for (int L = 0; L < 3; ++L) {
+        const Layout& lay = layouts[L];
+        if (lay.count == 0) continue;
+        const int wlL = (lay.wmax + 63) / 64 * 64;
+        const int n_slots = wlL + (wlL - 1) / 8 + 1;
+        const size_t lds_need = (size_t)16 * n_slots;
+        const int mode = L;
+        (void)static_lds;
+        int64_t* wide_ptr = nullptr;
+        LaunchArgs la;
+        la.lds = 0;
+        if (mode == 2) {
+            const int64_t g = std::min<int64_t>(
+                lay.count, persistent > 0 ? persistent : (int64_t)ctx->n_cu * 4);
+            if ((rc = ensure(ctx, ctx->wide, (size_t)g * lds_need))) return rc;
+            wide_ptr = static_cast<int64_t*>(ctx->wide.ptr);
+            la.grid = dim3((unsigned)g);
+        } else {
+            // fewer than 2^32 work-items per launch (a graph kernel node refuses more);
+            // the kernel loops over the items its grid does not cover
+            const int64_t cap = ((int64_t)1 << 31) / threads;
+            la.grid = dim3((unsigned)std::min<int64_t>(lay.count, persistent > 0 ? persistent : cap));
+            la.lds = lds_need;
+        }
+        la.st = st;
+        la.tau = tau;
+        la.yq = static_cast<const int64_t*>(ctx->yq.ptr);
+        la.wq = static_cast<const int64_t*>(ctx->wq.ptr);
+        la.curves = static_cast<const CurveParams*>(ctx->curves.ptr);
+        la.segs = sp;
+        la.items = ip + lay.begin;
+        la.n_items = lay.count;
+        la.periods = periods;
+        la.dur = dur_bins;
+        la.os = oversample;
+        la.wl = wlL;
+        la.wide = wide_ptr;
+        la.out = out;
+        la.n_out = n_out;
+        const hipError_t e = obj_flag == 0 ? dispatch_threads<0>(threads, mode, lay.dmax, la)
+                                           : dispatch_threads<1>(threads, mode, lay.dmax, la);
+        if (e != hipSuccess) return hip_fail(ctx, e, "bls_main_kernel launch");
+    }
     if (timed) {
         CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
         ctx->timed = true;

@@ -241,21 +241,23 @@ def _bls_search_hip_specs(
     dur_all = [np.asarray(spec.dur_bins, dtype=np.int32) for spec in specs]
     dur_h = np.concatenate(dur_all) if dur_all else np.zeros(0, dtype=np.int32)
     dur_d = torch.from_numpy(np.ascontiguousarray(dur_h if dur_h.size else np.zeros(1, np.int32))).to(device)
-    segs = (_lib.Segment * len(specs))()
-    p_off = d_off = 0
-    for k, spec in enumerate(specs):
-        sg = segs[k]
-        sg.bin_duration = float(spec.bin_duration)
-        sg.period_offset = p_off
-        sg.n_periods = sizes[k]
-        sg.out_offset = p_off
-        sg.width = int(spec.width)
-        sg.dur_offset = d_off
+    pieces = [(k, a, b, w) for k, spec in enumerate(specs)
+              for a, b, w in (_width_ranges(spec.periods, spec.bin_duration, oversample)
+                              or [(0, 0, int(spec.width))])]
+    segs = (_lib.Segment * max(1, len(pieces)))()
+    d_offs = np.concatenate([[0], np.cumsum([d.shape[0] for d in dur_all])]).astype(int)
+    p_base = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+    for j, (k, a, b, w) in enumerate(pieces):
+        sg = segs[j]
+        sg.bin_duration = float(specs[k].bin_duration)
+        sg.period_offset = int(p_base[k] + a)
+        sg.n_periods = b - a
+        sg.out_offset = int(p_base[k] + a)
+        sg.width = int(w)
+        sg.dur_offset = int(d_offs[k])
         sg.n_dur = int(dur_all[k].shape[0])
         sg.curve = 0
         sg.mode = 0
-        p_off += sizes[k]
-        d_off += int(dur_all[k].shape[0])
     if int(max(spec.width for spec in specs)) > (1 << 30):
         raise BackendUnavailableError("BLS: phase-bin count above 2^30 per period")
     out = torch.empty((_lib.N_OUTPUTS, total), dtype=torch.float64, device=device)
@@ -265,7 +267,7 @@ def _bls_search_hip_specs(
         status = ctx.lib.cpb_bls_search(
             ctx.handle, _stream_handle(device), _ptr(data["tau"]), _ptr(data["yw"]),
             _ptr(data["ivar"]), data["n"], data["t_min"], floor, _ptr(periods_d),
-            _ptr(dur_d), segs, len(specs), int(oversample), int(obj_flag), _ptr(out),
+            _ptr(dur_d), segs, len(pieces), int(oversample), int(obj_flag), _ptr(out),
             total,
         )
         ctx.check(status, "search")
@@ -277,6 +279,39 @@ def _bls_search_hip_specs(
         results.append({name: host[i, sl].copy() for i, name in enumerate(_OUTPUT_NAMES)})
         p_off += n_periods
     return results
+
+
+#: Phase-bin spans (bins per period, padded) the engine keeps in registers and LDS
+#: (256 threads x 8 bins) and in LDS at all (160 KiB); wider spans go to global memory.
+_SPAN_LIMITS = (2048, 8960)
+
+
+def _width_of(p_max: float, bin_duration: float, oversample: int) -> int:
+    """``_search_spec``'s histogram width for periods up to ``p_max``."""
+    return int(math.ceil(p_max / bin_duration)) + int(oversample) + 1
+
+
+def _width_ranges(periods: np.ndarray, bin_duration: float, oversample: int
+                  ) -> list[tuple[int, int, int]]:
+    """``[(start, stop, width)]``: a monotone period array cut where the width its
+    periods need crosses the engine's span limits, so narrow periods keep the fast
+    layouts. A period's outputs do not depend on the width (``n_bins = min(ceil(P/bd)
+    + os, width - 1)`` never binds, and no bin index reaches ``width - 1``); unsorted
+    arrays stay whole."""
+    n = int(periods.size)
+    whole = [(0, n, _width_of(float(periods.max()), bin_duration, oversample))] if n else []
+    if n < 2:
+        return whole
+    d = np.diff(periods)
+    if not (np.all(d > 0) or np.all(d < 0)):
+        return whole
+    if whole[0][2] <= _SPAN_LIMITS[0]:
+        return whole
+    need = np.ceil(periods / bin_duration).astype(np.int64) + int(oversample) + 1
+    cls = np.searchsorted(np.asarray(_SPAN_LIMITS), need, side="left")
+    cuts = np.flatnonzero(np.diff(cls)) + 1
+    bounds = [0, *cuts.tolist(), n]
+    return [(a, b, int(need[a:b].max())) for a, b in zip(bounds[:-1], bounds[1:])]
 
 
 # --- public entry points -------------------------------------------------------------
