@@ -15,6 +15,11 @@ peak; ``detail`` adds point-folds/s, box-evaluations/s and the LDS bytes they im
 
 ``cpu_baseline``: the FP64 C++ oracle (the reference's numba kernel restated, with its
 screen) on the first ``--cpu-curves`` curves of the same batch, all host threads.
+
+``next_rows``: the joint-ephemeris multi-band search (``multiband_survey``: 6 bands x 150
+points over 1,095 d per star, the default BLS grid) and the TLS matched filter
+(``tls_survey``: 20,000 points over 27 d per curve, the default TLS grid), each timed over
+one batched launch and beside its own CPU oracle.
 """
 
 from __future__ import annotations
@@ -154,6 +159,165 @@ def learned_leg(pb, wl, settings, plain, plain_s: float, steps: int) -> dict:
     }
 
 
+def _kernel_ms(ctx) -> float:
+    import ctypes
+
+    ms = ctypes.c_double()
+    ctx.check(ctx.lib.cpb_main_kernel_ms(ctx.handle, ctypes.byref(ms)), "timing")
+    return ms.value
+
+
+def multiband_leg(n_stars: int, cpu: bool) -> dict:
+    """Joint-ephemeris multi-band BLS: stars/s over one batched launch, the sweep's
+    box-band evaluations, and the CPU oracle on one star."""
+    from cuperiod_amd import _lib
+    from cuperiod_amd.core.config import BLSSettings
+    from cuperiod_amd.core.lightcurve import LightCurve
+    from cuperiod_amd.methods import bls as bls_mod
+    from cuperiod_amd.methods._bls_core import _search_spec
+    from cuperiod_amd.methods._bls_joint import joint_bands, joint_search_stars
+    from cuperiod_amd.workloads import multiband_survey
+
+    stars, truth = multiband_survey(n_stars)
+    settings = BLSSettings(mb_ephemeris="shared")
+    prepared = [joint_bands(*zip(*s)) for s in stars]
+    t_all = np.sort(np.concatenate([b[0] for b in stars[0]]))
+    grid = bls_mod.BLSMethod().default_grid(
+        LightCurve.from_arrays(t_all, np.ones(t_all.size), domain="flux"), settings)
+    segs = bls_mod._grid_segments(grid, settings)
+    os_ = settings.bins_per_duration
+    boxes = 0.0
+    for periods, durations in segs:
+        spec = _search_spec(periods, durations, os_)
+        nb = np.minimum(np.ceil(periods / spec.bin_duration).astype(np.int64) + os_,
+                        spec.width - 1)
+        for d in spec.dur_bins:
+            boxes += float(np.sum(np.where(nb > d, nb - d + 1, 0)))
+    k_bands = [p.n_bands for p in prepared]
+    box_band = boxes * float(sum(k_bands))
+    folds = float(grid.values.size) * float(sum(p.tau.size for p in prepared))
+    ctx = _lib.context(0)
+    ctx.lib.cpb_set_timing(ctx.handle, 1)
+    joint_search_stars(prepared[:2], [segs] * 2, os_, objective="snr", backend="hip")
+    t0 = time.perf_counter()
+    found = joint_search_stars(prepared, [segs] * len(prepared), os_, objective="snr",
+                               backend="hip")
+    wall = time.perf_counter() - t0
+    k_ms = _kernel_ms(ctx)
+    periods = np.concatenate([p for p, _ in segs])
+    hits = 0
+    for f, p_true in zip(found, truth):
+        if p_true is None:
+            continue
+        power = np.concatenate([r["power"] for r in f])
+        p_best = periods[int(np.argmax(power))]
+        hits += any(abs(p_best * k / p_true - 1.0) < 0.01 for k in (0.5, 1.0, 2.0))
+    out = {
+        "workload": f"joint-ephemeris multi-band BLS, {len(stars)} stars x 6 bands x 150 "
+                    f"points over 1095 d, {grid.values.size} default-grid periods per star",
+        "kernel": "bls_joint_kernel",
+        "value": round(len(stars) / (k_ms * 1e-3), 3),
+        "unit": "stars/s",
+        "kernel_ms": round(k_ms, 3),
+        "wall_s": round(wall, 3),
+        "box_band_evals_per_s": box_band / (k_ms * 1e-3),
+        "point_folds_per_s": folds / (k_ms * 1e-3),
+        "transits_recovered": f"{hits}/{sum(t is not None for t in truth)}",
+        "cpu_baseline": None,
+    }
+    if cpu:
+        from oracle import multiband as omb
+        from oracle import default_threads
+
+        n = 2
+        t0 = time.perf_counter()
+        refs = [omb.joint_search(omb.joint_bands(*zip(*s)), segs, os_,
+                                 threads=default_threads()) for s in stars[:n]]
+        el = time.perf_counter() - t0
+        diff = 0.0
+        for f, ref in zip(found, refs):
+            got = np.concatenate([r["power"] for r in f])
+            want = np.concatenate([r["power"] for r in ref])
+            diff = max(diff, float(np.max(np.abs(got - want))
+                                   / max(1e-300, float(np.max(np.abs(want))))))
+        out["cpu_baseline"] = {
+            "value": round(n / el, 4), "unit": "stars/s", "cores": default_threads(),
+            "kind": "port", "sample": f"the first {n} stars ({el:.1f} s)",
+            "gpu_over_cpu": round(out["value"] / (n / el), 1),
+            "max_rel_power_diff_of_max": diff,
+        }
+    return out
+
+
+def tls_leg(n_curves: int, cpu: bool) -> dict:
+    """TLS matched filter: curves/s over one batched launch, point-folds and template
+    taps, and the CPU oracle on a few curves."""
+    from cuperiod_amd import _lib
+    from cuperiod_amd.core.config import TLSSettings
+    from cuperiod_amd.core.lightcurve import LightCurve
+    from cuperiod_amd.methods import tls as tls_mod
+    from cuperiod_amd.workloads import tls_survey
+
+    curves, truth = tls_survey(n_curves)
+    settings = TLSSettings()
+    method = tls_mod.TLSMethod()
+    periods = [method.default_grid(LightCurve.from_arrays(t, y, e, domain="flux"),
+                                   settings).period for t, y, e in curves]
+    widths = tls_mod._duration_bins(settings)
+    n_bins = settings.n_phase_bins
+    folds = float(sum(p.size * t.size for p, (t, _, _) in zip(periods, curves)))
+    taps = float(sum(p.size for p in periods)) * n_bins * float(sum(widths))
+    ctx = _lib.context(0)
+    ctx.lib.cpb_set_timing(ctx.handle, 1)
+    tls_mod.tls_power_batch(curves[:2], periods[:2], settings=settings)
+    t0 = time.perf_counter()
+    res = tls_mod.tls_power_batch(curves, periods, settings=settings)
+    wall = time.perf_counter() - t0
+    k_ms = _kernel_ms(ctx)
+    hits = 0
+    for r, p, p_true in zip(res, periods, truth):
+        if p_true is not None:
+            p_best = p[int(np.argmax(r["sde"]))]
+            hits += any(abs(p_best * k / p_true - 1.0) < 0.01 for k in (0.5, 1.0, 2.0))
+    # FP64 operations: 4 per point-fold (divide, floor, subtract, scale), 6 per template
+    # tap (three multiply-adds)
+    flops = 4.0 * folds + 6.0 * taps
+    out = {
+        "workload": f"TLS, {len(curves)} curves x 20000 points over 27 d, "
+                    f"{periods[0].size} default-grid periods, {n_bins} phase bins, "
+                    f"widths {widths}",
+        "kernel": "tls_kernel",
+        "value": round(len(curves) / (k_ms * 1e-3), 3),
+        "unit": "light curves/s",
+        "kernel_ms": round(k_ms, 3),
+        "wall_s": round(wall, 3),
+        "point_folds_per_s": folds / (k_ms * 1e-3),
+        "template_taps_per_s": taps / (k_ms * 1e-3),
+        "fp64_tflops": round(flops / (k_ms * 1e-3) / 1e12, 4),
+        "fp64_frac_of_78.6": round(flops / (k_ms * 1e-3) / 78.6e12, 5),
+        "transits_recovered": f"{hits}/{sum(t is not None for t in truth)}",
+        "cpu_baseline": None,
+    }
+    if cpu:
+        from oracle import default_threads
+        from oracle import tls as otls
+
+        n = min(64, len(curves))
+        t0 = time.perf_counter()
+        refs = [otls.tls_power(t, y, e, p, threads=default_threads())
+                for (t, y, e), p in zip(curves[:n], periods[:n])]
+        el = time.perf_counter() - t0
+        diff = max(float(np.max(np.abs(r["sr"] - g["sr"]) / np.maximum(1e-300, np.abs(r["sr"]))))
+                   for r, g in zip(refs, res[:n]))
+        out["cpu_baseline"] = {
+            "value": round(n / el, 4), "unit": "light curves/s", "cores": default_threads(),
+            "kind": "port", "sample": f"the first {n} curves ({el:.1f} s)",
+            "gpu_over_cpu": round(out["value"] / (n / el), 1),
+            "max_rel_sr_diff": diff,
+        }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +333,10 @@ def main() -> int:
                     help="launch each step directly instead of replaying a captured graph")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned proposer + refinement leg")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the multi-band and TLS legs")
+    ap.add_argument("--mb-stars", type=int, default=32)
+    ap.add_argument("--tls-curves", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 256 MiB device copy (the HBM counter scale)")
     args = ap.parse_args()
@@ -288,6 +456,7 @@ def main() -> int:
         },
         "launch": launch,
         "learned": None,
+        "next_rows": None,
         "cpu_baseline": None,
     }
     from cuperiod_amd.learned.proposer import DEFAULT_WEIGHTS
@@ -298,6 +467,14 @@ def main() -> int:
                                       max(1, min(args.steps, 2)))
         log(f"learned: {line['learned']['value']} curves/s, match "
             f"{line['learned']['best_period_match']:.4f}")
+    if not args.no_next:
+        log("multi-band joint search ...")
+        mb = multiband_leg(args.mb_stars, not args.no_cpu)
+        log(f"multi-band: {mb['value']} stars/s ({mb['kernel_ms']} ms)")
+        log("TLS ...")
+        tl = tls_leg(args.tls_curves, not args.no_cpu)
+        log(f"TLS: {tl['value']} curves/s ({tl['kernel_ms']} ms)")
+        line["next_rows"] = {"multiband": mb, "tls": tl}
     if not args.no_cpu and args.cpu_curves > 0:
         n = min(args.cpu_curves, len(pb.plans))
         log(f"CPU oracle on {n} curves ...")

@@ -9,7 +9,7 @@ for gfx950 behind the C ABI of ``include/cuperiod_bls.h`` (backend name ``"hip"`
 from cuperiod_amd.api import best_periods, periodogram, to_input
 from cuperiod_amd.batch.runner import BatchSummary, batch_periodograms
 from cuperiod_amd.core.columns import Domain
-from cuperiod_amd.core.config import BLSSettings
+from cuperiod_amd.core.config import BLSSettings, TLSSettings
 from cuperiod_amd.core.errors import (
     BackendUnavailableError,
     CuPeriodError,
@@ -20,11 +20,13 @@ from cuperiod_amd.core.errors import (
     UnknownMethodError,
 )
 from cuperiod_amd.core.grid import GridSpec, log_period_grid
-from cuperiod_amd.core.lightcurve import LightCurve
+from cuperiod_amd.core.lightcurve import LightCurve, MultiBandLightCurve
 from cuperiod_amd.core.result import MultiResult, Peak, Periodogram
 from cuperiod_amd.methods import bls as _bls  # noqa: F401
+from cuperiod_amd.methods import tls as _tls  # noqa: F401
 from cuperiod_amd.methods._bls_core import BLSPower, bls_power, bls_power_segments
 from cuperiod_amd.methods.base import get_method, list_methods, method_names
+from cuperiod_amd.methods.tls import tls_power
 
 __version__ = "0.1.0"
 
@@ -41,9 +43,11 @@ __all__ = [
     "GridTooLargeError",
     "InsufficientDataError",
     "LightCurve",
+    "MultiBandLightCurve",
     "MultiResult",
     "Peak",
     "Periodogram",
+    "TLSSettings",
     "UnknownMethodError",
     "batch_periodograms",
     "best_periods",
@@ -54,5 +58,6 @@ __all__ = [
     "log_period_grid",
     "method_names",
     "periodogram",
+    "tls_power",
     "to_input",
 ]

@@ -29,6 +29,8 @@ CPB_E_DEVICE = 2
 CPB_E_RESOURCE = 3
 CPB_E_NODEVICE = 4
 N_OUTPUTS = 7
+N_JOINT_OUTPUTS = 6
+N_TLS_OUTPUTS = 4
 
 #: Every symbol the header declares, in declaration order.
 EXPORTED_SYMBOLS = (
@@ -42,6 +44,8 @@ EXPORTED_SYMBOLS = (
     "cpb_bls_search_batch",
     "cpb_bls_summarize",
     "cpb_peak_candidates",
+    "cpb_bls_joint_search",
+    "cpb_tls_search",
     "cpb_linspace_grid",
     "cpb_set_timing",
     "cpb_main_kernel_ms",
@@ -100,6 +104,39 @@ class Curve(ctypes.Structure):
     ]
 
 
+class Band(ctypes.Structure):
+    """``cpb_band``: one band of a multi-band star."""
+
+    _fields_ = [
+        ("point_offset", ctypes.c_int64),
+        ("n_points", ctypes.c_int64),
+        ("ivar_floor", ctypes.c_double),
+    ]
+
+
+class Star(ctypes.Structure):
+    """``cpb_star``: the bands of one star and their common time origin."""
+
+    _fields_ = [
+        ("band_offset", ctypes.c_int32),
+        ("n_bands", ctypes.c_int32),
+        ("t_min", ctypes.c_double),
+    ]
+
+
+class TlsCurve(ctypes.Structure):
+    """``cpb_tls_curve``: one light curve of a TLS search."""
+
+    _fields_ = [
+        ("point_offset", ctypes.c_int64),
+        ("n_points", ctypes.c_int64),
+        ("period_offset", ctypes.c_int64),
+        ("n_periods", ctypes.c_int64),
+        ("out_offset", ctypes.c_int64),
+        ("w_total", ctypes.c_double),
+    ]
+
+
 _LOCK = threading.Lock()
 _LIB: ctypes.CDLL | None = None
 _LOAD_ERROR: str | None = None
@@ -134,6 +171,16 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.cpb_bls_summarize.argtypes = [vp, vp, vp, vp, vp, i32, vp]
     lib.cpb_peak_candidates.restype = ctypes.c_int
     lib.cpb_peak_candidates.argtypes = [vp, vp, vp, i64, vp, vp, i32, i32, i32, vp, vp, vp]
+    lib.cpb_bls_joint_search.restype = ctypes.c_int
+    lib.cpb_bls_joint_search.argtypes = [
+        vp, vp, vp, vp, vp, i64, ctypes.POINTER(Band), i32, ctypes.POINTER(Star), i32, vp,
+        vp, ctypes.POINTER(Segment), i32, i32, i32, vp, i64,
+    ]
+    lib.cpb_tls_search.restype = ctypes.c_int
+    lib.cpb_tls_search.argtypes = [
+        vp, vp, vp, vp, vp, i64, ctypes.POINTER(TlsCurve), i32, vp, i32,
+        ctypes.POINTER(ctypes.c_int32), i32, ctypes.POINTER(f64), f64, vp, i64,
+    ]
     lib.cpb_linspace_grid.restype = ctypes.c_int
     lib.cpb_linspace_grid.argtypes = [f64, f64, i64, ctypes.POINTER(f64)]
     lib.cpb_set_timing.restype = ctypes.c_int
@@ -282,10 +329,15 @@ __all__ = [
     "EXPORTED_SYMBOLS",
     "PROPOSER_SYMBOLS",
     "LIB_PATH",
+    "N_JOINT_OUTPUTS",
     "N_OUTPUTS",
+    "N_TLS_OUTPUTS",
+    "Band",
     "Context",
     "Curve",
     "Segment",
+    "Star",
+    "TlsCurve",
     "context",
     "device_count",
     "library_available",

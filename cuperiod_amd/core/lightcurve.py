@@ -1,4 +1,4 @@
-"""Single-band light curve: the input of every BLS search.
+"""Light curves: the input of every BLS search.
 
 The subset of cuPeriod's ``LightCurve`` the BLS path reads
 (src/cuperiod/core/lightcurve.py:58-303, 625-646): construction from arrays, the
@@ -6,6 +6,10 @@ The subset of cuPeriod's ``LightCurve`` the BLS path reads
 magnitude -> flux conversion BLS applies before searching (a transit is a dip in flux),
 ``baseline`` and the unusable-error diagnosis. Times are kept verbatim; the search
 subtracts its own time origin in FP64.
+
+:class:`MultiBandLightCurve` holds several bands of one star for the multi-band search
+(src/cuperiod/core/lightcurve.py:329-575): every band a :class:`LightCurve`, all in one
+brightness domain, in insertion order.
 """
 
 from __future__ import annotations
@@ -150,7 +154,98 @@ class LightCurve:
         )
 
 
-def raise_if_unusable(data: LightCurve, label: str) -> None:
+@dataclass(frozen=True, eq=False)
+class MultiBandLightCurve:
+    """Several bands (filters) of one star: ``bands`` maps band name -> light curve, in
+    insertion order; ``meta`` is carried through results."""
+
+    bands: Mapping[str, LightCurve]
+    meta: Mapping[str, Any] = field(default_factory=dict)
+
+    def __post_init__(self) -> None:
+        if not self.bands:
+            raise ValueError("MultiBandLightCurve requires at least one band")
+        for name, lc in self.bands.items():
+            if not isinstance(lc, LightCurve):
+                raise TypeError(
+                    f"band {name!r} is a {type(lc).__name__}, not a LightCurve; "
+                    "build each band with LightCurve.from_arrays(...)"
+                )
+        domains = {name: lc.domain for name, lc in self.bands.items()}
+        if len(set(domains.values())) > 1:
+            listing = ", ".join(f"{k}: {v.value}" for k, v in domains.items())
+            raise ValueError(
+                f"bands are in different brightness domains ({listing}); convert them "
+                "with LightCurve.in_domain(...), or pass domain='magnitude' or "
+                "domain='flux' to periodogram()"
+            )
+
+    @property
+    def band_names(self) -> tuple[str, ...]:
+        """The band labels, in order."""
+        return tuple(self.bands)
+
+    @property
+    def n_bands(self) -> int:
+        """Number of bands."""
+        return len(self.bands)
+
+    @classmethod
+    def from_light_curves(
+        cls, bands: Mapping[str, LightCurve], *, meta: Mapping[str, Any] | None = None
+    ) -> "MultiBandLightCurve":
+        """Build from an explicit ``{band: LightCurve}`` mapping."""
+        return cls(bands=dict(bands), meta=dict(meta or {}))
+
+    def finite(self) -> "MultiBandLightCurve":
+        """Every band without its non-finite samples and non-positive errors."""
+        return MultiBandLightCurve(
+            bands={k: lc.finite() for k, lc in self.bands.items()}, meta=self.meta
+        )
+
+    def in_domain(self, domain: Domain | str) -> "MultiBandLightCurve":
+        """Every band converted to ``domain``."""
+        return MultiBandLightCurve(
+            bands={k: lc.in_domain(domain) for k, lc in self.bands.items()}, meta=self.meta
+        )
+
+    def unusable_reason(self) -> str | None:
+        """Why the errors leave no usable point in any band, or ``None``: a band wiped
+        out by its errors does not condemn the others."""
+        reasons: dict[str, str] = {}
+        for name, lc in self.bands.items():
+            reason = lc.unusable_reason()
+            if reason is not None:
+                reasons[name] = reason
+            elif lc._usable().any():
+                return None
+        if not reasons:
+            return None
+        return "; ".join(f"band {name!r}: {why}" for name, why in reasons.items())
+
+    def stacked(self) -> tuple[np.ndarray, np.ndarray, np.ndarray | None, np.ndarray]:
+        """All bands as time-sorted ``(time, value, error, band)`` arrays (stable sort;
+        ``error`` is ``None`` when a band has none)."""
+        times, values, errors, labels = [], [], [], []
+        missing = False
+        for name, lc in self.bands.items():
+            times.append(lc.time)
+            values.append(lc.value)
+            if lc.error is None:
+                missing = True
+            else:
+                errors.append(lc.error)
+            labels.append(np.full(lc.n, name, dtype=object))
+        time = np.concatenate(times)
+        value = np.concatenate(values)
+        band = np.concatenate(labels)
+        error = None if missing else np.concatenate(errors)
+        order = np.argsort(time, kind="stable")
+        return (time[order], value[order], None if error is None else error[order],
+                band[order])
+
+
+def raise_if_unusable(data: "LightCurve | MultiBandLightCurve", label: str) -> None:
     """Raise :class:`InsufficientDataError` when an unusable error column would empty
     ``data`` (src/cuperiod/core/lightcurve.py:625-646)."""
     reason = data.unusable_reason()
@@ -158,4 +253,4 @@ def raise_if_unusable(data: LightCurve, label: str) -> None:
         raise InsufficientDataError(f"{label}: {reason}")
 
 
-__all__ = ["LightCurve", "raise_if_unusable"]
+__all__ = ["LightCurve", "MultiBandLightCurve", "raise_if_unusable"]

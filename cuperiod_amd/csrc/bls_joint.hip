@@ -1,0 +1,502 @@
+// MI355X (gfx950) joint-ephemeris multi-band box search: one box shared by every band
+// of a star, each band fitting its own depth inside it (cuPeriod
+// src/cuperiod/multiband/_bls_joint.py:1-26, kernels :419-503 and :531-685).
+//
+// One workgroup takes one (star, period segment, tile of periods) work item and, within
+// it, one trial period after another:
+//
+//   (a) fold    every point of every band into that band's phase bins, against the
+//               star's common time origin, with the reference's FP64 bin expression
+//               (fold_bin: bit for bit the FP64 division's index) and 64-bit
+//               fixed-point sums per band (ds_add_u64: exact, order-free);
+//   (a') wrap   each band's copy of `oversample` bins (:450-457), in order;
+//   (b) scan    per band, inclusive prefix sums over [0, n_bins] in rounds of 64 bins
+//               (one wave per band, DPP scans carried across rounds);
+//   (c) sweep   every box (duration index, start n) of the period: each band's in-box
+//               sums are exact integer differences; a band adds
+//               max(depth,0)^2/(1/W_in + 1/W_out) (snr) or W_in*depth^2/2 (likelihood)
+//               when both of its sides clear its floor; the box counts when a band
+//               dips; bands are summed in band order; the first strict maximum in
+//               (duration index, start) order wins (:469-494);
+//   (d) finish  each band's depth and SNR at the winning box in FP64, the quadrature
+//               power, the strongest band and the box geometry (:359-406).
+//
+// The band prefix arrays of a period stay in LDS (K bands x width x 16 B) when they fit,
+// else in a per-workgroup slice of global scratch (an L2-resident tile). The sweep reads
+// 16-byte (y, w) pairs at consecutive starts across the lanes: conflict-free LDS reads.
+// In the sweep the box value is formed from the integer sums as
+// t^2 / (W_in W_out W_tot) * 2^(2 e_y - e_w) with t = Y_tot W_in - Y_in W_tot, which is
+// the reference's depth^2/(1/W_in + 1/W_out) without its divisions and cancellations.
+
+#include "engine_internal.h"
+
+using namespace cpb_engine;
+
+namespace {
+
+constexpr int kJointMaxBands = 64;
+
+// Per band of the star being searched, in LDS: the quantities the sweep reads for every
+// box (fixed-point units of the band).
+struct BandSweep {
+    double y_tot;   // sum of fixed-point y*ivar
+    double w_tot;   // sum of fixed-point ivar
+    double floor_q; // ivar floor in fixed-point units
+    int64_t wq_tot; // exact w_tot
+    int32_t scale;  // 2*y_exp - w_exp
+    int32_t pad;
+};
+
+__device__ __forceinline__ bool first_wins(double a, uint32_t ka, double b, uint32_t kb) {
+    return a > b || (a == b && ka < kb);
+}
+
+template <int T, int DMAX, int OBJ, bool GLOBAL>
+__global__ __launch_bounds__(T) void bls_joint_kernel(
+    const double* __restrict__ tau, const int64_t* __restrict__ yq,
+    const int64_t* __restrict__ wq, const CurveParams* __restrict__ bp,
+    const cpb_star* __restrict__ stars, const cpb_segment* __restrict__ segs,
+    const WorkItem* __restrict__ items, int64_t n_items, const double* __restrict__ periods,
+    const int32_t* __restrict__ dur_bins, int32_t oversample, int32_t wl, int64_t slice,
+    int64_t* __restrict__ scratch, double* __restrict__ out, int64_t n_out) {
+    extern __shared__ int64_t dyn_lds[];
+    constexpr int NW = T / kWave;
+    __shared__ BandSweep s_band[kJointMaxBands];
+    __shared__ int64_t s_cum[kJointMaxBands + 1];
+    __shared__ int32_t s_dur[kMaxDurations];
+    __shared__ double s_red_v[NW];
+    __shared__ uint32_t s_red_k[NW];
+    __shared__ double s_snr[kJointMaxBands];
+    __shared__ double s_depth[kJointMaxBands];
+
+    const int tid = threadIdx.x;
+    const int lane = tid % kWave;
+    const int wave = tid / kWave;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const WorkItem it = items[item];
+        const cpb_segment sg = segs[it.seg];
+        const cpb_star star = stars[sg.curve];
+        const int K = star.n_bands;
+        const int n_dur = sg.n_dur;
+        const double bd = sg.bin_duration;
+        const double inv_bd = 1.0 / bd;
+        const int width = sg.width;
+        int64_t* S;  // bins of band k: S[2*(k*wl + i)] = y, S[2*(k*wl + i) + 1] = w
+        if constexpr (GLOBAL)
+            S = scratch + (int64_t)blockIdx.x * slice;
+        else
+            S = dyn_lds;
+
+        __syncthreads();
+        if (tid < n_dur) s_dur[tid] = dur_bins[sg.dur_offset + tid];
+        if (tid < K) {
+            const CurveParams c = bp[star.band_offset + tid];
+            BandSweep b;
+            b.y_tot = (double)c.y_tot;
+            b.w_tot = (double)c.w_tot;
+            b.floor_q = ldexp(c.ivar_floor, -c.w_exp);
+            b.wq_tot = c.w_tot;
+            b.scale = 2 * c.y_exp - c.w_exp;
+            b.pad = 0;
+            s_band[tid] = b;
+        }
+        if (tid == 0) {
+            int64_t acc = 0;
+            for (int k = 0; k < K; ++k) {
+                s_cum[k] = acc;
+                acc += bp[star.band_offset + k].n_points;
+            }
+            s_cum[K] = acc;
+        }
+        __syncthreads();
+        const int64_t n_total = s_cum[K];
+
+        for (int q = 0; q < it.count; ++q) {
+            const int64_t col = sg.out_offset + it.begin + q;
+            const double P = segment_period(sg, periods, it.begin + q);
+            const double invP = 1.0 / P;
+            int64_t nb64 = (int64_t)ceil(P / bd) + oversample;
+            if (nb64 > width - 1) nb64 = width - 1;
+            const int nb = (int)nb64;
+
+            // (a) zero bins [0, nb] of every band, then fold every point.
+            {
+                longlong2* S2 = reinterpret_cast<longlong2*>(S);
+                const int span = nb + 1;
+                for (int i = tid; i < K * span; i += T) {
+                    const int k = i / span;
+                    S2[k * wl + (i - k * span)] = make_longlong2(0, 0);
+                }
+            }
+            if constexpr (GLOBAL) __threadfence_block();
+            __syncthreads();
+            {
+                int k = 0;
+                for (int64_t j = tid; j < n_total; j += T) {
+                    while (j >= s_cum[k + 1]) ++k;
+                    const int64_t p = bp[star.band_offset + k].point_offset + (j - s_cum[k]);
+                    const int b = fold_bin(tau[p], P, invP, bd, inv_bd, width);
+                    int64_t* slot = S + 2 * ((int64_t)k * wl + b);
+                    atomicAdd(reinterpret_cast<unsigned long long*>(slot),
+                              (unsigned long long)yq[p]);
+                    atomicAdd(reinterpret_cast<unsigned long long*>(slot + 1),
+                              (unsigned long long)wq[p]);
+                }
+            }
+            if constexpr (GLOBAL) __threadfence_block();
+            __syncthreads();
+            // (a') wrap copy per band, in order (a copy may read a bin an earlier one wrote).
+            for (int k = tid; k < K; k += T) {
+                longlong2* B = reinterpret_cast<longlong2*>(S) + (int64_t)k * wl;
+                for (int o = 0; o < oversample; ++o) {
+                    int dst = nb - oversample + o;
+                    dst = dst < 0 ? 0 : (dst > width - 1 ? width - 1 : dst);
+                    B[dst] = B[1 + o];
+                }
+            }
+            if constexpr (GLOBAL) __threadfence_block();
+            __syncthreads();
+            // (b) inclusive prefix over [0, nb], one wave per band, rounds of 64 bins.
+            for (int k = wave; k < K; k += NW) {
+                longlong2* B = reinterpret_cast<longlong2*>(S) + (int64_t)k * wl;
+                int64_t cy = 0, cw = 0;
+                for (int base = 0; base <= nb; base += kWave) {
+                    const int i = base + lane;
+                    longlong2 v = make_longlong2(0, 0);
+                    if (i <= nb) v = B[i];
+                    int64_t y = wave_incl_scan_dpp(v.x) + cy;
+                    int64_t w = wave_incl_scan_dpp(v.y) + cw;
+                    if (i <= nb) B[i] = make_longlong2(y, w);
+                    cy = readlane_i64(y, kWave - 1);
+                    cw = readlane_i64(w, kWave - 1);
+                }
+            }
+            if constexpr (GLOBAL) __threadfence_block();
+            __syncthreads();
+
+            // (c) sweep: thread-strided starts, every duration per start.
+            double best = -INFINITY;
+            uint32_t bkey = 0xFFFFFFFFu;
+            for (int n = tid; n < nb; n += T) {
+                double tot[DMAX];
+#pragma unroll
+                for (int di = 0; di < DMAX; ++di) tot[di] = 0.0;
+                uint32_t hit = 0;
+                for (int k = 0; k < K; ++k) {
+                    const BandSweep b = s_band[k];
+                    const longlong2* B = reinterpret_cast<const longlong2*>(S) + (int64_t)k * wl;
+                    const longlong2 p0 = B[n];
+#pragma unroll
+                    for (int di = 0; di < DMAX; ++di) {
+                        if (di < n_dur) {
+                            const int d = s_dur[di];
+                            if (d < nb && n + d <= nb) {
+                                const longlong2 p1 = B[n + d];
+                                const int64_t yi = p1.x - p0.x;
+                                const int64_t wi = p1.y - p0.y;
+                                const double Wi = (double)wi;
+                                const double Wo = (double)(b.wq_tot - wi);
+                                if (Wi >= b.floor_q && Wo >= b.floor_q) {
+                                    const double t = fma(b.y_tot, Wi, -((double)yi * b.w_tot));
+                                    if (t >= 0.0) {
+                                        hit |= 1u << di;
+                                        const double den =
+                                            OBJ == 0 ? Wi * Wo * b.w_tot : 2.0 * Wi * Wo * Wo;
+                                        tot[di] += ldexp(t * t / den, b.scale);
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int di = 0; di < DMAX; ++di) {
+                    if ((hit >> di) & 1u) {
+                        const uint32_t key = ((uint32_t)di << 24) | (uint32_t)n;
+                        if (first_wins(tot[di], key, best, bkey)) {
+                            best = tot[di];
+                            bkey = key;
+                        }
+                    }
+                }
+            }
+            // first-wins maximum over the workgroup
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(best, off, kWave);
+                const uint32_t ok = (uint32_t)__shfl_xor((int)bkey, off, kWave);
+                if (first_wins(ov, ok, best, bkey)) {
+                    best = ov;
+                    bkey = ok;
+                }
+            }
+            if (lane == 0) {
+                s_red_v[wave] = best;
+                s_red_k[wave] = bkey;
+            }
+            __syncthreads();
+            best = s_red_v[0];
+            bkey = s_red_k[0];
+            for (int w = 1; w < NW; ++w)
+                if (first_wins(s_red_v[w], s_red_k[w], best, bkey)) {
+                    best = s_red_v[w];
+                    bkey = s_red_k[w];
+                }
+            const bool found = best > -INFINITY;
+            const int bn = found ? (int)(bkey & 0xFFFFFFu) : 0;
+            const int bdur = found ? s_dur[bkey >> 24] : 0;
+
+            // (d) each band's depth and SNR at the shared box, then the fields.
+            if (found) {
+                for (int k = tid; k < K; k += T) {
+                    const CurveParams c = bp[star.band_offset + k];
+                    const longlong2* B = reinterpret_cast<const longlong2*>(S) + (int64_t)k * wl;
+                    const longlong2 p0 = B[bn], p1 = B[bn + bdur];
+                    const int64_t yi = p1.x - p0.x, wi = p1.y - p0.y;
+                    const double Y = ldexp((double)yi, c.y_exp);
+                    const double W = ldexp((double)wi, c.w_exp);
+                    const double Yo = ldexp((double)(c.y_tot - yi), c.y_exp);
+                    const double Wo = ldexp((double)(c.w_tot - wi), c.w_exp);
+                    double snr = 0.0, depth = 0.0;
+                    if (W >= c.ivar_floor && Wo >= c.ivar_floor) {
+                        const double yin = Y / W;
+                        const double yout = Yo / Wo;
+                        if (yout >= yin) {
+                            depth = yout - yin;
+                            snr = depth / sqrt(1.0 / W + 1.0 / Wo);
+                        }
+                    }
+                    s_snr[k] = snr;
+                    s_depth[k] = depth;
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double power = 0.0, depth = 0.0, dsnr = 0.0, dur = 0.0, t0 = 0.0, band = 0.0;
+                if (found) {
+                    int arg = 0;
+                    for (int k = 0; k < K; ++k) {
+                        power = power + s_snr[k] * s_snr[k];
+                        if (s_snr[k] > s_snr[arg]) arg = k;
+                    }
+                    depth = s_depth[arg];
+                    dsnr = sqrt(power);
+                    dur = (double)bdur * bd;
+                    t0 = fmod((double)bn * bd + 0.5 * dur, P) + star.t_min;
+                    band = (double)arg;
+                }
+                out[0 * n_out + col] = power;
+                out[1 * n_out + col] = depth;
+                out[2 * n_out + col] = dsnr;
+                out[3 * n_out + col] = dur;
+                out[4 * n_out + col] = t0;
+                out[5 * n_out + col] = band;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int T, int DMAX, int OBJ, bool GLOBAL>
+hipError_t launch_joint(dim3 grid, size_t lds, hipStream_t st, const double* tau,
+                        const int64_t* yq, const int64_t* wq, const CurveParams* bp,
+                        const cpb_star* stars, const cpb_segment* segs, const WorkItem* items,
+                        int64_t n_items, const double* periods, const int32_t* dur,
+                        int32_t os, int32_t wl, int64_t slice, int64_t* scratch, double* out,
+                        int64_t n_out) {
+    auto fn = bls_joint_kernel<T, DMAX, OBJ, GLOBAL>;
+    if (!GLOBAL && lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(fn, grid, dim3(T), GLOBAL ? 0 : lds, st, tau, yq, wq, bp, stars, segs,
+                       items, n_items, periods, dur, os, wl, slice, scratch, out, n_out);
+    return hipGetLastError();
+}
+
+template <int OBJ, bool GLOBAL>
+hipError_t dispatch_joint(int dmax, dim3 grid, size_t lds, hipStream_t st, const double* tau,
+                          const int64_t* yq, const int64_t* wq, const CurveParams* bp,
+                          const cpb_star* stars, const cpb_segment* segs,
+                          const WorkItem* items, int64_t n_items, const double* periods,
+                          const int32_t* dur, int32_t os, int32_t wl, int64_t slice,
+                          int64_t* scratch, double* out, int64_t n_out) {
+    if (dmax <= 8)
+        return launch_joint<256, 8, OBJ, GLOBAL>(grid, lds, st, tau, yq, wq, bp, stars, segs,
+                                                 items, n_items, periods, dur, os, wl, slice, scratch,
+                                                 out, n_out);
+    if (dmax <= 16)
+        return launch_joint<256, 16, OBJ, GLOBAL>(grid, lds, st, tau, yq, wq, bp, stars, segs,
+                                                  items, n_items, periods, dur, os, wl,
+                                                  slice, scratch, out, n_out);
+    return launch_joint<256, 32, OBJ, GLOBAL>(grid, lds, st, tau, yq, wq, bp, stars, segs,
+                                              items, n_items, periods, dur, os, wl, slice, scratch,
+                                              out, n_out);
+}
+
+}  // namespace
+
+extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double* tau,
+                                    const double* yw, const double* wv, int64_t total_points,
+                                    const cpb_band* bands, int32_t n_bands,
+                                    const cpb_star* stars, int32_t n_stars,
+                                    const double* periods, const int32_t* dur_bins,
+                                    const cpb_segment* segments, int32_t n_segments,
+                                    int32_t oversample, int32_t obj_flag, double* out,
+                                    int64_t n_out) {
+    if (!ctx) return CPB_E_INVALID;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (obj_flag != 0 && obj_flag != 1) return fail(ctx, CPB_E_INVALID, "obj_flag must be 0 or 1");
+    if (oversample < 1) return fail(ctx, CPB_E_INVALID, "oversample must be >= 1");
+    if (n_bands < 1 || n_stars < 1 || n_segments < 0 || total_points < 0 || n_out < 0)
+        return fail(ctx, CPB_E_INVALID, "negative or empty size");
+    if (!bands || !stars || (n_segments > 0 && !segments))
+        return fail(ctx, CPB_E_INVALID, "null descriptor array");
+    for (int32_t b = 0; b < n_bands; ++b) {
+        const cpb_band& bn = bands[b];
+        if (bn.point_offset < 0 || bn.n_points < 0 || bn.point_offset + bn.n_points > total_points)
+            return fail(ctx, CPB_E_INVALID, "band points out of range");
+        if (bn.n_points > (int64_t(1) << 30))
+            return fail(ctx, CPB_E_RESOURCE, "more than 2^30 points in one band");
+    }
+    int kmax = 1;
+    for (int32_t s = 0; s < n_stars; ++s) {
+        const cpb_star& sr = stars[s];
+        if (sr.n_bands < 1 || sr.band_offset < 0 || sr.band_offset + sr.n_bands > n_bands)
+            return fail(ctx, CPB_E_INVALID, "star bands out of range");
+        if (sr.n_bands > kJointMaxBands)
+            return fail(ctx, CPB_E_RESOURCE, "more than 64 bands in one star");
+        kmax = std::max(kmax, (int)sr.n_bands);
+    }
+    int dmax = 1, wmax = 2;
+    for (int32_t s = 0; s < n_segments; ++s) {
+        const cpb_segment& sg = segments[s];
+        if (sg.curve < 0 || sg.curve >= n_stars)
+            return fail(ctx, CPB_E_INVALID, "segment star index out of range");
+        if (sg.n_periods < 0 || sg.out_offset < 0 || sg.out_offset + sg.n_periods > n_out)
+            return fail(ctx, CPB_E_INVALID, "segment output range out of bounds");
+        if (sg.n_dur < 0 || sg.n_dur > kMaxDurations)
+            return fail(ctx, CPB_E_RESOURCE, "more than 32 distinct box widths in one segment");
+        if (sg.width < 2 || sg.width < oversample + 2)
+            return fail(ctx, CPB_E_INVALID, "segment width must be at least oversample + 2");
+        if (!(sg.bin_duration > 0.0))
+            return fail(ctx, CPB_E_INVALID, "bin_duration must be positive");
+        if (sg.mode == 0 && sg.n_periods > 0 && !periods)
+            return fail(ctx, CPB_E_INVALID, "explicit periods needed");
+        if (sg.mode < 0 || sg.mode > 2) return fail(ctx, CPB_E_INVALID, "unknown segment mode");
+        if (sg.mode != 0 && (sg.lin_num < 1 || sg.lin_k0 < 0 || sg.lin_k0 + sg.n_periods > sg.lin_num))
+            return fail(ctx, CPB_E_INVALID, "bad parametric segment");
+        if (sg.n_dur > 0 && sg.n_periods > 0) {
+            dmax = std::max(dmax, (int)sg.n_dur);
+            wmax = std::max(wmax, (int)sg.width);
+        }
+    }
+    if (wmax > (1 << 24)) return fail(ctx, CPB_E_RESOURCE, "more than 2^24 phase bins per period");
+
+    CPB_HIP(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure(ctx, ctx->curves, sizeof(CurveParams) * n_bands))) return rc;
+    if ((rc = ensure(ctx, ctx->yq, sizeof(int64_t) * std::max<int64_t>(1, total_points)))) return rc;
+    if ((rc = ensure(ctx, ctx->wq, sizeof(int64_t) * std::max<int64_t>(1, total_points)))) return rc;
+
+    // Work items: tiles of consecutive periods of one segment (a segment without box
+    // widths still gets items: its periods find no box and report zeros).
+    int64_t live = 0;
+    for (int32_t s = 0; s < n_segments; ++s) live += segments[s].n_periods;
+    const int64_t target_items = (int64_t)ctx->n_cu * 64;
+    int64_t tile = live / std::max<int64_t>(1, target_items);
+    tile = std::max<int64_t>(1, std::min<int64_t>(tile, 32));
+    ctx->h_items.clear();
+    for (int32_t s = 0; s < n_segments; ++s) {
+        for (int64_t b = 0; b < segments[s].n_periods; b += tile) {
+            WorkItem w;
+            w.seg = s;
+            w.begin = b;
+            w.count = (int32_t)std::min<int64_t>(tile, segments[s].n_periods - b);
+            ctx->h_items.push_back(w);
+        }
+    }
+    const int64_t n_items = (int64_t)ctx->h_items.size();
+
+    // Descriptor tables: [band curves | stars | segments | work items], 256-byte parts.
+    std::vector<cpb_curve> band_curves(n_bands);
+    for (int32_t b = 0; b < n_bands; ++b)
+        band_curves[b] = {bands[b].point_offset, bands[b].n_points, 0.0, bands[b].ivar_floor};
+    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t off_star = align(sizeof(cpb_curve) * n_bands);
+    const size_t off_seg = off_star + align(sizeof(cpb_star) * n_stars);
+    const size_t off_item = off_seg + align(sizeof(cpb_segment) * std::max(1, n_segments));
+    const size_t tbytes = off_item + align(sizeof(WorkItem) * std::max<int64_t>(1, n_items));
+    if ((rc = ensure(ctx, ctx->tables, tbytes))) return rc;
+    if (ctx->copy_pending) {
+        CPB_HIP(ctx, hipEventSynchronize(ctx->copied));
+        ctx->copy_pending = false;
+    }
+    if (ctx->h_tables_bytes < tbytes) {
+        if (ctx->h_tables) CPB_HIP(ctx, hipHostFree(ctx->h_tables));
+        ctx->h_tables = nullptr;
+        ctx->h_tables_bytes = 0;
+        CPB_HIP(ctx, hipHostMalloc(&ctx->h_tables, tbytes + tbytes / 4, hipHostMallocDefault));
+        ctx->h_tables_bytes = tbytes + tbytes / 4;
+    }
+    char* hb = static_cast<char*>(ctx->h_tables);
+    std::memcpy(hb, band_curves.data(), sizeof(cpb_curve) * n_bands);
+    std::memcpy(hb + off_star, stars, sizeof(cpb_star) * n_stars);
+    if (n_segments > 0) std::memcpy(hb + off_seg, segments, sizeof(cpb_segment) * n_segments);
+    if (n_items > 0) std::memcpy(hb + off_item, ctx->h_items.data(), sizeof(WorkItem) * n_items);
+    CPB_HIP(ctx, hipMemcpyAsync(ctx->tables.ptr, hb, tbytes, hipMemcpyHostToDevice, st));
+    if (!ctx->capturing) {
+        CPB_HIP(ctx, hipEventRecord(ctx->copied, st));
+        ctx->copy_pending = true;
+    }
+    char* db = static_cast<char*>(ctx->tables.ptr);
+    const auto* curve_in = reinterpret_cast<const cpb_curve*>(db);
+    const auto* sp = reinterpret_cast<const cpb_star*>(db + off_star);
+    const auto* gp = reinterpret_cast<const cpb_segment*>(db + off_seg);
+    const auto* ip = reinterpret_cast<const WorkItem*>(db + off_item);
+    auto* params = static_cast<CurveParams*>(ctx->curves.ptr);
+    auto* yq = static_cast<int64_t*>(ctx->yq.ptr);
+    auto* wq = static_cast<int64_t*>(ctx->wq.ptr);
+
+    hipError_t e = launch_prep_curves(st, yw, wv, curve_in, n_bands, obj_flag, params, yq, wq);
+    if (e != hipSuccess) return hip_fail(ctx, e, "prep_curves_kernel launch");
+    if (n_items == 0) return CPB_OK;
+
+    const int wl = (wmax + 63) / 64 * 64;
+    const size_t lds_need = (size_t)16 * kmax * wl;
+    const size_t static_lds = 8192;
+    const bool global = lds_need + static_lds > (size_t)ctx->lds_limit;
+    const int64_t slice = 2 * (int64_t)kmax * wl;  // int64 words of one workgroup's bins
+    int64_t* scratch = nullptr;
+    dim3 grid;
+    if (global) {
+        const int64_t g = std::min<int64_t>(n_items, (int64_t)ctx->n_cu * 4);
+        if ((rc = ensure(ctx, ctx->wide, (size_t)g * lds_need))) return rc;
+        scratch = static_cast<int64_t*>(ctx->wide.ptr);
+        grid = dim3((unsigned)g);
+    } else {
+        grid = dim3((unsigned)std::min<int64_t>(n_items, ((int64_t)1 << 31) / 256));
+    }
+    const bool timed = ctx->timing && !ctx->capturing;
+    if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
+    if (obj_flag == 0)
+        e = global ? dispatch_joint<0, true>(dmax, grid, lds_need, st, tau, yq, wq, params, sp, gp,
+                                             ip, n_items, periods, dur_bins, oversample, wl, slice,
+                                             scratch, out, n_out)
+                   : dispatch_joint<0, false>(dmax, grid, lds_need, st, tau, yq, wq, params, sp,
+                                              gp, ip, n_items, periods, dur_bins, oversample, wl, slice,
+                                              scratch, out, n_out);
+    else
+        e = global ? dispatch_joint<1, true>(dmax, grid, lds_need, st, tau, yq, wq, params, sp, gp,
+                                             ip, n_items, periods, dur_bins, oversample, wl, slice,
+                                             scratch, out, n_out)
+                   : dispatch_joint<1, false>(dmax, grid, lds_need, st, tau, yq, wq, params, sp,
+                                              gp, ip, n_items, periods, dur_bins, oversample, wl, slice,
+                                              scratch, out, n_out);
+    if (e != hipSuccess) return hip_fail(ctx, e, "bls_joint_kernel launch");
+    if (timed) {
+        CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
+        ctx->timed = true;
+    }
+    return CPB_OK;
+}

@@ -119,6 +119,18 @@ class PeriodogramMethod(ABC):
     ) -> Periodogram:
         """The periodogram of ``lc`` on ``grid``."""
 
+    def multiband_power(
+        self,
+        grid: GridSpec,
+        mblc: Any,
+        settings: Any,
+        backend: str,
+        engine: object | None = None,
+    ) -> Periodogram:
+        """The multi-band periodogram of ``mblc`` on ``grid`` (methods with
+        ``supports_multiband``; src/cuperiod/methods/base.py:344-357)."""
+        raise NotImplementedError(f"{self.name} does not support multi-band input")
+
     def make_engine(self, backend: str, settings: Any) -> object | None:
         return None
 

@@ -207,7 +207,7 @@ class BLSMethod(PeriodogramMethod):
 
     name: ClassVar[str] = "BLS"
     objective_sense: ClassVar[str] = "max"
-    supports_multiband: ClassVar[bool] = False
+    supports_multiband: ClassVar[bool] = True
     natural_domain: ClassVar[Domain] = Domain.FLUX
     settings_cls: ClassVar[type] = BLSSettings
     cpu_backend: ClassVar[str | None] = None
@@ -264,6 +264,20 @@ class BLSMethod(PeriodogramMethod):
             for name in _SEGMENT_FIELDS:
                 chunks[name].append(seg[name])
         return _assemble(chunks, n, finite.baseline, backend, finite.meta)
+
+    def multiband_power(
+        self,
+        grid: GridSpec,
+        mblc: Any,
+        settings: BLSSettings,
+        backend: str,
+        engine: object | None = None,
+    ) -> Periodogram:
+        """Multi-band BLS (src/cuperiod/methods/bls.py:397-411): the independent stack
+        or the joint-ephemeris search, as ``settings.mb_ephemeris`` says."""
+        from cuperiod_amd.methods.bls_mb import bls_multiband_power
+
+        return bls_multiband_power(grid, mblc, settings, backend, self)
 
     def estimate_device_bytes(self, n_points: int) -> int:
         return 128 * 1024**2 + n_points * 8 * 10

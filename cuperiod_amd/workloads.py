@@ -16,7 +16,16 @@ of the list below:
    6 period-scaled durations per log segment;
 4. ``proposer_batch``: like 3 with 16,384 curves (the learned proposer's workload);
 5. ``long_baseline_batch``: 256 curves, N = 100,000 over 10,000 d offset by 2.45e6 d, P = 0.7 d, depth 1e-3,
-   duration 0.05 d; 4,000,000 frequencies in [1e-4, 10] /d; 16 durations.
+   duration 0.05 d; 4,000,000 frequencies in [1e-4, 10] /d; 16 durations;
+6. ``multiband_survey``: stars of 6 bands x 150 points each over 1,095 d (random
+   epochs offset by 2.46e6 d), band noise 2-8 mmag-like (sigma 2e-3 .. 8e-3 in flux),
+   chromatic box transits (P log-uniform in [0.5, 20] d, depth 0.7-1.3 x a
+   log-uniform 2e-3 .. 1e-2, duration 0.02-0.05 of P) in every other star; searched
+   jointly over the default BLS grid of a 1,095 d baseline;
+7. ``tls_survey``: TESS-sector-like curves, 20,000 points over 27 d (offset 2.459e6 d),
+   sigma log-uniform in [2e-4, 2e-3], limb-darkened transits (u1 0.4, u2 0.3; P
+   log-uniform in [0.6, 12] d, depth log-uniform in [3e-4, 5e-3], duration 0.02-0.06 of
+   P) in every other curve; the default TLS grid of a 27 d baseline.
 
 A curve is ``(t, flux, dy)`` with flux in the flux domain (a transit is a dip) and a
 constant ``dy`` per curve (uniform weights).
@@ -126,3 +135,53 @@ def long_baseline_batch(n_curves: int = 256, n_freq: int = 4_000_000, n_points: 
 
 __all__ = ["SEED", "Workload", "gapped_curve", "long_baseline_batch", "proposer_batch",
            "small_curve", "survey_batch", "survey_curves"]
+
+
+def multiband_survey(n_stars: int = 64, n_bands: int = 6, n_per_band: int = 150,
+                     span: float = 1095.0):
+    """``(stars, truth)``: ``stars[i]`` is a list of ``(t, flux, dy)`` per band."""
+    rng = np.random.default_rng(SEED + 6)
+    stars, truth = [], []
+    for i in range(n_stars):
+        bands = []
+        period = float(np.exp(rng.uniform(np.log(0.5), np.log(20.0))))
+        depth = float(np.exp(rng.uniform(np.log(2e-3), np.log(1e-2))))
+        frac = float(rng.uniform(0.02, 0.05))
+        t0 = 2.46e6 + float(rng.uniform(0.0, period))
+        for b in range(n_bands):
+            t = 2.46e6 + np.sort(rng.uniform(0.0, span, n_per_band))
+            sigma = float(rng.uniform(2e-3, 8e-3))
+            y = 1.0 + rng.normal(0.0, sigma, n_per_band)
+            if i % 2 == 0:
+                scale = 0.7 + 0.6 * b / max(1, n_bands - 1)
+                y[_box_dip(t, period, t0, frac * period)] -= depth * scale
+            bands.append((t, y, np.full(n_per_band, sigma)))
+        stars.append(bands)
+        truth.append(period if i % 2 == 0 else None)
+    return stars, truth
+
+
+def tls_survey(n_curves: int = 256, n_points: int = 20_000, span: float = 27.0):
+    """``(curves, truth)``: ``curves[i] = (t, flux, dy)``."""
+    rng = np.random.default_rng(SEED + 7)
+    curves, truth = [], []
+    for i in range(n_curves):
+        t = 2.459e6 + np.sort(rng.uniform(0.0, span, n_points))
+        sigma = float(np.exp(rng.uniform(np.log(2e-4), np.log(2e-3))))
+        y = 1.0 + rng.normal(0.0, sigma, n_points)
+        period = None
+        if i % 2 == 0:
+            period = float(np.exp(rng.uniform(np.log(0.6), np.log(12.0))))
+            depth = float(np.exp(rng.uniform(np.log(3e-4), np.log(5e-3))))
+            q = float(rng.uniform(0.02, 0.06))
+            t0 = 2.459e6 + float(rng.uniform(0.0, period))
+            ph = ((t - t0) / period) % 1.0
+            x = np.where(ph > 0.5, ph - 1.0, ph)
+            z = np.clip(2.0 * np.abs(x) / q, 0.0, 1.0)
+            mu = np.sqrt(1.0 - z * z)
+            prof = 1.0 - 0.4 * (1.0 - mu) - 0.3 * (1.0 - mu) ** 2
+            inside = np.abs(x) < q / 2.0
+            y[inside] -= depth * prof[inside] / prof.max()
+        curves.append((t, y, np.full(n_points, sigma)))
+        truth.append(period)
+    return curves, truth

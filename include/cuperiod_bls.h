@@ -150,6 +150,84 @@ int cpb_peak_candidates(cpb_context* ctx, void* stream, const double* out, int64
                         int32_t n_curves, int32_t want, int32_t cap, int32_t* cand,
                         int64_t* counts, double* values);
 
+/* One band of a multi-band star (its points in the concatenated tau/yw/wv arrays). */
+typedef struct cpb_band {
+    int64_t point_offset;   /* first point of the band */
+    int64_t n_points;
+    double ivar_floor;      /* `_ivar_floor(n_points, sum(ivar))` of the band alone */
+} cpb_band;
+
+/* One star: bands[band_offset .. band_offset + n_bands), their times measured from the
+ * star's common origin t_min (`joint_bands`, src/cuperiod/multiband/_bls_joint.py:94-128). */
+typedef struct cpb_star {
+    int32_t band_offset;
+    int32_t n_bands;        /* 1 .. 64 */
+    double t_min;
+} cpb_star;
+
+#define CPB_N_JOINT_OUTPUTS 6  /* power, depth, depth_snr, duration, transit_time, band_index */
+
+/*
+ * Joint-ephemeris multi-band box search of one or more stars: per trial period, one box
+ * (one duration, one start on the phase grid every band of the star shares), each band
+ * fitting its own depth inside it; the box maximises the sum over the bands that dip in
+ * it of depth^2/(1/W_in + 1/W_out) (obj_flag 0) or W_in*depth^2/2 (obj_flag 1). Outputs
+ * per period: the quadrature power sum_k max(SNR_k, 0)^2 at that box, the depth of the
+ * band with the largest SNR, sqrt(power), the box duration and mid-transit time, and
+ * that band's index within the star (all 0 where no box is admissible).
+ *
+ * Replaces `_joint_cuda` (src/cuperiod/multiband/_bls_joint.py:703-796), its
+ * `bls_joint_block` kernel (:531-685) and the host `_finish` (:359-406).
+ *
+ * tau, yw, wv  device float64[total_points]: times minus the star's t_min, median-centred
+ *              y*ivar of each band, ivar
+ * bands, stars, segments  HOST arrays; segment.curve indexes `stars`
+ * periods, dur_bins       as for cpb_bls_search_batch
+ * out          device float64[CPB_N_JOINT_OUTPUTS * n_out], row k at out + k*n_out
+ */
+int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double* tau, const double* yw,
+                         const double* wv, int64_t total_points, const cpb_band* bands,
+                         int32_t n_bands, const cpb_star* stars, int32_t n_stars,
+                         const double* periods, const int32_t* dur_bins,
+                         const cpb_segment* segments, int32_t n_segments, int32_t oversample,
+                         int32_t obj_flag, double* out, int64_t n_out);
+
+/* One light curve of a TLS search. */
+typedef struct cpb_tls_curve {
+    int64_t point_offset;   /* first point in the concatenated tau/yw/wv arrays */
+    int64_t n_points;
+    int64_t period_offset;  /* first trial period of the curve in `periods` */
+    int64_t n_periods;
+    int64_t out_offset;     /* output column of its first period */
+    double w_total;         /* sum of the curve's weights (the W of the level fit) */
+} cpb_tls_curve;
+
+#define CPB_N_TLS_OUTPUTS 4  /* sr, depth, duration, t0 (phase time from tau = 0) */
+
+/*
+ * TLS matched-filter search of one or more light curves: per trial period the points
+ * are folded into `n_bins` phase bins (bin = int(frac(tau/P)*n_bins)) and every
+ * limb-darkened template (widths[i] bins, samples templates[sum(widths[:i]) ..]) is slid
+ * over every start phase (circularly), fitting level and depth jointly; outputs per
+ * period: the best signal residue SR = num^2/D with D = den - cwg^2/W, its depth -num/D,
+ * duration width/n_bins*P and the phase time of its centre, frac((start +
+ * width/2)/n_bins)*P (0 and 0 where no dip qualifies).
+ *
+ * Replaces the `tls_block` kernel and `_tls_cuda` (src/cuperiod/methods/tls.py:270-479).
+ *
+ * tau, yw, wv  device float64[total_points]: times minus the curve's first time,
+ *              w*(y - weighted mean), weights w (any common unit)
+ * curves       HOST array; periods: device float64[...]
+ * widths, templates  HOST arrays (1..32 widths, each in [1, n_bins])
+ * n_bins       2 .. 4096; den_rel: the relative floor on D (1e-9 in FP64)
+ * out          device float64[CPB_N_TLS_OUTPUTS * n_out], row k at out + k*n_out
+ */
+int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau, const double* yw,
+                   const double* wv, int64_t total_points, const cpb_tls_curve* curves,
+                   int32_t n_curves, const double* periods, int32_t n_bins,
+                   const int32_t* widths, int32_t n_widths, const double* templates,
+                   double den_rel, double* out, int64_t n_out);
+
 /* Frequencies of np.linspace(start, stop, num) by numpy's recipe (value_i = i*step +
  * start, last = stop), computed on the HOST: the values mode-1 segments use. */
 int cpb_linspace_grid(double start, double stop, int64_t num, double* out);

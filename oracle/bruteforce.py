@@ -123,3 +123,75 @@ def box_table(t, y, ivar, period, *, bin_duration, dur_bins, oversample, width):
                 (di, n, math.fsum(yw[j] for j in idx), math.fsum(wv[j] for j in idx))
             )
     return rows, n_bins
+
+
+def joint_bruteforce(bands, periods, *, bin_duration, dur_bins, oversample, width,
+                     obj_flag):
+    """The joint-ephemeris sweep with exactly summed box statistics.
+
+    ``bands`` is an :class:`oracle.multiband.JointBands`. Every band is folded against
+    the common origin; box ``(n, d)`` holds bins ``n+1 .. n+d`` of every band; the box
+    value is the sum over the admissible, dipping bands of ``depth^2 / (1/W_in +
+    1/W_out)`` (snr) or ``W_in depth^2 / 2`` (likelihood); a box counts when one band
+    dips; first strict maximum in (duration index, start) order. Returns
+    ``(best_n, best_d, found, y_in, ivar_in)`` like :func:`oracle.multiband.joint_kernel`.
+    """
+    periods = np.asarray(periods, dtype=np.float64)
+    k_all = bands.n_bands
+    n_p = int(periods.size)
+    o_n = np.zeros(n_p, dtype=np.int64)
+    o_d = np.zeros(n_p, dtype=np.int64)
+    found = np.zeros(n_p, dtype=bool)
+    o_y = np.zeros((k_all, n_p))
+    o_i = np.zeros((k_all, n_p))
+    per_band = []
+    for k in range(k_all):
+        sl = slice(int(bands.starts[k]), int(bands.starts[k + 1]))
+        per_band.append((bands.tau[sl], bands.yw[sl].tolist(), bands.ivar[sl].tolist()))
+    sums_y = [math.fsum(yw) for _, yw, _ in per_band]
+    sums_i = [math.fsum(iv) for _, _, iv in per_band]
+    for p, period in enumerate(periods.tolist()):
+        folded = [_members(tau, period, bin_duration, oversample, width)
+                  for tau, _, _ in per_band]
+        n_bins = folded[0][1]
+        best = -math.inf
+        best_box = None
+        for d in dur_bins:
+            if d < 1 or d >= n_bins or d >= width:
+                continue
+            for n in range(0, n_bins - d + 1):
+                total = 0.0
+                hit = False
+                sums = []
+                for k in range(k_all):
+                    bins = folded[k][0]
+                    idx = [j for b in range(n + 1, n + d + 1) for j in bins[b]]
+                    y_in = math.fsum(per_band[k][1][j] for j in idx)
+                    iv_in = math.fsum(per_band[k][2][j] for j in idx)
+                    sums.append((y_in, iv_in))
+                    iv_out = sums_i[k] - iv_in
+                    if iv_in < bands.floor[k] or iv_out < bands.floor[k]:
+                        continue
+                    yin = y_in / iv_in
+                    yout = (sums_y[k] - y_in) / iv_out
+                    if not yout >= yin:
+                        continue
+                    depth = yout - yin
+                    hit = True
+                    if obj_flag == 0:
+                        total += depth * depth / (1.0 / iv_in + 1.0 / iv_out)
+                    else:
+                        total += 0.5 * iv_in * depth * depth
+                if hit and total > best:
+                    best = total
+                    best_box = (n, d, sums)
+        if best_box is None:
+            continue
+        n, d, sums = best_box
+        found[p] = True
+        o_n[p] = n
+        o_d[p] = d
+        for k, (y_in, iv_in) in enumerate(sums):
+            o_y[k, p] = y_in
+            o_i[k, p] = iv_in
+    return o_n, o_d, found, o_y, o_i

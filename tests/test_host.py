@@ -74,6 +74,34 @@ def test_segment_struct_layout_matches_header():
     assert ctypes.sizeof(_lib.Curve) == 32
 
 
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    # sizes and offsets as a C compiler lays out the header's structs
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stddef.h>\n#include <stdio.h>\n#include "cuperiod_bls.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu %zu %zu %zu %zu\\n", sizeof(cpb_segment), sizeof(cpb_curve),\n'
+        "         sizeof(cpb_band), sizeof(cpb_star), sizeof(cpb_tls_curve));\n"
+        '  printf("%zu %zu %zu\\n", offsetof(cpb_segment, lin_start),\n'
+        "         offsetof(cpb_star, t_min), offsetof(cpb_tls_curve, w_total));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    sizes, offsets = subprocess.run([str(exe)], check=True, capture_output=True,
+                                    text=True).stdout.splitlines()
+    assert [int(v) for v in sizes.split()] == [
+        ctypes.sizeof(_lib.Segment), ctypes.sizeof(_lib.Curve), ctypes.sizeof(_lib.Band),
+        ctypes.sizeof(_lib.Star), ctypes.sizeof(_lib.TlsCurve)]
+    assert [int(v) for v in offsets.split()] == [
+        _lib.Segment.lin_start.offset, _lib.Star.t_min.offset, _lib.TlsCurve.w_total.offset]
+
+
 @pytest.mark.parametrize("start,stop,num", [(1 / 50, 2.0, 200000), (0.001, 5.0, 1000000),
                                              (1 / 1337.5, 2.0, 777), (0.3, 0.3000001, 5)])
 def test_linspace_recipe_is_numpys(start, stop, num):
