@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "cpb_bls_summarize",
     "cpb_peak_candidates",
     "cpb_bls_joint_search",
+    "cpb_set_joint_kernel",
     "cpb_tls_search",
     "cpb_linspace_grid",
     "cpb_set_timing",
@@ -176,6 +177,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         vp, vp, vp, vp, vp, i64, ctypes.POINTER(Band), i32, ctypes.POINTER(Star), i32, vp,
         vp, ctypes.POINTER(Segment), i32, i32, i32, vp, i64,
     ]
+    lib.cpb_set_joint_kernel.restype = ctypes.c_int
+    lib.cpb_set_joint_kernel.argtypes = [vp, ctypes.c_int]
     lib.cpb_tls_search.restype = ctypes.c_int
     lib.cpb_tls_search.argtypes = [
         vp, vp, vp, vp, vp, i64, ctypes.POINTER(TlsCurve), i32, vp, i32,

@@ -384,6 +384,7 @@ class _Prepared:
     key: str
     label: str
     finite: LightCurve
+    periods: np.ndarray | None = None
 
 
 def _prepare(position: int, key: str, lc: LightCurve, settings: Any,
@@ -411,6 +412,43 @@ def _prepare(position: int, key: str, lc: LightCurve, settings: Any,
     elif not _grid_segments(grid, settings):
         raise InsufficientDataError("BLS: empty trial grid")
     return _Prepared(position, key, source_label(lc), finite)
+
+
+def _prepare_tls(position: int, key: str, lc: LightCurve, settings: Any,
+                 grid: GridSpec | None) -> _Prepared:
+    """The checks and the grid of ``TLSMethod.power`` for one light curve
+    (src/cuperiod/methods/tls.py:767-797), raising the same errors."""
+    from cuperiod_amd.methods.tls import TLSMethod
+
+    raise_if_unusable(lc, "TLS")
+    finite = lc.in_domain(Domain.FLUX).finite()
+    if finite.n < settings.min_detections:
+        raise InsufficientDataError(
+            f"TLS: {finite.n} finite points < min_detections {settings.min_detections}"
+        )
+    if finite.baseline <= 0.0:
+        raise InsufficientDataError("TLS: no usable time baseline")
+    g = grid if grid is not None else TLSMethod().default_grid(finite, settings)
+    return _Prepared(position, key, source_label(lc), finite,
+                     np.asarray(g.period, dtype=np.float64))
+
+
+def _tls_rows(prepared: Sequence[_Prepared], settings: Any, cfg: "_ChunkConfig"):
+    """One TLS launch for the chunk and its rows (the periodograms of
+    ``TLSMethod.power``)."""
+    from cuperiod_amd.methods.tls import tls_periodogram, tls_power_batch
+
+    found = tls_power_batch(
+        [(p.finite.time, p.finite.value, p.finite.error) for p in prepared],
+        [p.periods for p in prepared], settings=settings)
+    rows = []
+    for prep, res in zip(prepared, found):
+        pg = tls_periodogram(prep.finite, prep.periods, res, "hip")
+        pg = pg.with_peak_defaults(peak_defaults_from_settings(settings))
+        rows.append(periodogram_to_row(prep.key, pg, n_best=cfg.n_best,
+                                       store_raw=cfg.store_raw, extra_keys=cfg.extra_keys,
+                                       source=prep.label))
+    return rows
 
 
 def _curve_arrays(lc: LightCurve) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -580,15 +618,19 @@ class _ChunkRunner:
         errors: dict[int, tuple[str, str]] = {}
         for name in cfg.methods:
             settings = cfg.settings_map[name]
+            prepare = _prepare_tls if name == "TLS" else _prepare
             prepared: list[_Prepared] = []
             for pos, (key, lc) in enumerate(items):
                 if pos in errors:
                     continue
                 try:
-                    prepared.append(_prepare(pos, key, lc, settings, cfg.grid))
+                    prepared.append(prepare(pos, key, lc, settings, cfg.grid))
                 except Exception as exc:  # noqa: BLE001 - one bad light curve
                     errors[pos] = (key, f"{type(exc).__name__}: {exc}")
-            handle = _launch(prepared, settings, cfg.grid) if prepared else None
+            if name == "TLS":
+                handle = ("tls", _tls_rows(prepared, settings, cfg)) if prepared else None
+            else:
+                handle = _launch(prepared, settings, cfg.grid) if prepared else None
             launches.append((settings, prepared, handle))
         self._complete()
         self.pending = (idx, cfg, launches, errors)
@@ -601,6 +643,11 @@ class _ChunkRunner:
         by_pos: dict[int, list[dict[str, Any]]] = {}
         for settings, prepared, handle in launches:
             if handle is None:
+                continue
+            if isinstance(handle[0], str):
+                for prep, row in zip(prepared, handle[1]):
+                    if prep.position not in errors:
+                        by_pos.setdefault(prep.position, []).append(row)
                 continue
             for prep, res in zip(prepared, _collect(handle, candidates=not cfg.store_raw)):
                 if prep.position in errors:

@@ -334,7 +334,356 @@ hipError_t dispatch_joint(int dmax, dim3 grid, size_t lds, hipStream_t st, const
                                               out, n_out);
 }
 
+// ---------------------------------------------------------------------------------
+// Band-group kernel (1,024 threads): the bands of a star pass through LDS a group at a
+// time (as many as fit), while every thread keeps the running box totals of its starts
+// (tid + r*1024, r < RMAX) and durations in registers; bands are still added in band
+// order. After each group's exact int64 prefix, the sums are rounded once to FP64 in
+// place, so a band's box sums are FP64 differences of prefix sums -- the reference's own
+// arithmetic (`_bls_joint.py:474-476`) -- and a term costs one 16-byte LDS read, a few
+// FP64 operations and a Newton-refined reciprocal. The winning box's per-band sums are
+// read from the resident group and from re-folds of the others.
+
+constexpr int kGroupThreads = 1024;
+
+struct BandTerm {
+    double y_tot;    // sum of fixed-point y*ivar
+    double w_tot;    // sum of fixed-point ivar
+    double floor_q;  // ivar floor in fixed-point units
+    double c;        // 2^(2 e_y - e_w) / w_tot (snr) or 2^(2 e_y - e_w) / 2 (likelihood)
+};
+
+// n / d for finite d > 0: hardware reciprocal, two Newton steps and a residual
+// correction (within an ulp or two of the correctly rounded quotient).
+__device__ __forceinline__ double div_newton(double n, double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    const double q = n * r;
+    return fma(fma(-d, q, n), r, q);
+}
+
+// Zero, fold, wrap-copy and prefix the bins of bands g0 .. g0+gk-1 into LDS slots
+// 0 .. gk-1 (FP64 prefix sums of the exact integer bins). Ends with a barrier.
+__device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, int nb,
+                                           int width, int oversample, double P, double invP,
+                                           double bd, double inv_bd,
+                                           const int64_t* __restrict__ s_cum,
+                                           const int64_t* __restrict__ s_off,
+                                           const double* __restrict__ tau,
+                                           const int64_t* __restrict__ yq,
+                                           const int64_t* __restrict__ wq) {
+    constexpr int T = kGroupThreads;
+    constexpr int NW = T / kWave;
+    const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+    longlong2* S2 = reinterpret_cast<longlong2*>(S);
+    const int span = nb + 1;
+    for (int i = tid; i < gk * span; i += T) {
+        const int kk = i / span;
+        S2[kk * wl + (i - kk * span)] = make_longlong2(0, 0);
+    }
+    __syncthreads();
+    {
+        int k = g0;
+        for (int64_t j = s_cum[g0] + tid; j < s_cum[g0 + gk]; j += T) {
+            while (j >= s_cum[k + 1]) ++k;
+            const int64_t p = s_off[k] + (j - s_cum[k]);
+            const int b = fold_bin(tau[p], P, invP, bd, inv_bd, width);
+            int64_t* slot = S + 2 * ((int64_t)(k - g0) * wl + b);
+            atomicAdd(reinterpret_cast<unsigned long long*>(slot), (unsigned long long)yq[p]);
+            atomicAdd(reinterpret_cast<unsigned long long*>(slot + 1), (unsigned long long)wq[p]);
+        }
+    }
+    __syncthreads();
+    for (int kk = tid; kk < gk; kk += T) {
+        longlong2* B = S2 + (int64_t)kk * wl;
+        for (int o = 0; o < oversample; ++o) {
+            int dst = nb - oversample + o;
+            dst = dst < 0 ? 0 : (dst > width - 1 ? width - 1 : dst);
+            B[dst] = B[1 + o];
+        }
+    }
+    __syncthreads();
+    for (int kk = wave; kk < gk; kk += NW) {
+        longlong2* B = S2 + (int64_t)kk * wl;
+        double2* D = reinterpret_cast<double2*>(B);
+        int64_t cy = 0, cw = 0;
+        for (int base = 0; base <= nb; base += kWave) {
+            const int i = base + lane;
+            longlong2 v = make_longlong2(0, 0);
+            if (i <= nb) v = B[i];
+            const int64_t y = wave_incl_scan_dpp(v.x) + cy;
+            const int64_t w = wave_incl_scan_dpp(v.y) + cw;
+            if (i <= nb) D[i] = make_double2((double)y, (double)w);
+            cy = readlane_i64(y, kWave - 1);
+            cw = readlane_i64(w, kWave - 1);
+        }
+    }
+    __syncthreads();
+}
+
+template <int DMAX, int RMAX, int OBJ>
+__global__ __launch_bounds__(kGroupThreads) void bls_joint_group_kernel(
+    const double* __restrict__ tau, const int64_t* __restrict__ yq,
+    const int64_t* __restrict__ wq, const CurveParams* __restrict__ bp,
+    const cpb_star* __restrict__ stars, const cpb_segment* __restrict__ segs,
+    const WorkItem* __restrict__ items, int64_t n_items, const double* __restrict__ periods,
+    const int32_t* __restrict__ dur_bins, int32_t oversample, int32_t wl, int32_t group,
+    double* __restrict__ out, int64_t n_out) {
+    extern __shared__ int64_t dyn_lds[];
+    constexpr int T = kGroupThreads;
+    constexpr int NW = T / kWave;
+    __shared__ BandTerm s_term[kJointMaxBands];
+    __shared__ int64_t s_cum[kJointMaxBands + 1];
+    __shared__ int64_t s_off[kJointMaxBands];
+    __shared__ int32_t s_dur[kMaxDurations];
+    __shared__ double s_red_v[NW];
+    __shared__ uint32_t s_red_k[NW];
+    __shared__ double s_snr[kJointMaxBands];
+    __shared__ double s_depth[kJointMaxBands];
+
+    const int tid = threadIdx.x;
+    const int lane = tid % kWave;
+    const int wave = tid / kWave;
+    int64_t* S = dyn_lds;
+    const double2* D2 = reinterpret_cast<const double2*>(dyn_lds);
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const WorkItem it = items[item];
+        const cpb_segment sg = segs[it.seg];
+        const cpb_star star = stars[sg.curve];
+        const int K = star.n_bands;
+        const int n_dur = sg.n_dur;
+        const double bd = sg.bin_duration;
+        const double inv_bd = 1.0 / bd;
+        const int width = sg.width;
+
+        __syncthreads();
+        if (tid < n_dur) s_dur[tid] = dur_bins[sg.dur_offset + tid];
+        if (tid < K) {
+            const CurveParams c = bp[star.band_offset + tid];
+            BandTerm b;
+            b.y_tot = (double)c.y_tot;
+            b.w_tot = (double)c.w_tot;
+            b.floor_q = ldexp(c.ivar_floor, -c.w_exp);
+            const int sc = 2 * c.y_exp - c.w_exp;
+            b.c = OBJ == 0 ? ldexp(1.0 / b.w_tot, sc) : ldexp(0.5, sc);
+            s_term[tid] = b;
+            s_off[tid] = c.point_offset;
+        }
+        if (tid == 0) {
+            int64_t acc = 0;
+            for (int k = 0; k < K; ++k) {
+                s_cum[k] = acc;
+                acc += bp[star.band_offset + k].n_points;
+            }
+            s_cum[K] = acc;
+        }
+        __syncthreads();
+
+        for (int q = 0; q < it.count; ++q) {
+            const int64_t col = sg.out_offset + it.begin + q;
+            const double P = segment_period(sg, periods, it.begin + q);
+            const double invP = 1.0 / P;
+            int64_t nb64 = (int64_t)ceil(P / bd) + oversample;
+            if (nb64 > width - 1) nb64 = width - 1;
+            const int nb = (int)nb64;
+
+            double tot[RMAX][DMAX];
+            uint32_t hit[RMAX];
+#pragma unroll
+            for (int r = 0; r < RMAX; ++r) {
+                hit[r] = 0;
+#pragma unroll
+                for (int di = 0; di < DMAX; ++di) tot[r][di] = 0.0;
+            }
+            int last_g0 = 0;
+            for (int g0 = 0; g0 < K; g0 += group) {
+                const int gk = min(group, K - g0);
+                fold_group(S, wl, g0, gk, nb, width, oversample, P, invP, bd, inv_bd, s_cum,
+                           s_off, tau, yq, wq);
+                for (int kk = 0; kk < gk; ++kk) {
+                    const BandTerm bt = s_term[g0 + kk];
+                    const double2* B = D2 + (int64_t)kk * wl;
+#pragma unroll
+                    for (int r = 0; r < RMAX; ++r) {
+                        const int n = tid + r * T;
+                        if (n < nb) {
+                            const double2 p0 = B[n];
+#pragma unroll
+                            for (int di = 0; di < DMAX; ++di) {
+                                if (di < n_dur) {
+                                    const int d = s_dur[di];
+                                    if (d < nb && n + d <= nb) {
+                                        const double2 p1 = B[n + d];
+                                        const double wi = p1.y - p0.y;
+                                        const double wo = bt.w_tot - wi;
+                                        if (wi >= bt.floor_q && wo >= bt.floor_q) {
+                                            const double yi = p1.x - p0.x;
+                                            const double t = fma(bt.y_tot, wi, -(yi * bt.w_tot));
+                                            if (t >= 0.0) {
+                                                hit[r] |= 1u << di;
+                                                const double den = OBJ == 0 ? wi * wo : wi * wo * wo;
+                                                tot[r][di] += div_newton(t * t, den) * bt.c;
+                                            }
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                last_g0 = g0;
+                __syncthreads();
+            }
+
+            // first-wins maximum in (duration index, start) order
+            double best = -INFINITY;
+            uint32_t bkey = 0xFFFFFFFFu;
+#pragma unroll
+            for (int r = 0; r < RMAX; ++r) {
+                const int n = tid + r * T;
+#pragma unroll
+                for (int di = 0; di < DMAX; ++di) {
+                    if ((hit[r] >> di) & 1u) {
+                        const uint32_t key = ((uint32_t)di << 24) | (uint32_t)n;
+                        if (first_wins(tot[r][di], key, best, bkey)) {
+                            best = tot[r][di];
+                            bkey = key;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(best, off, kWave);
+                const uint32_t ok = (uint32_t)__shfl_xor((int)bkey, off, kWave);
+                if (first_wins(ov, ok, best, bkey)) {
+                    best = ov;
+                    bkey = ok;
+                }
+            }
+            if (lane == 0) {
+                s_red_v[wave] = best;
+                s_red_k[wave] = bkey;
+            }
+            __syncthreads();
+            best = s_red_v[0];
+            bkey = s_red_k[0];
+            for (int w = 1; w < NW; ++w)
+                if (first_wins(s_red_v[w], s_red_k[w], best, bkey)) {
+                    best = s_red_v[w];
+                    bkey = s_red_k[w];
+                }
+            const bool found = best > -INFINITY;
+            const int bn = found ? (int)(bkey & 0xFFFFFFu) : 0;
+            const int bdur = found ? s_dur[bkey >> 24] : 0;
+
+            if (found) {
+                // the resident group first, then re-folds of the others
+                for (int pass = 0; pass * group < K; ++pass) {
+                    const int g0 = pass == 0 ? last_g0 : (pass - 1) * group;
+                    if (pass > 0 && g0 == last_g0) continue;
+                    const int gk = min(group, K - g0);
+                    if (pass > 0)
+                        fold_group(S, wl, g0, gk, nb, width, oversample, P, invP, bd, inv_bd,
+                                   s_cum, s_off, tau, yq, wq);
+                    for (int kk = tid; kk < gk; kk += T) {
+                        const CurveParams c = bp[star.band_offset + g0 + kk];
+                        const double2* B = D2 + (int64_t)kk * wl;
+                        const double2 p0 = B[bn], p1 = B[bn + bdur];
+                        const double yi = p1.x - p0.x, wi = p1.y - p0.y;
+                        const double Y = ldexp(yi, c.y_exp);
+                        const double W = ldexp(wi, c.w_exp);
+                        const double Yo = ldexp((double)c.y_tot - yi, c.y_exp);
+                        const double Wo = ldexp((double)c.w_tot - wi, c.w_exp);
+                        double snr = 0.0, depth = 0.0;
+                        if (W >= c.ivar_floor && Wo >= c.ivar_floor) {
+                            const double yin = Y / W;
+                            const double yout = Yo / Wo;
+                            if (yout >= yin) {
+                                depth = yout - yin;
+                                snr = depth / sqrt(1.0 / W + 1.0 / Wo);
+                            }
+                        }
+                        s_snr[g0 + kk] = snr;
+                        s_depth[g0 + kk] = depth;
+                    }
+                    __syncthreads();
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double power = 0.0, depth = 0.0, dsnr = 0.0, dur = 0.0, t0 = 0.0, band = 0.0;
+                if (found) {
+                    int arg = 0;
+                    for (int k = 0; k < K; ++k) {
+                        power = power + s_snr[k] * s_snr[k];
+                        if (s_snr[k] > s_snr[arg]) arg = k;
+                    }
+                    depth = s_depth[arg];
+                    dsnr = sqrt(power);
+                    dur = (double)bdur * bd;
+                    t0 = fmod((double)bn * bd + 0.5 * dur, P) + star.t_min;
+                    band = (double)arg;
+                }
+                out[0 * n_out + col] = power;
+                out[1 * n_out + col] = depth;
+                out[2 * n_out + col] = dsnr;
+                out[3 * n_out + col] = dur;
+                out[4 * n_out + col] = t0;
+                out[5 * n_out + col] = band;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int DMAX, int RMAX, int OBJ>
+hipError_t launch_group(dim3 grid, size_t lds, hipStream_t st, const double* tau,
+                        const int64_t* yq, const int64_t* wq, const CurveParams* bp,
+                        const cpb_star* stars, const cpb_segment* segs, const WorkItem* items,
+                        int64_t n_items, const double* periods, const int32_t* dur, int32_t os,
+                        int32_t wl, int32_t group, double* out, int64_t n_out) {
+    auto fn = bls_joint_group_kernel<DMAX, RMAX, OBJ>;
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(fn, grid, dim3(kGroupThreads), lds, st, tau, yq, wq, bp, stars, segs,
+                       items, n_items, periods, dur, os, wl, group, out, n_out);
+    return hipGetLastError();
+}
+
+// Largest start count a group-kernel variant covers: RMAX * 1024 with RMAX * DMAX <= 32.
+__host__ __forceinline__ int group_rmax(int dmax) { return dmax <= 8 ? 4 : (dmax <= 16 ? 2 : 1); }
+
+template <int OBJ>
+hipError_t dispatch_group(int dmax, dim3 grid, size_t lds, hipStream_t st, const double* tau,
+                          const int64_t* yq, const int64_t* wq, const CurveParams* bp,
+                          const cpb_star* stars, const cpb_segment* segs,
+                          const WorkItem* items, int64_t n_items, const double* periods,
+                          const int32_t* dur, int32_t os, int32_t wl, int32_t group,
+                          double* out, int64_t n_out) {
+    if (dmax <= 8)
+        return launch_group<8, 4, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
+                                       n_items, periods, dur, os, wl, group, out, n_out);
+    if (dmax <= 16)
+        return launch_group<16, 2, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
+                                        n_items, periods, dur, os, wl, group, out, n_out);
+    return launch_group<32, 1, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
+                                    n_items, periods, dur, os, wl, group, out, n_out);
+}
+
 }  // namespace
+
+extern "C" int cpb_set_joint_kernel(cpb_context* ctx, int mode) {
+    if (!ctx) return CPB_E_INVALID;
+    if (mode < 0 || mode > 2) return fail(ctx, CPB_E_INVALID, "joint kernel mode must be 0, 1 or 2");
+    ctx->joint_kernel = mode;
+    return CPB_OK;
+}
 
 extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double* tau,
                                     const double* yw, const double* wv, int64_t total_points,
@@ -463,8 +812,36 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
     if (n_items == 0) return CPB_OK;
 
     const int wl = (wmax + 63) / 64 * 64;
-    const size_t lds_need = (size_t)16 * kmax * wl;
     const size_t static_lds = 8192;
+    const bool timed = ctx->timing && !ctx->capturing;
+    // Band-group kernel when its register tiles cover every start and one band fits in
+    // LDS; otherwise every band of a period at once, in LDS or global scratch.
+    const size_t band_bytes = (size_t)16 * wl;
+    const bool group_fits = wmax - 1 <= group_rmax(dmax) * kGroupThreads &&
+                            band_bytes + static_lds <= (size_t)ctx->lds_limit;
+    const int mode = ctx->joint_kernel;
+    if (mode == 2 && !group_fits)
+        return fail(ctx, CPB_E_RESOURCE, "the band-group joint kernel does not cover this span");
+    if (mode != 1 && group_fits) {
+        const int gmax = std::max<int>(1, (int)((ctx->lds_limit - static_lds) / band_bytes));
+        const int n_groups = (kmax + gmax - 1) / gmax;
+        const int group = (kmax + n_groups - 1) / n_groups;
+        const size_t lds = band_bytes * group;
+        const dim3 grid((unsigned)std::min<int64_t>(n_items, ((int64_t)1 << 31) / kGroupThreads));
+        if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
+        e = obj_flag == 0
+                ? dispatch_group<0>(dmax, grid, lds, st, tau, yq, wq, params, sp, gp, ip, n_items,
+                                    periods, dur_bins, oversample, wl, group, out, n_out)
+                : dispatch_group<1>(dmax, grid, lds, st, tau, yq, wq, params, sp, gp, ip, n_items,
+                                    periods, dur_bins, oversample, wl, group, out, n_out);
+        if (e != hipSuccess) return hip_fail(ctx, e, "bls_joint_group_kernel launch");
+        if (timed) {
+            CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
+            ctx->timed = true;
+        }
+        return CPB_OK;
+    }
+    const size_t lds_need = (size_t)16 * kmax * wl;
     const bool global = lds_need + static_lds > (size_t)ctx->lds_limit;
     const int64_t slice = 2 * (int64_t)kmax * wl;  // int64 words of one workgroup's bins
     int64_t* scratch = nullptr;
@@ -477,7 +854,6 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
     } else {
         grid = dim3((unsigned)std::min<int64_t>(n_items, ((int64_t)1 << 31) / 256));
     }
-    const bool timed = ctx->timing && !ctx->capturing;
     if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
     if (obj_flag == 0)
         e = global ? dispatch_joint<0, true>(dmax, grid, lds_need, st, tau, yq, wq, params, sp, gp,

@@ -236,6 +236,9 @@ struct cpb_context {
     int grid_per_cu = 0;
     // Set while a graph captures a search: no event is recorded on the stream.
     bool capturing = false;
+    // Joint multi-band search (cpb_set_joint_kernel): 0 automatic, 1 every band of a
+    // period at once, 2 band groups with register-resident box totals.
+    int joint_kernel = 0;
 };
 
 namespace cpb_engine {

@@ -283,6 +283,24 @@ def tls_power(
     return _finish(res, scale=scale, t_ref=t_ref, errors=dy is not None)
 
 
+def tls_periodogram(finite: LightCurve, periods: np.ndarray, res: dict[str, np.ndarray],
+                    backend: str) -> Periodogram:
+    """The periodogram of one searched curve: SDE as the power, ``sr``, ``depth``,
+    ``duration`` and ``t0`` as extras (src/cuperiod/methods/tls.py:799-823)."""
+    return Periodogram.from_spectrum(
+        method="TLS",
+        backend=backend,
+        frequency=1.0 / periods,
+        power=res["sde"],
+        objective_sense="max",
+        n_samples=finite.n,
+        baseline=finite.baseline,
+        extras={"sr": res["sr"], "depth": res["depth"], "duration": res["duration"],
+                "t0": res["t0"]},
+        meta=finite.meta,
+    )
+
+
 class TLSMethod(PeriodogramMethod):
     """Transit Least Squares -- limb-darkened matched filter (backend ``"hip"``)."""
 
@@ -323,18 +341,7 @@ class TLSMethod(PeriodogramMethod):
         periods = grid.period
         res = tls_power(finite.time, finite.value, finite.error, periods, settings=settings,
                         backend=backend)
-        return Periodogram.from_spectrum(
-            method="TLS",
-            backend=backend,
-            frequency=1.0 / periods,
-            power=res["sde"],
-            objective_sense="max",
-            n_samples=n,
-            baseline=finite.baseline,
-            extras={"sr": res["sr"], "depth": res["depth"], "duration": res["duration"],
-                    "t0": res["t0"]},
-            meta=finite.meta,
-        )
+        return tls_periodogram(finite, periods, res, backend)
 
     def estimate_device_bytes(self, n_points: int) -> int:
         return 64 * 1024**2 + n_points * 8 * 6
@@ -342,4 +349,10 @@ class TLSMethod(PeriodogramMethod):
 
 register(TLSMethod())
 
-__all__ = ["TLSMethod", "limb_darkened_template", "tls_power", "tls_power_batch"]
+__all__ = [
+    "TLSMethod",
+    "limb_darkened_template",
+    "tls_periodogram",
+    "tls_power",
+    "tls_power_batch",
+]

@@ -192,6 +192,15 @@ int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double* tau, cons
                          const cpb_segment* segments, int32_t n_segments, int32_t oversample,
                          int32_t obj_flag, double* out, int64_t n_out);
 
+/* Kernel of later joint searches on `ctx`: 0 (default) the band-group kernel whenever it
+ * covers the segments' phase spans (1,024 threads; bands pass through LDS a group at a
+ * time, box totals stay in registers), else every band of a period at once; 1 always
+ * every band at once (bins in LDS, or in global scratch when they do not fit); 2 always
+ * band groups (CPB_E_RESOURCE when they cannot cover the spans). Outputs agree to
+ * round-off across modes: mode 1 forms box sums from exact integers, the band-group
+ * kernel from FP64 prefix sums as the reference does. */
+int cpb_set_joint_kernel(cpb_context* ctx, int mode);
+
 /* One light curve of a TLS search. */
 typedef struct cpb_tls_curve {
     int64_t point_offset;   /* first point in the concatenated tau/yw/wv arrays */

@@ -74,8 +74,17 @@ def _check_shared(pg, ref, *, same_min=0.999):
     assert pg.best_period() == period[int(np.argmax(r["power"]))]
 
 
+@pytest.fixture(params=[0, 1], ids=["band-groups", "all-bands"])
+def joint_kernel(request):
+    """Both joint kernels: band groups (the default) and every band at once."""
+    ctx = _lib.context(0)
+    ctx.check(ctx.lib.cpb_set_joint_kernel(ctx.handle, request.param), "joint kernel")
+    yield request.param
+    ctx.check(ctx.lib.cpb_set_joint_kernel(ctx.handle, 0), "joint kernel")
+
+
 @pytest.mark.parametrize("objective", ["snr", "likelihood"])
-def test_shared_matches_oracle(objective):
+def test_shared_matches_oracle(objective, joint_kernel):
     bands, _ = multiband_star(7)
     mblc = _mblc(bands)
     grid = _grid(mblc)
@@ -86,7 +95,7 @@ def test_shared_matches_oracle(objective):
 
 
 @pytest.mark.parametrize("n_bands", [3, 6])
-def test_shared_default_grid_matches_oracle(n_bands):
+def test_shared_default_grid_matches_oracle(n_bands, joint_kernel):
     # default settings over a year: up to ~2000 phase bins per period, so 3 bands keep
     # their bins in LDS and 6 bands use the global-scratch layout
     bands, _ = multiband_star(21, n_bands=n_bands, n_band=120, span=365.0)
@@ -98,7 +107,7 @@ def test_shared_default_grid_matches_oracle(n_bands):
     _check_shared(pg, _oracle_shared(bands, grid, settings))
 
 
-def test_one_band_shared_is_the_single_band_search():
+def test_one_band_shared_is_the_single_band_search(joint_kernel):
     bands, _ = multiband_star(9, n_bands=1, n_band=120)
     mblc = _mblc(bands)
     grid = _grid(mblc)
@@ -106,8 +115,12 @@ def test_one_band_shared_is_the_single_band_search():
     single = BLS.power(grid, mblc.bands["b0"], SETTINGS, "hip")
     expected = np.clip(single.extras["depth_snr"], 0.0, None) ** 2
     np.testing.assert_allclose(shared.power, expected, rtol=1e-12, atol=1e-12)
+    # Boxes holding the same points tie exactly in the single-band engine (integer sums)
+    # and in the all-bands joint kernel; the band-group kernel differences FP64 prefix
+    # sums, as the reference does, so such ties break by rounding there. The powers
+    # agree everywhere (above), so a different box is always a tie.
     same = np.isclose(shared.extras["t0"], single.extras["t0"], rtol=0.0, atol=1e-9)
-    assert same.mean() >= 0.999
+    assert same.mean() >= (0.999 if joint_kernel == 1 else 0.99)
 
 
 def test_independent_stack_matches_oracle():
@@ -245,7 +258,7 @@ def _abi_joint(stars_bands, periods, durations, oversample, obj_flag):
     return [host[:, s * n:(s + 1) * n] for s in range(len(prepared))]
 
 
-def test_c_abi_stars_batch_equals_single_star_calls():
+def test_c_abi_stars_batch_equals_single_star_calls(joint_kernel):
     stars = []
     for seed, k in ((31, 2), (32, 5), (33, 1)):
         bands, _ = multiband_star(seed, n_bands=k, n_band=60)

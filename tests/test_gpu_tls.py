@@ -217,3 +217,26 @@ def test_batch_equals_single_curve_calls():
         alone = tls_power(t, y, dy, per, settings=settings)
         for key in ("sde", "sr", "depth", "duration", "t0"):
             np.testing.assert_array_equal(got[key], alone[key], err_msg=key)
+
+
+def test_batch_periodograms_rows_equal_single_curve_rows():
+    from cuperiod_amd.batch import io as bio
+
+    items = []
+    for seed in range(5):
+        t, flux, err = inject_transit(seed=seed, period=2.0 + 0.4 * seed, n=500, span=40.0)
+        items.append((f"c{seed}", cup.LightCurve.from_arrays(t, flux, err, domain="flux")))
+    settings = {"TLS": cup.TLSSettings(min_period_days=1.0)}
+    res = cup.batch_periodograms(items, ["BLS", "TLS"], settings=settings, chunk_size=3)
+    assert res.n_failed == 0
+    extra = bio.peak_extra_columns(["BLS", "TLS"])
+    tls_rows = [r for r in res.rows if r["method"] == "TLS"]
+    assert len(tls_rows) == len(items)
+    for row, (key, lc) in zip(tls_rows, items):
+        pg = cup.periodogram(lc, "TLS", settings=settings["TLS"])
+        want = bio.periodogram_to_row(key, pg, n_best=10, store_raw=False,
+                                      extra_keys=extra, source="")
+        assert row.keys() == want.keys()
+        for k, v in want.items():
+            same = (v == row[k]) or (isinstance(v, float) and np.isnan(v) and np.isnan(row[k]))
+            assert same, (k, row[k], v)

@@ -187,3 +187,31 @@ def test_periodogram_routes_tls(monkeypatch):
     peak = pg.best_periods(1, alias_diverse=True)[0]
     assert peak.period == pytest.approx(3.0, rel=5e-3)
     assert {"depth", "duration", "t0", "sr"} <= set(peak.extra)
+
+
+def test_batch_rows_equal_single_curve_rows(monkeypatch):
+    from cuperiod_amd.batch import io as bio
+
+    def oracle_curves(curves, *, n_bins, dur_bins, templates, den_rel):
+        return [otls.kernel(tau, yw, w, per, n_bins, dur_bins, templates, w_total, den_rel)
+                for tau, yw, w, w_total, per in curves]
+
+    monkeypatch.setattr(tls_mod, "_tls_hip_curves", oracle_curves)
+    monkeypatch.setattr(cup.get_method("TLS"), "resolve_backend_for", lambda b, s: "hip")
+    items = []
+    for seed in range(4):
+        t, flux, err = inject_transit(seed=seed, period=2.0 + 0.4 * seed, n=400, span=30.0)
+        items.append((f"c{seed}", cup.LightCurve.from_arrays(t, flux, err, domain="flux")))
+    items.append(("few", cup.LightCurve.from_arrays([0.0, 1.0], [1.0, 2.0], domain="flux")))
+    settings = cup.TLSSettings(min_period_days=1.0)
+    res = cup.batch_periodograms(items, "TLS", settings=settings, chunk_size=2)
+    assert res.n_failed == 1 and res.errors[0][0] == "few"
+    extra = bio.peak_extra_columns(["TLS"])
+    for row, (key, lc) in zip(res.rows, items):
+        pg = cup.periodogram(lc, "TLS", settings=settings)
+        want = bio.periodogram_to_row(key, pg, n_best=10, store_raw=False,
+                                      extra_keys=extra, source="")
+        assert row.keys() == want.keys()
+        for k, v in want.items():
+            same = (v == row[k]) or (isinstance(v, float) and np.isnan(v) and np.isnan(row[k]))
+            assert same, (k, row[k], v)
