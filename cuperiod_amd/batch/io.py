@@ -21,7 +21,7 @@ from collections.abc import Iterable, Mapping, Sequence
 from pathlib import Path
 from typing import Any
 
-from cuperiod_amd.core.lightcurve import LightCurve
+from cuperiod_amd.core.lightcurve import LightCurve, MultiBandLightCurve
 from cuperiod_amd.core.result import Peak, Periodogram
 
 #: Methods whose secondary peaks skip harmonics and aliases of the chosen ones.
@@ -32,6 +32,11 @@ PEAK_EXTRA_COLUMNS: Mapping[str, tuple[str, ...]] = {
     "BLS": ("depth", "depth_snr", "duration", "t0", "sde"),
     "TLS": ("sr", "depth", "duration", "t0"),
     "GLS": ("fap",),
+}
+
+#: Extras only multi-band results carry (src/cuperiod/batch/io.py:73-75).
+MULTIBAND_PEAK_EXTRA_COLUMNS: Mapping[str, tuple[str, ...]] = {
+    "BLS": ("band_index",),
 }
 
 _CSV_FORMULA_PREFIXES = ("=", "+", "-", "@", "\t", "\r")
@@ -48,12 +53,17 @@ def alias_diverse_for(method: str) -> bool:
     return _canonical_method(method) in ALIAS_DIVERSE_METHODS
 
 
-def peak_extra_columns(methods: Sequence[str]) -> tuple[str, ...]:
-    """Ordered union of the per-peak extras ``methods`` declare."""
+def peak_extra_columns(methods: Sequence[str], *, multiband: bool = False) -> tuple[str, ...]:
+    """Ordered union of the per-peak extras ``methods`` declare; ``multiband`` adds the
+    extras only multi-band results carry."""
     seen: dict[str, None] = {}
     for name in methods:
-        for extra in PEAK_EXTRA_COLUMNS.get(_canonical_method(name), ()):
+        key = _canonical_method(name)
+        for extra in PEAK_EXTRA_COLUMNS.get(key, ()):
             seen.setdefault(extra, None)
+        if multiband:
+            for extra in MULTIBAND_PEAK_EXTRA_COLUMNS.get(key, ()):
+                seen.setdefault(extra, None)
     return tuple(seen)
 
 
@@ -71,23 +81,25 @@ def _is_key(value: object) -> bool:
 
 
 def resolve_inputs(inputs: Any) -> list[InputItem]:
-    """``[(key, LightCurve)]`` from an iterable of light curves or ``(key, lc)`` pairs.
+    """``[(key, source)]`` from an iterable of light curves (single- or multi-band) or
+    ``(key, lc)`` pairs.
 
     A bare light curve is keyed by ``meta["id"]`` or its position. Keys must be strings
     or integers; repeated keys are renamed ``key#2``, ``key#3``, ... with a warning.
     File, glob and table inputs are not read by this package: load them into
-    :class:`LightCurve` objects first.
+    :class:`LightCurve` / :class:`MultiBandLightCurve` objects first.
     """
     if isinstance(inputs, (str, os.PathLike)):
         raise TypeError(
             "batch_periodograms: file and glob inputs are not read by this package; pass "
             "LightCurve objects or (key, LightCurve) pairs"
         )
-    if isinstance(inputs, LightCurve):
+    sources = (LightCurve, MultiBandLightCurve)
+    if isinstance(inputs, sources):
         inputs = [inputs]
     items: list[InputItem] = []
     for i, entry in enumerate(inputs):
-        if isinstance(entry, tuple) and len(entry) == 2 and isinstance(entry[1], LightCurve):
+        if isinstance(entry, tuple) and len(entry) == 2 and isinstance(entry[1], sources):
             key, lc = entry
             if not _is_key(key):
                 raise TypeError(
@@ -95,7 +107,7 @@ def resolve_inputs(inputs: Any) -> list[InputItem]:
                     f"{type(key).__name__}"
                 )
             items.append((str(key), lc))
-        elif isinstance(entry, LightCurve):
+        elif isinstance(entry, sources):
             key = str(entry.meta.get("id", i)) if entry.meta else str(i)
             items.append((key, entry))
         elif isinstance(entry, tuple) and len(entry) == 2 and isinstance(
@@ -294,6 +306,7 @@ def read_rows(path: Path) -> list[dict[str, Any]]:
 
 __all__ = [
     "ALIAS_DIVERSE_METHODS",
+    "MULTIBAND_PEAK_EXTRA_COLUMNS",
     "PEAK_EXTRA_COLUMNS",
     "InputItem",
     "alias_diverse_for",

@@ -54,7 +54,7 @@ from cuperiod_amd.batch.io import (
 from cuperiod_amd.core.columns import Domain, coerce_domain
 from cuperiod_amd.core.errors import InsufficientDataError
 from cuperiod_amd.core.grid import GridSpec
-from cuperiod_amd.core.lightcurve import LightCurve, raise_if_unusable
+from cuperiod_amd.core.lightcurve import LightCurve, MultiBandLightCurve, raise_if_unusable
 from cuperiod_amd.core.result import Peak, Periodogram, peak_defaults_from_settings
 from cuperiod_amd.methods._bls_core import _OUTPUT_NAMES
 from cuperiod_amd.methods.base import (
@@ -451,6 +451,23 @@ def _tls_rows(prepared: Sequence[_Prepared], settings: Any, cfg: "_ChunkConfig")
     return rows
 
 
+def _multiband_row(key: str, mblc: MultiBandLightCurve, name: str, settings: Any,
+                   cfg: "_ChunkConfig") -> dict[str, Any]:
+    """The row of one multi-band light curve: ``method.multiband_power`` on the given
+    grid or the stacked bands' default grid (src/cuperiod/batch/runner.py:185-207)."""
+    from cuperiod_amd.api import _multiband_grid
+
+    method = get_method(name)
+    if not method.supports_multiband:
+        raise ValueError(f"{method.name} does not support multi-band input")
+    raise_if_unusable(mblc, method.name)
+    g = cfg.grid if cfg.grid is not None else _multiband_grid(method, mblc, settings)
+    pg = method.multiband_power(g, mblc, settings, "hip")
+    pg = pg.with_peak_defaults(peak_defaults_from_settings(settings))
+    return periodogram_to_row(key, pg, n_best=cfg.n_best, store_raw=cfg.store_raw,
+                              extra_keys=cfg.extra_keys, source=source_label(mblc))
+
+
 def _curve_arrays(lc: LightCurve) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     err = lc.error if lc.error is not None else np.ones_like(lc.value)
     return lc.time, lc.value, err
@@ -620,11 +637,15 @@ class _ChunkRunner:
             settings = cfg.settings_map[name]
             prepare = _prepare_tls if name == "TLS" else _prepare
             prepared: list[_Prepared] = []
+            mb_rows: list[tuple[int, dict[str, Any]]] = []
             for pos, (key, lc) in enumerate(items):
                 if pos in errors:
                     continue
                 try:
-                    prepared.append(prepare(pos, key, lc, settings, cfg.grid))
+                    if isinstance(lc, MultiBandLightCurve):
+                        mb_rows.append((pos, _multiband_row(key, lc, name, settings, cfg)))
+                    else:
+                        prepared.append(prepare(pos, key, lc, settings, cfg.grid))
                 except Exception as exc:  # noqa: BLE001 - one bad light curve
                     errors[pos] = (key, f"{type(exc).__name__}: {exc}")
             if name == "TLS":
@@ -632,6 +653,8 @@ class _ChunkRunner:
             else:
                 handle = _launch(prepared, settings, cfg.grid) if prepared else None
             launches.append((settings, prepared, handle))
+            if mb_rows:
+                launches.append((settings, [], ("rows", mb_rows)))
         self._complete()
         self.pending = (idx, cfg, launches, errors)
 
@@ -645,9 +668,13 @@ class _ChunkRunner:
             if handle is None:
                 continue
             if isinstance(handle[0], str):
-                for prep, row in zip(prepared, handle[1]):
-                    if prep.position not in errors:
-                        by_pos.setdefault(prep.position, []).append(row)
+                # rows computed at submission: ("tls", rows of `prepared`) or
+                # ("rows", [(position, row)]) of multi-band light curves
+                pairs = (zip((p.position for p in prepared), handle[1])
+                         if handle[0] == "tls" else handle[1])
+                for pos, row in pairs:
+                    if pos not in errors:
+                        by_pos.setdefault(pos, []).append(row)
                 continue
             for prep, res in zip(prepared, _collect(handle, candidates=not cfg.store_raw)):
                 if prep.position in errors:
@@ -761,7 +788,8 @@ def batch_periodograms(
         return BatchSummary(0, 0, 0, 0, methods, device, summary_sink,
                             rows=[] if sink_kind == "memory" else None, n_workers=0)
 
-    extra_keys = peak_extra_columns(methods)
+    multiband = any(isinstance(src, MultiBandLightCurve) for _, src in items)
+    extra_keys = peak_extra_columns(methods, multiband=multiband)
     cfg = _ChunkConfig(methods, backends, settings_map, int(n_best), bool(store_raw),
                        grid, extra_keys)
     done: dict = {}

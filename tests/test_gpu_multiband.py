@@ -291,3 +291,23 @@ def test_oracle_single_band_sanity():
     r = oracle.bls_power(t, y, e, np.linspace(1.5, 4.5, 200), [0.05, 0.1], 5)
     got = cup.bls_power(t, y, e, np.linspace(1.5, 4.5, 200), [0.05, 0.1], 5, backend="hip")
     np.testing.assert_allclose(got.power, r["power"], rtol=1e-10, atol=1e-12)
+
+
+def test_batch_rows_equal_single_star_rows():
+    from cuperiod_amd.batch import io as bio
+
+    stars = [(f"s{i}", _mblc(multiband_star(50 + i, n_bands=3 + i % 2)[0])) for i in range(4)]
+    single = [("lc", cup.LightCurve.from_arrays(*multiband_star(60, n_bands=1)[0]["b0"],
+                                               domain="flux"))]
+    settings = SETTINGS.model_copy(update={"mb_ephemeris": "shared"})
+    res = cup.batch_periodograms(stars + single, "BLS", settings=settings, chunk_size=2)
+    assert res.n_failed == 0 and len(res.rows) == len(stars) + 1
+    extra = bio.peak_extra_columns(["BLS"], multiband=True)
+    for row, (key, mblc) in zip(res.rows, stars):
+        pg = cup.periodogram(mblc, "BLS", settings=settings)
+        want = bio.periodogram_to_row(key, pg, n_best=10, store_raw=False, extra_keys=extra,
+                                      source="")
+        assert row.keys() == want.keys()
+        for k, v in want.items():
+            assert v == row[k] or (isinstance(v, float) and np.isnan(v) and np.isnan(row[k])), k
+    assert res.rows[-1]["key"] == "lc" and np.isnan(res.rows[-1]["band_index_1"])

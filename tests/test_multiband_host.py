@@ -271,3 +271,30 @@ def test_periodogram_routes_multiband(monkeypatch):
     shared = SETTINGS.model_copy(update={"mb_ephemeris": "shared"})
     assert cup.periodogram(mblc, "BLS", settings=shared).meta["mb_ephemeris"] == "shared"
     assert cup.periodogram(mblc, "BLS", settings=SETTINGS).meta["mb_ephemeris"] == "independent"
+
+
+def test_batch_rows_for_multiband_inputs(monkeypatch):
+    from cuperiod_amd.batch import io as bio
+
+    monkeypatch.setattr("cuperiod_amd.methods._bls_joint.joint_search", _oracle_joint_search)
+    monkeypatch.setattr(bls_mod, "_segments_power", _oracle_segments_power)
+    monkeypatch.setattr(cup.get_method("BLS"), "resolve_backend_for", lambda b, s: "hip")
+    stars = [(f"s{i}", _mblc(multiband_star(40 + i, n_bands=3)[0])) for i in range(3)]
+    stars.insert(1, ("empty", _mblc({"g": (np.array([0.0, 1.0]), np.array([1.0, 1.0]),
+                                          np.array([1.0, 1.0]))})))
+    settings = SETTINGS.model_copy(update={"mb_ephemeris": "shared"})
+    res = cup.batch_periodograms(stars, "BLS", settings=settings)
+    assert res.n_failed == 1 and res.errors[0][0] == "empty"
+    extra = bio.peak_extra_columns(["BLS"], multiband=True)
+    assert extra[-1] == "band_index"
+    rows = iter(res.rows)
+    for key, mblc in stars:
+        if key == "empty":
+            continue
+        pg = cup.periodogram(mblc, "BLS", settings=settings)
+        want = bio.periodogram_to_row(key, pg, n_best=10, store_raw=False, extra_keys=extra,
+                                      source="")
+        row = next(rows)
+        assert row.keys() == want.keys()
+        for k, v in want.items():
+            assert v == row[k] or (isinstance(v, float) and np.isnan(v) and np.isnan(row[k])), k
