@@ -284,8 +284,10 @@ __device__ __forceinline__ float screen_value(int2 h0, int2 h1, int s_hi, int e_
     return (-t) * fabsf(t) * __builtin_amdgcn_rcpf(den);
 }
 
-template <int T, int DMAX, int OBJ, int MODE>
-__global__ __launch_bounds__(T) void bls_main_kernel(
+// KMODE 3 is MODE 0 for spans of up to 1,024 bins: its 18 KiB of LDS admit 8 waves per
+// SIMD, and the register budget is capped for them; MODE 0 is capped at 6 waves per SIMD.
+template <int T, int DMAX, int OBJ, int KMODE>
+__global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_main_kernel(
     const double* __restrict__ tau, const int64_t* __restrict__ yq,
     const int64_t* __restrict__ wq, const CurveParams* __restrict__ curves,
     const cpb_segment* __restrict__ segs, const WorkItem* __restrict__ items,
@@ -293,6 +295,7 @@ __global__ __launch_bounds__(T) void bls_main_kernel(
     const int32_t* __restrict__ dur_bins, int32_t oversample, int32_t wl,
     int64_t* __restrict__ wide_scratch, double* __restrict__ out, int64_t n_out) {
     extern __shared__ int64_t dyn_lds[];
+    constexpr int MODE = KMODE == 3 ? 0 : KMODE;
     constexpr int NW = T / kWave;
     constexpr bool GLOBAL = MODE == 2;
     __shared__ int32_t s_dur[DMAX];
@@ -1135,6 +1138,7 @@ hipError_t dispatch_d(int dmax, const LaunchArgs& a) {
 
 template <int T, int OBJ>
 hipError_t dispatch_mode(int mode, int dmax, const LaunchArgs& a) {
+    if (mode == 3) return dispatch_d<T, OBJ, 3>(dmax, a);
     if (mode == 0) return dispatch_d<T, OBJ, 0>(dmax, a);
     if (mode == 1) return dispatch_d<T, OBJ, 1>(dmax, a);
     return dispatch_d<T, OBJ, 2>(dmax, a);
@@ -1232,7 +1236,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         if (w > threads * kChunk) return 2;
         return w <= 1024 ? 0 : 1;
     };
-    constexpr int kModeOf[4] = {0, 0, 1, 2};
+    constexpr int kModeOf[4] = {3, 0, 1, 2};
     struct Layout {
         int64_t begin = 0, count = 0;
         int wmax = 2, dmax = 1;

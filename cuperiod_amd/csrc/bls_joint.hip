@@ -344,7 +344,11 @@ hipError_t dispatch_joint(int dmax, dim3 grid, size_t lds, hipStream_t st, const
 // FP64 operations and a Newton-refined reciprocal. The winning box's per-band sums are
 // read from the resident group and from re-folds of the others.
 
+// Workgroup sizes of the band-group kernel: 1,024 threads for wide spans (one workgroup
+// per CU holds up to 160 KiB of band bins), 256 for spans up to 1,024 bins (smaller band
+// groups, several workgroups per CU).
 constexpr int kGroupThreads = 1024;
+constexpr int kGroupThreadsNarrow = 256;
 
 struct BandTerm {
     double y_tot;    // sum of fixed-point y*ivar
@@ -367,6 +371,7 @@ __device__ __forceinline__ double div_newton(double n, double d) {
 
 // Zero, fold, wrap-copy and prefix the bins of bands g0 .. g0+gk-1 into LDS slots
 // 0 .. gk-1 (FP64 prefix sums of the exact integer bins). Ends with a barrier.
+template <int T>
 __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, int nb,
                                            int width, int oversample, double P, double invP,
                                            double bd, double inv_bd,
@@ -375,7 +380,6 @@ __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, i
                                            const double* __restrict__ tau,
                                            const int64_t* __restrict__ yq,
                                            const int64_t* __restrict__ wq) {
-    constexpr int T = kGroupThreads;
     constexpr int NW = T / kWave;
     const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
     longlong2* S2 = reinterpret_cast<longlong2*>(S);
@@ -424,8 +428,8 @@ __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, i
     __syncthreads();
 }
 
-template <int DMAX, int RMAX, int OBJ>
-__global__ __launch_bounds__(kGroupThreads) void bls_joint_group_kernel(
+template <int T, int DMAX, int RMAX, int OBJ>
+__global__ __launch_bounds__(T) void bls_joint_group_kernel(
     const double* __restrict__ tau, const int64_t* __restrict__ yq,
     const int64_t* __restrict__ wq, const CurveParams* __restrict__ bp,
     const cpb_star* __restrict__ stars, const cpb_segment* __restrict__ segs,
@@ -433,7 +437,6 @@ __global__ __launch_bounds__(kGroupThreads) void bls_joint_group_kernel(
     const int32_t* __restrict__ dur_bins, int32_t oversample, int32_t wl, int32_t group,
     double* __restrict__ out, int64_t n_out) {
     extern __shared__ int64_t dyn_lds[];
-    constexpr int T = kGroupThreads;
     constexpr int NW = T / kWave;
     __shared__ BandTerm s_term[kJointMaxBands];
     __shared__ int64_t s_cum[kJointMaxBands + 1];
@@ -502,8 +505,8 @@ __global__ __launch_bounds__(kGroupThreads) void bls_joint_group_kernel(
             int last_g0 = 0;
             for (int g0 = 0; g0 < K; g0 += group) {
                 const int gk = min(group, K - g0);
-                fold_group(S, wl, g0, gk, nb, width, oversample, P, invP, bd, inv_bd, s_cum,
-                           s_off, tau, yq, wq);
+                fold_group<T>(S, wl, g0, gk, nb, width, oversample, P, invP, bd, inv_bd,
+                              s_cum, s_off, tau, yq, wq);
                 for (int kk = 0; kk < gk; ++kk) {
                     const BandTerm bt = s_term[g0 + kk];
                     const double2* B = D2 + (int64_t)kk * wl;
@@ -588,8 +591,8 @@ __global__ __launch_bounds__(kGroupThreads) void bls_joint_group_kernel(
                     if (pass > 0 && g0 == last_g0) continue;
                     const int gk = min(group, K - g0);
                     if (pass > 0)
-                        fold_group(S, wl, g0, gk, nb, width, oversample, P, invP, bd, inv_bd,
-                                   s_cum, s_off, tau, yq, wq);
+                        fold_group<T>(S, wl, g0, gk, nb, width, oversample, P, invP, bd,
+                                      inv_bd, s_cum, s_off, tau, yq, wq);
                     for (int kk = tid; kk < gk; kk += T) {
                         const CurveParams c = bp[star.band_offset + g0 + kk];
                         const double2* B = D2 + (int64_t)kk * wl;
@@ -641,25 +644,26 @@ __global__ __launch_bounds__(kGroupThreads) void bls_joint_group_kernel(
     }
 }
 
-template <int DMAX, int RMAX, int OBJ>
+template <int T, int DMAX, int RMAX, int OBJ>
 hipError_t launch_group(dim3 grid, size_t lds, hipStream_t st, const double* tau,
                         const int64_t* yq, const int64_t* wq, const CurveParams* bp,
                         const cpb_star* stars, const cpb_segment* segs, const WorkItem* items,
                         int64_t n_items, const double* periods, const int32_t* dur, int32_t os,
                         int32_t wl, int32_t group, double* out, int64_t n_out) {
-    auto fn = bls_joint_group_kernel<DMAX, RMAX, OBJ>;
+    auto fn = bls_joint_group_kernel<T, DMAX, RMAX, OBJ>;
     if (lds > 65536)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(fn, grid, dim3(kGroupThreads), lds, st, tau, yq, wq, bp, stars, segs,
+    hipLaunchKernelGGL(fn, grid, dim3(T), lds, st, tau, yq, wq, bp, stars, segs,
                        items, n_items, periods, dur, os, wl, group, out, n_out);
     return hipGetLastError();
 }
 
-// Largest start count a group-kernel variant covers: RMAX * 1024 with RMAX * DMAX <= 32.
+// Starts per thread of a group-kernel variant (RMAX * DMAX <= 32): it covers spans of up to
+// RMAX * threads + 1 bins.
 __host__ __forceinline__ int group_rmax(int dmax) { return dmax <= 8 ? 4 : (dmax <= 16 ? 2 : 1); }
 
-template <int OBJ>
+template <int T, int OBJ>
 hipError_t dispatch_group(int dmax, dim3 grid, size_t lds, hipStream_t st, const double* tau,
                           const int64_t* yq, const int64_t* wq, const CurveParams* bp,
                           const cpb_star* stars, const cpb_segment* segs,
@@ -667,13 +671,13 @@ hipError_t dispatch_group(int dmax, dim3 grid, size_t lds, hipStream_t st, const
                           const int32_t* dur, int32_t os, int32_t wl, int32_t group,
                           double* out, int64_t n_out) {
     if (dmax <= 8)
-        return launch_group<8, 4, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
-                                       n_items, periods, dur, os, wl, group, out, n_out);
+        return launch_group<T, 8, 4, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
+                                          n_items, periods, dur, os, wl, group, out, n_out);
     if (dmax <= 16)
-        return launch_group<16, 2, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
-                                        n_items, periods, dur, os, wl, group, out, n_out);
-    return launch_group<32, 1, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
-                                    n_items, periods, dur, os, wl, group, out, n_out);
+        return launch_group<T, 16, 2, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
+                                           n_items, periods, dur, os, wl, group, out, n_out);
+    return launch_group<T, 32, 1, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
+                                       n_items, periods, dur, os, wl, group, out, n_out);
 }
 
 }  // namespace
@@ -755,15 +759,32 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
     const int64_t target_items = (int64_t)ctx->n_cu * 64;
     int64_t tile = live / std::max<int64_t>(1, target_items);
     tile = std::max<int64_t>(1, std::min<int64_t>(tile, 32));
-    ctx->h_items.clear();
+    // Span classes of the band-group kernel: 0 up to 1,024 bins (256 threads), 1 wider.
+    struct SpanClass {
+        int64_t begin = 0, count = 0;
+        int wmax = 2, dmax = 1;
+    } cls[2];
+    std::vector<WorkItem> by_class[2];
     for (int32_t s = 0; s < n_segments; ++s) {
-        for (int64_t b = 0; b < segments[s].n_periods; b += tile) {
+        const cpb_segment& sg = segments[s];
+        const int c = (sg.width + 63) / 64 * 64 <= 1024 ? 0 : 1;
+        if (sg.n_periods > 0) {
+            cls[c].wmax = std::max(cls[c].wmax, (int)sg.width);
+            cls[c].dmax = std::max(cls[c].dmax, (int)std::max(sg.n_dur, 1));
+        }
+        for (int64_t b = 0; b < sg.n_periods; b += tile) {
             WorkItem w;
             w.seg = s;
             w.begin = b;
-            w.count = (int32_t)std::min<int64_t>(tile, segments[s].n_periods - b);
-            ctx->h_items.push_back(w);
+            w.count = (int32_t)std::min<int64_t>(tile, sg.n_periods - b);
+            by_class[c].push_back(w);
         }
+    }
+    ctx->h_items.clear();
+    for (int c = 0; c < 2; ++c) {
+        cls[c].begin = (int64_t)ctx->h_items.size();
+        cls[c].count = (int64_t)by_class[c].size();
+        ctx->h_items.insert(ctx->h_items.end(), by_class[c].begin(), by_class[c].end());
     }
     const int64_t n_items = (int64_t)ctx->h_items.size();
 
@@ -814,27 +835,59 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
     const int wl = (wmax + 63) / 64 * 64;
     const size_t static_lds = 8192;
     const bool timed = ctx->timing && !ctx->capturing;
-    // Band-group kernel when its register tiles cover every start and one band fits in
-    // LDS; otherwise every band of a period at once, in LDS or global scratch.
-    const size_t band_bytes = (size_t)16 * wl;
-    const bool group_fits = wmax - 1 <= group_rmax(dmax) * kGroupThreads &&
-                            band_bytes + static_lds <= (size_t)ctx->lds_limit;
+    // Band-group kernel when, in every span class, its register tiles cover every start
+    // and one band fits in LDS; otherwise every band of a period at once, in LDS or
+    // global scratch.
+    int threads_of[2] = {0, 0};
+    bool group_fits = true;
+    for (int c = 0; c < 2; ++c) {
+        if (cls[c].count == 0) continue;
+        const size_t bb = (size_t)16 * ((cls[c].wmax + 63) / 64 * 64);
+        const int span = cls[c].wmax - 1;
+        const int rmax = group_rmax(cls[c].dmax);
+        if (c == 0 && span <= rmax * kGroupThreadsNarrow) threads_of[c] = kGroupThreadsNarrow;
+        else if (span <= rmax * kGroupThreads) threads_of[c] = kGroupThreads;
+        if (threads_of[c] == 0 || bb + static_lds > (size_t)ctx->lds_limit) group_fits = false;
+    }
     const int mode = ctx->joint_kernel;
     if (mode == 2 && !group_fits)
         return fail(ctx, CPB_E_RESOURCE, "the band-group joint kernel does not cover this span");
     if (mode != 1 && group_fits) {
-        const int gmax = std::max<int>(1, (int)((ctx->lds_limit - static_lds) / band_bytes));
-        const int n_groups = (kmax + gmax - 1) / gmax;
-        const int group = (kmax + n_groups - 1) / n_groups;
-        const size_t lds = band_bytes * group;
-        const dim3 grid((unsigned)std::min<int64_t>(n_items, ((int64_t)1 << 31) / kGroupThreads));
         if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
-        e = obj_flag == 0
-                ? dispatch_group<0>(dmax, grid, lds, st, tau, yq, wq, params, sp, gp, ip, n_items,
-                                    periods, dur_bins, oversample, wl, group, out, n_out)
-                : dispatch_group<1>(dmax, grid, lds, st, tau, yq, wq, params, sp, gp, ip, n_items,
-                                    periods, dur_bins, oversample, wl, group, out, n_out);
-        if (e != hipSuccess) return hip_fail(ctx, e, "bls_joint_group_kernel launch");
+        for (int c = 0; c < 2; ++c) {
+            if (cls[c].count == 0) continue;
+            const int T = threads_of[c];
+            const int wlc = (cls[c].wmax + 63) / 64 * 64;
+            const size_t bb = (size_t)16 * wlc;
+            // 256-thread workgroups keep their band groups to a third of the LDS, so three
+            // or more share a CU; 1,024-thread ones use all of it.
+            const size_t budget = T == kGroupThreadsNarrow
+                                      ? (size_t)ctx->lds_limit / 3 - static_lds
+                                      : (size_t)ctx->lds_limit - static_lds;
+            const int gmax = std::max<int>(1, (int)(budget / bb));
+            const int n_groups = (kmax + gmax - 1) / gmax;
+            const int group = (kmax + n_groups - 1) / n_groups;
+            const size_t lds = bb * group;
+            const dim3 grid((unsigned)std::min<int64_t>(cls[c].count, ((int64_t)1 << 31) / T));
+            const WorkItem* items = ip + cls[c].begin;
+            if (T == kGroupThreadsNarrow)
+                e = obj_flag == 0
+                        ? dispatch_group<kGroupThreadsNarrow, 0>(
+                              cls[c].dmax, grid, lds, st, tau, yq, wq, params, sp, gp, items,
+                              cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out)
+                        : dispatch_group<kGroupThreadsNarrow, 1>(
+                              cls[c].dmax, grid, lds, st, tau, yq, wq, params, sp, gp, items,
+                              cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out);
+            else
+                e = obj_flag == 0
+                        ? dispatch_group<kGroupThreads, 0>(
+                              cls[c].dmax, grid, lds, st, tau, yq, wq, params, sp, gp, items,
+                              cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out)
+                        : dispatch_group<kGroupThreads, 1>(
+                              cls[c].dmax, grid, lds, st, tau, yq, wq, params, sp, gp, items,
+                              cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out);
+            if (e != hipSuccess) return hip_fail(ctx, e, "bls_joint_group_kernel launch");
+        }
         if (timed) {
             CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
             ctx->timed = true;

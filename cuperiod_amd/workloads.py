@@ -1,6 +1,6 @@
 """Seeded synthetic survey workloads used by the benchmark and the tests.
 
-Five workloads, each generated with ``np.random.default_rng(20261203 + k)`` for the k-th
+Seven workloads, each generated with ``np.random.default_rng(20261203 + k)`` for the k-th
 of the list below:
 
 1. ``small_curve``: one curve, N = 2,000 over 100 d, box transit P = 3.7 d, depth 2e-3, duration 0.1 d,
@@ -133,8 +133,6 @@ def long_baseline_batch(n_curves: int = 256, n_freq: int = 4_000_000, n_points: 
     return Workload("long_baseline_batch", curves, grids, n_durations=16, truth=[0.7] * n_curves)
 
 
-__all__ = ["SEED", "Workload", "gapped_curve", "long_baseline_batch", "proposer_batch",
-           "small_curve", "survey_batch", "survey_curves"]
 
 
 def multiband_survey(n_stars: int = 64, n_bands: int = 6, n_per_band: int = 150,
@@ -185,3 +183,7 @@ def tls_survey(n_curves: int = 256, n_points: int = 20_000, span: float = 27.0):
         curves.append((t, y, np.full(n_points, sigma)))
         truth.append(period)
     return curves, truth
+
+
+__all__ = ["SEED", "Workload", "gapped_curve", "long_baseline_batch", "multiband_survey",
+           "proposer_batch", "small_curve", "survey_batch", "survey_curves", "tls_survey"]
