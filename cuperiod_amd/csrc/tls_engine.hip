@@ -2,13 +2,20 @@
 // src/cuperiod/methods/tls.py:1-26; kernels :270-355 and :491-595).
 //
 // One workgroup takes one (light curve, tile of trial periods) work item and, within it,
-// one trial period after another:
+// G trial periods at a time (G = 4 while their bins fit a quarter of the LDS):
 //
 //   (a) fold    every point's phase bin from the reference's FP64 expressions
 //               q = tau/P, ph = q - floor(q), bin = int(ph*n_bins) (clipped), bit for
-//               bit (correctly rounded division), and its weighted residual and weight
-//               into LDS bins as 64-bit fixed-point integers (ds_add_u64: exact,
-//               independent of the atomics' order);
+//               bit: q is formed through the reciprocal of P and, where it or ph*n_bins
+//               lies too close to an integer for that to decide the bin, by the
+//               correctly rounded division. Each point is loaded once for the G
+//               periods. Its weighted residual and weight enter LDS bins as 64-bit
+//               fixed-point integers: the points of a wave are 64 consecutive times, so
+//               runs of equal bins are common (short periods aside); the wave's
+//               inclusive prefix sums (the same for all G periods) give each run's sum as
+//               a difference, added by the run's last lane, less the prefix before it,
+//               subtracted by its first lane (ds_add_u64; exact and independent of the
+//               atomics' order, as integer sums modulo 2^64);
 //   (b) bins    the integer bins rounded once to FP64, with one circular pad of
 //               max_width-1 bins;
 //   (c) sweep   every (template width, start) window: num = sum g_k a, cwg = sum g_k b,
@@ -43,6 +50,25 @@ __device__ __forceinline__ bool tls_better(double a, uint32_t ka, double b, uint
     return a > b || (a == b && ka < kb);
 }
 
+// Adds the wave's 64 consecutive points to the bins of one period: `bb` the point's bin
+// (-1 for a lane past the curve's end), (a, b) its values and (pa, pb) the wave's
+// inclusive prefix sums of them.
+__device__ __forceinline__ void fold_runs(int64_t* bins, int bb, int64_t a, int64_t b,
+                                          int64_t pa, int64_t pb, int lane) {
+    const int prev = __shfl_up(bb, 1, kWave);
+    const bool head = lane == 0 || bb != prev;
+    const unsigned long long heads = __ballot(head);
+    const bool tail = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
+    if (bb >= 0 && (head || tail)) {
+        using u64 = unsigned long long;
+        const u64 da = (tail ? (u64)pa : 0ull) - (head ? (u64)pa - (u64)a : 0ull);
+        const u64 db = (tail ? (u64)pb : 0ull) - (head ? (u64)pb - (u64)b : 0ull);
+        atomicAdd(reinterpret_cast<u64*>(bins + 2 * bb), da);
+        atomicAdd(reinterpret_cast<u64*>(bins + 2 * bb + 1), db);
+    }
+}
+
+template <int G>
 __global__ __launch_bounds__(kTlsThreads) void tls_kernel(
     const double* __restrict__ tau, const int64_t* __restrict__ yq,
     const int64_t* __restrict__ wq, const CurveParams* __restrict__ cp,
@@ -56,18 +82,19 @@ __global__ __launch_bounds__(kTlsThreads) void tls_kernel(
     constexpr int NW = T / kWave;
     __shared__ int32_t s_w[kTlsMaxWidths];
     __shared__ int32_t s_off[kTlsMaxWidths];
-    __shared__ double s_red_v[NW];
-    __shared__ double s_red_d[NW];
-    __shared__ uint32_t s_red_k[NW];
+    __shared__ double s_red_v[G][NW];
+    __shared__ double s_red_d[G][NW];
+    __shared__ uint32_t s_red_k[G][NW];
 
     const int tid = threadIdx.x;
     const int lane = tid % kWave;
     const int wave = tid / kWave;
     const int n_ext = n_bins + max_width - 1;
-    // LDS: pairs (a, b) of n_ext bins: int64 during the fold, then FP64 in place;
-    // after them the templates.
+    const double dn = (double)n_bins;
+    // LDS: G arrays of n_ext (a, b) pairs, int64 during the fold and FP64 in place after
+    // it, then the templates.
     int64_t* P2 = dyn_lds;
-    double* s_tmpl = reinterpret_cast<double*>(dyn_lds + 2 * (int64_t)n_ext);
+    double* s_tmpl = reinterpret_cast<double*>(dyn_lds + 2 * (int64_t)G * n_ext);
 
     if (tid < n_widths) {
         s_w[tid] = widths[tid];
@@ -84,110 +111,157 @@ __global__ __launch_bounds__(kTlsThreads) void tls_kernel(
         const int64_t npts = cv.n_points;
         const double W = cv.w_total;
 
-        for (int q = 0; q < it.count; ++q) {
-            const int64_t k_per = it.begin + q;
-            const double P = periods[cv.period_offset + k_per];
-            const int64_t col = cv.out_offset + k_per;
+        for (int q0 = 0; q0 < it.count; q0 += G) {
+            const int ng = min(G, it.count - q0);
+            double Pg[G], invPg[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                Pg[g] = g < ng ? periods[cv.period_offset + it.begin + q0 + g] : 1.0;
+                invPg[g] = 1.0 / Pg[g];
+            }
 
-            // (a) zero, fold.
+            // (a) zero the bins, fold every point into each period's bins.
             longlong2* B2 = reinterpret_cast<longlong2*>(P2);
             __syncthreads();
-            for (int i = tid; i < n_bins; i += T) B2[i] = make_longlong2(0, 0);
+            for (int i = tid; i < ng * n_ext; i += T) B2[i] = make_longlong2(0, 0);
             __syncthreads();
-            for (int64_t j = tid; j < npts; j += T) {
-                const double qq = tau[poff + j] / P;
-                const double ph = qq - floor(qq);
-                int64_t bb = (int64_t)(ph * (double)n_bins);
-                bb = bb > n_bins - 1 ? n_bins - 1 : (bb < 0 ? 0 : bb);
-                atomicAdd(reinterpret_cast<unsigned long long*>(P2 + 2 * bb),
-                          (unsigned long long)yq[poff + j]);
-                atomicAdd(reinterpret_cast<unsigned long long*>(P2 + 2 * bb + 1),
-                          (unsigned long long)wq[poff + j]);
+            for (int64_t jb = 0; jb < npts; jb += T) {
+                const int64_t j = jb + tid;
+                const bool valid = j < npts;
+                const double x = valid ? tau[poff + j] : 0.0;
+                const int64_t a = valid ? yq[poff + j] : 0;
+                const int64_t b = valid ? wq[poff + j] : 0;
+                const int64_t pa = wave_incl_scan_dpp(a);
+                const int64_t pb = wave_incl_scan_dpp(b);
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    if (g < ng) {
+                        // q within 2^-51.4 |q| of tau/P through the reciprocal; where q lies
+                        // within 2^-49 |q| of an integer, or ph*n_bins within
+                        // (2^-49 |q| + 2^-50) n_bins of one, the division decides instead
+                        double qq = x * invPg[g];
+                        double ph = qq - floor(qq);
+                        double r = ph * dn;
+                        const double tq = fabs(qq) * 0x1p-49;
+                        if (ph < tq || 1.0 - ph <= tq ||
+                            fabs(r - rint(r)) <= (tq + 0x1p-50) * dn) {
+                            qq = x / Pg[g];
+                            ph = qq - floor(qq);
+                            r = ph * dn;
+                        }
+                        int bb = (int)r;
+                        bb = bb > n_bins - 1 ? n_bins - 1 : (bb < 0 ? 0 : bb);
+                        fold_runs(P2 + 2 * (int64_t)g * n_ext, valid ? bb : -1, a, b, pa, pb,
+                                  lane);
+                    }
+                }
             }
             __syncthreads();
-            // (b) FP64 bins in place, then the circular pad.
+            // (b) FP64 bins in place, then the circular pads.
             double2* D2 = reinterpret_cast<double2*>(P2);
-            for (int i = tid; i < n_bins; i += T) {
-                const longlong2 v = B2[i];
-                D2[i] = make_double2(ldexp((double)v.x, pr.y_exp), ldexp((double)v.y, pr.w_exp));
+            for (int i = tid; i < ng * n_bins; i += T) {
+                const int g = i / n_bins;
+                const int k = g * n_ext + (i - g * n_bins);
+                const longlong2 v = B2[k];
+                D2[k] = make_double2(ldexp((double)v.x, pr.y_exp), ldexp((double)v.y, pr.w_exp));
             }
             __syncthreads();
-            for (int i = tid; i < max_width - 1; i += T) D2[n_bins + i] = D2[i];
+            const int pad = max_width - 1;
+            for (int i = tid; i < ng * pad; i += T) {
+                const int g = i / pad;
+                const int k = i - g * pad;
+                D2[g * n_ext + n_bins + k] = D2[g * n_ext + k];
+            }
             __syncthreads();
 
-            // (c) sweep every (width, start) window.
-            double best = 0.0, best_depth = 0.0;
-            uint32_t bkey = 0xFFFFFFFFu;
+            // (c) sweep every (width, start) window of every period.
             const int n_win = n_widths * n_bins;
-            for (int idx = tid; idx < n_win; idx += T) {
-                const int di = idx / n_bins;
-                const int s = idx - di * n_bins;
-                const int width = s_w[di];
-                const double* g = s_tmpl + s_off[di];
-                const double g0 = g[0];
-                const double2 v0 = D2[s];
-                double num = g0 * v0.x;
-                double cwg = g0 * v0.y;
-                double den = (g0 * g0) * v0.y;
-                for (int k = 1; k < width; ++k) {
-                    const double gk = g[k];
-                    const double gkk = gk * gk;
-                    const double2 v = D2[s + k];
-                    num += gk * v.x;
-                    cwg += gk * v.y;
-                    den += gkk * v.y;
-                }
-                if (num < 0.0 && den > kWEps) {
-                    const double de = den - cwg * cwg / W;
-                    if (de > den_rel * den) {
-                        const double sr = num * num / de;
-                        const uint32_t key = (uint32_t)idx;
-                        if (sr > 0.0 && tls_better(sr, key, best, bkey)) {
-                            best = sr;
-                            bkey = key;
-                            best_depth = -num / de;
+            double best[G], best_depth[G];
+            uint32_t bkey[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                best[g] = 0.0;
+                best_depth[g] = 0.0;
+                bkey[g] = 0xFFFFFFFFu;
+                if (g < ng) {
+                    const double2* Dg = D2 + (int64_t)g * n_ext;
+                    for (int idx = tid; idx < n_win; idx += T) {
+                        const int di = idx / n_bins;
+                        const int s = idx - di * n_bins;
+                        const int width = s_w[di];
+                        const double* tg = s_tmpl + s_off[di];
+                        const double g0 = tg[0];
+                        const double2 v0 = Dg[s];
+                        double num = g0 * v0.x;
+                        double cwg = g0 * v0.y;
+                        double den = (g0 * g0) * v0.y;
+                        for (int k = 1; k < width; ++k) {
+                            const double gk = tg[k];
+                            const double gkk = gk * gk;
+                            const double2 v = Dg[s + k];
+                            num += gk * v.x;
+                            cwg += gk * v.y;
+                            den += gkk * v.y;
+                        }
+                        if (num < 0.0 && den > kWEps) {
+                            const double de = den - cwg * cwg / W;
+                            if (de > den_rel * den) {
+                                const double sr = num * num / de;
+                                const uint32_t key = (uint32_t)idx;
+                                if (sr > 0.0 && tls_better(sr, key, best[g], bkey[g])) {
+                                    best[g] = sr;
+                                    bkey[g] = key;
+                                    best_depth[g] = -num / de;
+                                }
+                            }
                         }
                     }
                 }
             }
-            // first-wins maximum over the workgroup
+            // first-wins maximum over the workgroup, per period
 #pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(best, off, kWave);
-                const double od = __shfl_xor(best_depth, off, kWave);
-                const uint32_t ok = (uint32_t)__shfl_xor((int)bkey, off, kWave);
-                if (tls_better(ov, ok, best, bkey)) {
-                    best = ov;
-                    bkey = ok;
-                    best_depth = od;
+            for (int g = 0; g < G; ++g) {
+#pragma unroll
+                for (int off = kWave / 2; off > 0; off >>= 1) {
+                    const double ov = __shfl_xor(best[g], off, kWave);
+                    const double od = __shfl_xor(best_depth[g], off, kWave);
+                    const uint32_t ok = (uint32_t)__shfl_xor((int)bkey[g], off, kWave);
+                    if (tls_better(ov, ok, best[g], bkey[g])) {
+                        best[g] = ov;
+                        bkey[g] = ok;
+                        best_depth[g] = od;
+                    }
+                }
+                if (lane == 0) {
+                    s_red_v[g][wave] = best[g];
+                    s_red_d[g][wave] = best_depth[g];
+                    s_red_k[g][wave] = bkey[g];
                 }
             }
-            if (lane == 0) {
-                s_red_v[wave] = best;
-                s_red_d[wave] = best_depth;
-                s_red_k[wave] = bkey;
-            }
             __syncthreads();
-            if (tid == 0) {
-                double b = s_red_v[0], d = s_red_d[0];
-                uint32_t k = s_red_k[0];
+            if (tid < ng) {
+                const int g = tid;
+                double bv = s_red_v[g][0], d = s_red_d[g][0];
+                uint32_t k = s_red_k[g][0];
                 for (int w = 1; w < NW; ++w)
-                    if (tls_better(s_red_v[w], s_red_k[w], b, k)) {
-                        b = s_red_v[w];
-                        d = s_red_d[w];
-                        k = s_red_k[w];
+                    if (tls_better(s_red_v[g][w], s_red_k[g][w], bv, k)) {
+                        bv = s_red_v[g][w];
+                        d = s_red_d[g][w];
+                        k = s_red_k[g][w];
                     }
                 int start = 0, width = 0;
-                if (b > 0.0) {
+                if (bv > 0.0) {
                     const int di = (int)(k / (uint32_t)n_bins);
                     start = (int)(k - (uint32_t)di * (uint32_t)n_bins);
                     width = s_w[di];
                 } else {
                     d = 0.0;
                 }
+                const double P = periods[cv.period_offset + it.begin + q0 + g];
+                const int64_t col = cv.out_offset + it.begin + q0 + g;
                 const double dur = (double)width / (double)n_bins * P;
                 const double centre = ((double)start + (double)width / 2.0) / (double)n_bins;
-                out[0 * n_out + col] = b;
+                out[0 * n_out + col] = bv;
                 out[1 * n_out + col] = d;
                 out[2 * n_out + col] = dur;
                 out[3 * n_out + col] = (centre - floor(centre)) * P;
@@ -234,9 +308,16 @@ extern "C" int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau,
         if (!(cv.w_total > 0.0)) return fail(ctx, CPB_E_INVALID, "w_total must be positive");
         live += cv.n_periods;
     }
-    const size_t lds = (size_t)16 * (n_bins + max_width - 1) + (size_t)8 * tmpl_len;
-    if (lds + 1024 > (size_t)ctx->lds_limit)
+    const size_t per_period = (size_t)16 * (n_bins + max_width - 1);
+    const size_t tmpl_bytes = (size_t)8 * tmpl_len;
+    if (per_period + tmpl_bytes + 1024 > (size_t)ctx->lds_limit)
         return fail(ctx, CPB_E_RESOURCE, "phase bins and templates exceed the LDS");
+    // periods per pass: 4 (or 2) while their bins and the templates stay within a quarter
+    // (a half) of the LDS, so that several workgroups share a CU
+    const int n_pass = per_period * 4 + tmpl_bytes <= (size_t)ctx->lds_limit / 4   ? 4
+                       : per_period * 2 + tmpl_bytes <= (size_t)ctx->lds_limit / 2 ? 2
+                                                                                   : 1;
+    const size_t lds = per_period * n_pass + tmpl_bytes;
 
     CPB_HIP(ctx, hipSetDevice(ctx->device));
     int rc;
@@ -246,6 +327,7 @@ extern "C" int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau,
 
     const int64_t target = (int64_t)ctx->n_cu * 64;
     int64_t tile = std::max<int64_t>(1, std::min<int64_t>(live / std::max<int64_t>(1, target), 32));
+    if (tile > n_pass) tile -= tile % n_pass;
     std::vector<TlsItem> items;
     for (int32_t c = 0; c < n_curves; ++c)
         for (int64_t b = 0; b < curves[c].n_periods; b += tile)
@@ -296,19 +378,27 @@ extern "C" int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau,
     if (e != hipSuccess) return hip_fail(ctx, e, "prep_curves_kernel launch");
     if (n_items == 0) return CPB_OK;
 
-    if (lds > 65536)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(tls_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const dim3 grid((unsigned)std::min<int64_t>(n_items, ((int64_t)1 << 31) / kTlsThreads));
     const bool timed = ctx->timing && !ctx->capturing;
     if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
-    hipLaunchKernelGGL(tls_kernel, grid, dim3(kTlsThreads), lds, st, tau, yq, wq, params,
-                       reinterpret_cast<const cpb_tls_curve*>(db + o_tc),
-                       reinterpret_cast<const TlsItem*>(db + o_it), n_items, periods, n_bins,
-                       reinterpret_cast<const int32_t*>(db + o_w),
-                       reinterpret_cast<const int32_t*>(db + o_off),
-                       reinterpret_cast<const double*>(db + o_tm), n_widths, max_width, tmpl_len,
-                       den_rel, out, n_out);
+    auto launch = [&](auto kernel) {
+        if (lds > 65536)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(kernel, grid, dim3(kTlsThreads), lds, st, tau, yq, wq, params,
+                           reinterpret_cast<const cpb_tls_curve*>(db + o_tc),
+                           reinterpret_cast<const TlsItem*>(db + o_it), n_items, periods,
+                           n_bins, reinterpret_cast<const int32_t*>(db + o_w),
+                           reinterpret_cast<const int32_t*>(db + o_off),
+                           reinterpret_cast<const double*>(db + o_tm), n_widths, max_width,
+                           tmpl_len, den_rel, out, n_out);
+    };
+    if (n_pass == 4)
+        launch(tls_kernel<4>);
+    else if (n_pass == 2)
+        launch(tls_kernel<2>);
+    else
+        launch(tls_kernel<1>);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "tls_kernel launch");
     if (timed) {

@@ -215,3 +215,29 @@ def test_batch_rows_equal_single_curve_rows(monkeypatch):
         for k, v in want.items():
             same = (v == row[k]) or (isinstance(v, float) and np.isnan(v) and np.isnan(row[k]))
             assert same, (k, row[k], v)
+
+
+def test_reciprocal_fold_guard_selects_the_division_bin():
+    """The TLS kernel folds through tau * (1/P) and falls back to tau / P wherever q or
+    ph * n_bins lies too close to an integer for the product to decide the bin; the bin
+    chosen this way must equal the bin of the correctly rounded division everywhere,
+    including points planted at exact multiples of the period and bin edges."""
+    rng = np.random.default_rng(5)
+    n_bins = 256
+    periods = np.concatenate([rng.uniform(0.3, 15.0, 400), [0.5, 1.0, 3.0, 7.1, 10.0]])
+    x = rng.uniform(0.0, 1200.0, 4000)
+    for P in periods:
+        k = rng.integers(0, 80, 600).astype(float)
+        edge = (k + rng.integers(0, n_bins, 600) / n_bins) * P
+        xs = np.concatenate([x, k * P, edge, np.nextafter(edge, 0), np.nextafter(edge, 1e9),
+                             -x[:200]])
+        q = xs / P
+        want = np.clip((q - np.floor(q)) * n_bins, 0, n_bins - 1).astype(np.int64)
+        qq = xs * (1.0 / P)
+        ph = qq - np.floor(qq)
+        r = ph * n_bins
+        tq = np.abs(qq) * 2.0**-49
+        redo = (ph < tq) | (1.0 - ph <= tq) | (np.abs(r - np.rint(r)) <= (tq + 2.0**-50) * n_bins)
+        r[redo] = (q[redo] - np.floor(q[redo])) * n_bins
+        got = np.clip(r, 0, n_bins - 1).astype(np.int64)
+        np.testing.assert_array_equal(got, want)
