@@ -230,7 +230,11 @@ __device__ __forceinline__ bool exact_key(const BinView<MODE>& bv, int n, int d,
     const double num = fma(Yo, Wi, -(Yi * Wo));
     if (num < 0.0) return false;
     k.num2 = num * num;
-    k.den = OBJ == 0 ? Wi * Wo * (double)cv.w_tot : Wi * Wo * Wo;
+    // converted at the use (an empty asm hides the value from loop-invariant code
+    // motion), not held in registers across the whole work item
+    int64_t wt = cv.w_tot;
+    asm volatile("" : "+s"(wt));
+    k.den = OBJ == 0 ? Wi * Wo * (double)wt : Wi * Wo * Wo;
     k.yin = yin_q;
     k.win = win_q;
     return true;
@@ -367,7 +371,11 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
             if constexpr (GLOBAL) __threadfence();
             __syncthreads();
             // two points per thread per pass, both loaded before either is folded
-            for (int j0 = tid; j0 < npts; j0 += 2 * T) {
+            // the lane's addresses are formed per period (the empty asm keeps them from
+            // being hoisted and held in registers across the work item)
+            int jstart = tid;
+            asm volatile("" : "+v"(jstart));
+            for (int j0 = jstart; j0 < npts; j0 += 2 * T) {
                 const int j1 = j0 + T;
                 const double x0 = tau[poff + j0];
                 const int64_t a0 = yq[poff + j0];
@@ -539,29 +547,31 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
             const float thr = V * kKeep;
             const int thr_bits = __float_as_int(thr);
 
-            // (b3) exact re-evaluation of what the screen kept, first wins.
+            // (b3) exact re-evaluation of what the screen kept, first wins: the wanted
+            // durations of this thread, in ascending order.
             BoxKey best{-1.0, 1.0, 0, 0};
             uint32_t bkey = 0xFFFFFFFFu;
+            uint32_t want_bits = 0;
 #pragma unroll
-            for (int di = 0; di < DMAX; ++di) {
-                if (di < n_dur) {
-                    const int d = s_dur[di];
-                    const bool want =
-                        force || (cm[di] >= thr_bits) || ((unc_bits >> di) & 1u);
-                    if (d >= 1 && d < n_bins && want) {
-                        const int m = n_bins - d + 1;
-                        for (int n = tid; n < m; n += T) {
-                            bool cert, unc;
-                            const float v = screen_value<OBJ>(bv.hi(n), bv.hi(n + d), s_hi,
-                                                              e_unc, ybar, cert, unc);
-                            if (force || (cert && v >= thr) || unc) {
-                                BoxKey kb;
-                                if (exact_key<OBJ, MODE>(bv, n, d, cv, kb) &&
-                                    kb.num2 * best.den > best.num2 * kb.den) {
-                                    best = kb;
-                                    bkey = ((uint32_t)di << 24) | (uint32_t)n;
-                                }
-                            }
+            for (int di = 0; di < DMAX; ++di)
+                if (force || (cm[di] >= thr_bits) || ((unc_bits >> di) & 1u))
+                    want_bits |= 1u << di;
+            want_bits &= dok;
+            while (want_bits) {
+                const int di = __builtin_ctz(want_bits);
+                want_bits &= want_bits - 1;
+                const int d = s_dur[di];
+                const int m = n_bins - d + 1;
+                for (int n = tid; n < m; n += T) {
+                    bool cert, unc;
+                    const float v = screen_value<OBJ>(bv.hi(n), bv.hi(n + d), s_hi, e_unc,
+                                                      ybar, cert, unc);
+                    if (force || (cert && v >= thr) || unc) {
+                        BoxKey kb;
+                        if (exact_key<OBJ, MODE>(bv, n, d, cv, kb) &&
+                            kb.num2 * best.den > best.num2 * kb.den) {
+                            best = kb;
+                            bkey = ((uint32_t)di << 24) | (uint32_t)n;
                         }
                     }
                 }
