@@ -143,8 +143,13 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
         p.s_hi = (int32_t)s_hi;
         const int64_t e = s_hi >> 20 > 1 ? s_hi >> 20 : 1;
         p.e_unc = (int32_t)e;
-        const double terr = 1.0 + fabs(ybar);
+        // Bound on |t_screen - t| in high-word units, t = Y - ybar*W: the truncated high
+        // words (1 unit of Y, 1 of W times |ybar|) and the FP32 screen's roundings --
+        // Y and W converted to float (|Y| <= |t| + |ybar| W, W < s: 2^-24 relative
+        // each), ybar rounded to float (2^-24 relative, times W) and the fma itself
+        // (|t| 2^-24, left to the margin): 3 |ybar| s 2^-24 more.
         const double s = (double)s_hi, ed = (double)e;
+        const double terr = 1.0 + fabs(ybar) * (1.0 + 3.0 * s * 0x1p-24);
         double vdeg, vbnd;
         if (obj_flag == 0) {
             vdeg = 4096.0 * terr * terr / (ed * (s - ed));
@@ -271,8 +276,12 @@ __device__ __forceinline__ void wave_best_ballot(BoxKey& k, uint32_t& key) {
 }
 
 // FP32 screen value of box (n, d): -t|t| / den with t = Y - ybar*W (a dip has t <= 0),
-// den = W*Wout (snr) or W*Wout^2 (likelihood), from the prefix high words; `cert` is
-// false when either side holds at most `e_unc` high-word units of weight.
+// den = W*Wout (snr) or W*Wout^2 (likelihood), from the prefix high words. The high
+// words carry W to within 1 unit and Wout to within 2, so den is within 2/e_unc of its
+// value when both sides hold more than `e_unc` units (snr), or the box more than e_unc
+// and the rest more than 4*e_unc (likelihood: Wout enters squared); `cert` says so.
+// A likelihood box with little weight outside can still score high: `unc` marks those
+// (the rest at most 4*e_unc units) for exact evaluation.
 template <int OBJ>
 __device__ __forceinline__ float screen_value(int2 h0, int2 h1, int s_hi, int e_unc,
                                               float ybar, bool& cert, bool& unc) {
@@ -283,8 +292,9 @@ __device__ __forceinline__ float screen_value(int2 h0, int2 h1, int s_hi, int e_
     const float Wo = (float)dwo;
     const float t = fmaf(-ybar, W, Y);
     const float den = OBJ == 0 ? W * Wo : W * Wo * Wo;
-    cert = min(dw, dwo) > e_unc;
-    unc = OBJ == 1 && (dw > e_unc) & (dwo <= e_unc) & (dwo > -2);
+    const int e_out = OBJ == 0 ? e_unc : 4 * e_unc;
+    cert = (dw > e_unc) & (dwo > e_out);
+    unc = OBJ == 1 && (dw > e_unc) & (dwo <= e_out) & (dwo > -2);
     return (-t) * fabsf(t) * __builtin_amdgcn_rcpf(den);
 }
 

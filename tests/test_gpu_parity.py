@@ -154,6 +154,28 @@ def test_mid_span_histogram_path_matches_oracle():
         _say(f"  errors {errs}")
 
 
+@pytest.mark.parametrize("objective", ["snr", "likelihood"])
+def test_skewed_flux_and_wide_boxes_match_oracle(objective):
+    # The FP32 screen's error bounds at their edges: skewed flux (median far from the
+    # weighted mean, so |ybar| is large in the screen's units), boxes holding most of
+    # the weight (high words above 2^24, rounded when converted to float), and a few
+    # points of tiny weight (near-empty out-of-box sides for the likelihood).
+    _say(f"case skewed flux, wide boxes objective={objective}")
+    rng = np.random.default_rng(11)
+    t = np.sort(rng.uniform(0.0, 40.0, 2500))
+    flux = 1e-3 * rng.standard_normal(t.size)
+    flux[rng.random(t.size) < 0.4] += 1.0
+    flux[np.mod(t - 0.7, 3.1) < 1.2] -= 0.3
+    err = np.full(t.size, 1e-3)
+    err[rng.random(t.size) < 0.01] = 1e2
+    periods = np.linspace(0.8, 4.0, 300)
+    durations = np.array([0.2, 0.6, 1.2, 2.0])
+    got = _hip_power(t, flux, err, periods, durations, 5, objective)
+    ref = oracle.bls_power(t, flux, err, periods, durations, 5, objective=objective)
+    errs = compare(got, ref, name=f"skewed/{objective}")
+    _say(f"  errors {errs}")
+
+
 def test_segments_equal_per_segment_calls():
     # tests/test_bls_segments.py:42-61: the segmented call equals one call per
     # segment; an empty segment comes back empty, in place.
