@@ -141,7 +141,8 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
         p.n_points = (int32_t)n;
         const int64_t s_hi = tw >> 32;
         p.s_hi = (int32_t)s_hi;
-        const int64_t e = s_hi >> 20 > 1 ? s_hi >> 20 : 1;
+        // s_hi >= 2^26 - 1 (the weights map to [2^58, 2^59) units), so e_unc >= 511.
+        const int64_t e = s_hi >> 17 > 1 ? s_hi >> 17 : 1;
         p.e_unc = (int32_t)e;
         // Bound on |t_screen - t| in high-word units, t = Y - ybar*W: the truncated high
         // words (1 unit of Y, 1 of W times |ybar|) and the FP32 screen's roundings --
@@ -152,17 +153,17 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
         const double terr = 1.0 + fabs(ybar) * (1.0 + 3.0 * s * 0x1p-24);
         double vdeg, vbnd;
         if (obj_flag == 0) {
-            vdeg = 4096.0 * terr * terr / (ed * (s - ed));
+            vdeg = 65536.0 * terr * terr / (ed * (s - ed));
             vbnd = 2.0 * cmax * cmax * (ed + 2.0) / (s - ed - 2.0);
         } else {
-            vdeg = 4096.0 * terr * terr / (ed * ed * (s - ed));
+            vdeg = 65536.0 * terr * terr / (ed * ed * (s - ed));
             vbnd = 2.0 * cmax * cmax * (ed + 2.0) / ((s - ed - 2.0) * (s - ed - 2.0));
         }
         p.vdeg = (float)fmin(vdeg, 3.0e38);
         p.vbnd = (float)fmin(vbnd, 3.0e38);
         int flags = 0;
         if (!finite) flags |= 2;
-        if (s_hi < (int64_t(1) << 22)) flags |= 1;  // no weight to speak of: exact sweep
+        if (e < 511) flags |= 1;  // too little weight for the screen's bounds: exact sweep
         p.flags = flags;
         params[blockIdx.x] = p;
     }

@@ -23,13 +23,13 @@ constexpr int kWave = 64;
 // prefix sums that also count the `oversample` wrapped bins twice stay below 2^61 and
 // their high words (value >> 32) below 2^29 in magnitude.
 constexpr int kFixBits = 59;
-// Screen margin: boxes whose FP32 value reaches 7/8 of the best FP32 value are
+// Screen margin: boxes whose FP32 value reaches 0.97 of the best FP32 value are
 // re-evaluated exactly. When the best screen value is above the curve's `vdeg` bound,
-// the boxes that matter have |t| at least 64 times its error bound and a denominator
-// within 2/e_unc <= 1/32 of exact, so a screen value lies within [0.94, 1.065] of its
-// box's exact objective and the best box reaches at least 0.882 of the best screen
+// the boxes that matter have |t| at least 256 times its error bound and a denominator
+// within 2/e_unc <= 2/511 of exact, so a screen value lies within [0.9883, 1.0118] of
+// its box's exact objective and the best box reaches at least 0.9767 of the best screen
 // value; below `vdeg` every box is evaluated exactly.
-constexpr float kKeep = 0.875f;
+constexpr float kKeep = 0.97f;
 // Distinct box widths (duration bins) per segment.
 constexpr int kMaxDurations = 32;
 
