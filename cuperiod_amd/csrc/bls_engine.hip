@@ -1247,22 +1247,24 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     // launch per layout, so a wide segment does not push narrow ones out of LDS. (The
     // reduced-precision kernel takes every item in one launch.)
     const int threads = ctx->threads;
-    // Classes: 0 MODE 0 up to 1,024 bins (18 KiB of LDS, more workgroups per CU), 1 MODE 0
-    // up to threads*8 bins, 2 MODE 1, 3 MODE 2.
+    // Classes: 0 MODE 0 up to 1,024 bins (18 KiB of LDS; its own instantiation for 8
+    // waves per SIMD), 1-3 MODE 0 up to 1,280 / 1,536 / threads*8 bins (23, 28, 37 KiB:
+    // 6, 5, 4 workgroups per CU), 4 MODE 1, 5 MODE 2.
+    constexpr int kClasses = 6;
     auto layout_of = [&](int width) {
         if (ctx->accumulation != 0) return 0;
         const int w = (width + 63) / 64 * 64;
         const size_t need = (size_t)16 * (w + (w - 1) / 8 + 1);
-        if (need + 2048 > (size_t)ctx->lds_limit) return 3;
-        if (w > threads * kChunk) return 2;
-        return w <= 1024 ? 0 : 1;
+        if (need + 2048 > (size_t)ctx->lds_limit) return 5;
+        if (w > threads * kChunk) return 4;
+        return w <= 1024 ? 0 : (w <= 1280 ? 1 : (w <= 1536 ? 2 : 3));
     };
-    constexpr int kModeOf[4] = {3, 0, 1, 2};
+    constexpr int kModeOf[kClasses] = {3, 0, 0, 0, 1, 2};
     struct Layout {
         int64_t begin = 0, count = 0;
         int wmax = 2, dmax = 1;
-    } layouts[4];
-    std::vector<WorkItem> by_layout[4];
+    } layouts[kClasses];
+    std::vector<WorkItem> by_layout[kClasses];
     std::vector<cpb_segment> empties;
     for (int32_t s = 0; s < n_segments; ++s) {
         const cpb_segment& sg = segments[s];
@@ -1283,7 +1285,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         }
     }
     ctx->h_items.clear();
-    for (int L = 0; L < 4; ++L) {
+    for (int L = 0; L < kClasses; ++L) {
         layouts[L].begin = (int64_t)ctx->h_items.size();
         layouts[L].count = (int64_t)by_layout[L].size();
         ctx->h_items.insert(ctx->h_items.end(), by_layout[L].begin(), by_layout[L].end());
@@ -1371,7 +1373,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     const int64_t persistent = (int64_t)ctx->n_cu * ctx->grid_per_cu;
     const bool timed = ctx->timing && !ctx->capturing;
     if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
-    for (int L = 0; L < 4; ++L) {
+    for (int L = 0; L < kClasses; ++L) {
         const Layout& lay = layouts[L];
         if (lay.count == 0) continue;
         const int wlL = (lay.wmax + 63) / 64 * 64;
