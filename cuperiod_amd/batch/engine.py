@@ -394,6 +394,66 @@ def plan_grid(cs: CurveSet, c: int, grid: LinspaceGrid | GridSpec | None,
     return _explicit_plan(c, grid, settings)
 
 
+def _parametric_need(par: dict[str, np.ndarray], idx: np.ndarray, k: np.ndarray,
+                     oversample: int) -> np.ndarray:
+    """Phase bins ``ceil(P/bd) + os + 1`` of period ``k`` of parametric segment ``idx``
+    (arrays; ``par`` holds the segments' parameters), the periods by the device's recipe
+    (``periods_at``)."""
+    num = par["num"][idx]
+    i = num - 1 - (par["k0"][idx] + k)
+    start = par["start"][idx]
+    f = i.astype(np.float64) * par["step"][idx] + start
+    mode = par["mode"][idx]
+    f = np.where((mode == 1) & (i == num - 1), par["stop"][idx], f)
+    f = np.where((mode == 2) & (i == 0), start, f)
+    return np.ceil((1.0 / f) / par["bd"][idx]).astype(np.int64) + int(oversample) + 1
+
+
+def _parametric_width_ranges(sgs: Sequence[_SegPlan], oversample: int
+                             ) -> list[list[tuple[int, int, int]]]:
+    """``core._width_ranges`` of many parametric segments at once: their periods ascend
+    with the index, so the first index past each span limit is found by a vectorised
+    bisection instead of materialising every period."""
+    par = {
+        "start": np.array([sg.lin_start for sg in sgs], dtype=np.float64),
+        "step": np.array([sg.lin_step for sg in sgs], dtype=np.float64),
+        "stop": np.array([sg.lin_stop for sg in sgs], dtype=np.float64),
+        "num": np.array([sg.lin_num for sg in sgs], dtype=np.int64),
+        "k0": np.array([sg.lin_k0 for sg in sgs], dtype=np.int64),
+        "mode": np.array([sg.mode for sg in sgs], dtype=np.int64),
+        "bd": np.array([float(sg.durations.min()) / oversample for sg in sgs]),
+    }
+    n_seg = len(sgs)
+    limits = np.asarray(core._SPAN_LIMITS, dtype=np.int64)
+    n = np.array([sg.n_periods for sg in sgs], dtype=np.int64)
+    idx = np.repeat(np.arange(n_seg), limits.size)
+    lim = np.tile(limits, n_seg)
+    lo = np.zeros(idx.size, dtype=np.int64)
+    hi = n[idx].copy()
+    while np.any(lo < hi):
+        mid = (lo + hi) // 2
+        past = _parametric_need(par, idx, np.minimum(mid, n[idx] - 1), oversample) > lim
+        open_ = lo < hi
+        hi = np.where(open_ & past, mid, hi)
+        lo = np.where(open_ & ~past, mid + 1, lo)
+    first = np.minimum(lo.reshape(n_seg, limits.size), n[:, None])
+    # intervals [0, first_0), [first_0, first_1), ..., [first_last, n); empty ones dropped
+    starts = np.concatenate([np.zeros((n_seg, 1), dtype=np.int64), first], axis=1).ravel()
+    ends = np.concatenate([first, n[:, None]], axis=1).ravel()
+    seg_of = np.repeat(np.arange(n_seg), limits.size + 1)
+    keep = ends > starts
+    starts, ends, seg_of = starts[keep], ends[keep], seg_of[keep]
+    widths = _parametric_need(par, seg_of, ends - 1, oversample)
+    counts = np.bincount(seg_of, minlength=n_seg)
+    rows = list(zip(starts.tolist(), ends.tolist(), widths.tolist()))
+    out: list[list[tuple[int, int, int]]] = []
+    pos = 0
+    for m in counts.tolist():
+        out.append(rows[pos:pos + m])
+        pos += m
+    return out
+
+
 def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) -> PackedBatch:
     """Segment descriptors of ``plans`` (one per curve of ``cs``, in order) over the
     uploaded curves."""
@@ -405,17 +465,30 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
     # descriptor pieces: a segment whose periods need different phase-bin layouts is cut
     # where the width crosses the engine's span limits (outputs do not depend on it)
     pieces: list[tuple[int, _SegPlan, int, int, int]] = []
+    parametric: list[tuple[int, int, _SegPlan]] = []
     for c, plan in enumerate(plans):
         for sg in plan.segments:
             bd = float(sg.durations.min()) / oversample
             width = int(np.ceil(sg.p_max / bd)) + oversample + 1
             if width <= core._SPAN_LIMITS[0] or sg.n_periods < 2:
                 pieces.append((c, sg, 0, sg.n_periods, width))
-                continue
-            sub = CurvePlan(segments=[sg], n_periods=sg.n_periods)
-            per = sg.periods if sg.mode == 0 else sub.periods_at(np.arange(sg.n_periods))
-            for a, b, w in core._width_ranges(per, bd, oversample):
-                pieces.append((c, sg, a, b, w))
+            elif sg.mode == 0:
+                for a, b, w in core._width_ranges(sg.periods, bd, oversample):
+                    pieces.append((c, sg, a, b, w))
+            else:
+                parametric.append((len(pieces), c, sg))
+                pieces.append((c, sg, -1, -1, -1))  # filled in below
+    if parametric:
+        cut = _parametric_width_ranges([sg for _, _, sg in parametric], oversample)
+        expanded: list[tuple[int, _SegPlan, int, int, int]] = []
+        slots = {pos: (c, sg, rng) for (pos, c, sg), rng in zip(parametric, cut)}
+        for pos, piece in enumerate(pieces):
+            if pos in slots:
+                c, sg, rng = slots[pos]
+                expanded.extend((c, sg, a, b, w) for a, b, w in rng)
+            else:
+                expanded.append(piece)
+        pieces = expanded
     segs = (_lib.Segment * max(1, len(pieces)))()
     dur_parts: list[np.ndarray] = []
     per_parts: list[np.ndarray] = []

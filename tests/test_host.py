@@ -339,3 +339,31 @@ def test_local_maxima_and_downsample():
     p[1234] = 5.0
     fd, pd = peak_preserving_downsample(f, p, 100)
     assert pd.max() == np.float32(5.0) and fd[np.argmax(pd)] == np.float32(1234.0)
+
+
+def test_parametric_width_cuts_equal_materialised_cuts():
+    # The batch planner cuts parametric (linspace) segments at the engine's span classes
+    # by bisection on the period index; the cuts must equal those of the materialised
+    # periods (core._width_ranges), including segments that cross several classes.
+    from cuperiod_amd.batch.engine import (
+        CurvePlan, LinspaceGrid, _linspace_plan, _parametric_width_ranges)
+
+    settings = cup.BLSSettings(n_durations=6)
+    rng = np.random.default_rng(2)
+    sgs = []
+    for c, base in enumerate(rng.uniform(50.0, 2000.0, 6)):
+        plan = _linspace_plan(c, LinspaceGrid(1.0 / base, 2.0, 20_000), settings)
+        sgs += [sg for sg in plan.segments if sg.mode != 0 and sg.n_periods >= 2]
+    got = _parametric_width_ranges(sgs, settings.bins_per_duration)
+    crossing = 0
+    for sg, ranges in zip(sgs, got):
+        bd = float(sg.durations.min()) / settings.bins_per_duration
+        per = CurvePlan(segments=[sg], n_periods=sg.n_periods).periods_at(
+            np.arange(sg.n_periods))
+        want = core._width_ranges(per, bd, settings.bins_per_duration)
+        if len(want) == 1:
+            assert len(ranges) == 1 and ranges[0][:2] == (0, sg.n_periods)
+        else:
+            assert ranges == want
+            crossing += 1
+    assert crossing >= 6
