@@ -1152,6 +1152,7 @@ hipError_t launch_main(const LaunchArgs& a) {
 
 template <int T, int OBJ, int MODE>
 hipError_t dispatch_d(int dmax, const LaunchArgs& a) {
+    if (dmax <= 6) return launch_main<T, 6, OBJ, MODE>(a);
     if (dmax <= 8) return launch_main<T, 8, OBJ, MODE>(a);
     if (dmax <= 16) return launch_main<T, 16, OBJ, MODE>(a);
     return launch_main<T, 32, OBJ, MODE>(a);
