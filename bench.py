@@ -42,11 +42,11 @@ PEAK_FP32 = 157.3e12
 PEAK_LDS = 256 * 128 * 2.4e9
 PEAK_HBM = 8.0e12
 COPY_BYTES = 256 << 20
-#: HBM bytes per step of bls_main_kernel (its two launch classes), by (curves, frequencies
+#: HBM bytes per step of bls_main_kernel (its launch classes), by (curves, frequencies
 #: per curve) of the survey batch: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of
 #: this script (--steps 1 --warmup 0 --no-cpu --no-learned --calibrate-copy), each counter
 #: scaled by what it reads for the plain 256 MiB copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
-MEASURED_TRAFFIC = {(4096, 200_000): 7.198e9 + 140.76e9}
+MEASURED_TRAFFIC = {(4096, 200_000): 9.507e9 + 141.75e9}
 
 
 def log(msg: str) -> None:
