@@ -311,6 +311,9 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
     int64_t* __restrict__ wide_scratch, double* __restrict__ out, int64_t n_out) {
     extern __shared__ int64_t dyn_lds[];
     constexpr int MODE = KMODE == 3 ? 0 : KMODE;
+    // bins per thread of the MODE 0 prefix: spans of up to 1,024 bins (KMODE 3) need
+    // 1,024/T of them, the wider ones kChunk
+    constexpr int CHUNK = KMODE == 3 ? (1024 / T < kChunk ? 1024 / T : kChunk) : kChunk;
     constexpr int NW = T / kWave;
     constexpr bool GLOBAL = MODE == 2;
     __shared__ int32_t s_dur[DMAX];
@@ -440,10 +443,10 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                 const int base = tid * c;
                 const longlong2* S2 = reinterpret_cast<const longlong2*>(S);
                 int64_t sy = 0, sw = 0;
-                int64_t ry[kChunk], rw[kChunk];
+                int64_t ry[CHUNK], rw[CHUNK];
                 if (MODE == 0) {
 #pragma unroll
-                    for (int kk = 0; kk < kChunk; ++kk) {
+                    for (int kk = 0; kk < CHUNK; ++kk) {
                         const int i = base + kk;
                         if (kk < c && i < span) {
                             const int src = (fused_wrap && i >= wrap_lo && i < n_bins)
@@ -483,7 +486,7 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                     int2* Hw = reinterpret_cast<int2*>(S);
                     uint2* Lw = reinterpret_cast<uint2*>(reinterpret_cast<char*>(S) + 8 * (size_t)wl);
 #pragma unroll
-                    for (int kk = 0; kk < kChunk; ++kk) {
+                    for (int kk = 0; kk < CHUNK; ++kk) {
                         const int i = base + kk;
                         if (kk < c && i < span) {
                             const uint64_t vy = (uint64_t)(ry[kk] + oy);
