@@ -523,28 +523,37 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
             int dd[DMAX];
             uint32_t unc_bits = 0;
             uint32_t dok = 0;  // durations with 1 <= d < n_bins (uniform)
+            int dmax_ok = 0;  // the longest duration in use (uniform)
 #pragma unroll
             for (int di = 0; di < DMAX; ++di) {
                 cm[di] = 0;
                 dd[di] = di < n_dur ? s_dur[di] : 0;
-                if (di < n_dur && dd[di] >= 1 && dd[di] < n_bins) dok |= 1u << di;
+                if (di < n_dur && dd[di] >= 1 && dd[di] < n_bins) {
+                    dok |= 1u << di;
+                    dmax_ok = max(dmax_ok, dd[di]);
+                }
             }
+            auto screen_box = [&](int n, int2 h0, int di, int d) {
+                bool cert, unc;
+                const float v = screen_value<OBJ>(h0, bv.hi(n + d), s_hi, e_unc, ybar, cert, unc);
+                cm[di] = max(cm[di], cert ? __float_as_int(v) : 0);
+                if (OBJ == 1 && unc) unc_bits |= 1u << di;
+            };
             // Start-outer, duration-inner: one read of H[n] per start, and the
-            // durations' boxes are independent work for the scheduler.
-            for (int n = tid; n < n_bins; n += T) {
-                const int2 h0 = bv.hi(n);
+            // durations' boxes are independent work for the scheduler. Rounds of starts
+            // whose every box ends within the bins skip the per-box bound check.
+            for (int n0 = 0; n0 < n_bins; n0 += T) {
+                const int n = n0 + tid;
+                if (n0 + T - 1 + dmax_ok <= n_bins) {
+                    const int2 h0 = bv.hi(n);
 #pragma unroll
-                for (int di = 0; di < DMAX; ++di) {
-                    if ((dok >> di) & 1u) {
-                        const int d = dd[di];
-                        if (n + d <= n_bins) {
-                            bool cert, unc;
-                            const float v = screen_value<OBJ>(h0, bv.hi(n + d), s_hi, e_unc,
-                                                              ybar, cert, unc);
-                            cm[di] = max(cm[di], cert ? __float_as_int(v) : 0);
-                            if (OBJ == 1 && unc) unc_bits |= 1u << di;
-                        }
-                    }
+                    for (int di = 0; di < DMAX; ++di)
+                        if ((dok >> di) & 1u) screen_box(n, h0, di, dd[di]);
+                } else if (n < n_bins) {
+                    const int2 h0 = bv.hi(n);
+#pragma unroll
+                    for (int di = 0; di < DMAX; ++di)
+                        if (((dok >> di) & 1u) && n + dd[di] <= n_bins) screen_box(n, h0, di, dd[di]);
                 }
             }
             int vmax = 0;
