@@ -1261,8 +1261,8 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     // reduced-precision kernel takes every item in one launch.)
     const int threads = ctx->threads;
     // Classes: 0 MODE 0 up to 1,024 bins (18 KiB of LDS; its own instantiation for 8
-    // waves per SIMD), 1-3 MODE 0 up to 1,280 / 1,536 / threads*8 bins (23, 28, 37 KiB:
-    // 6, 5, 4 workgroups per CU), 4 MODE 1, 5 MODE 2.
+    // waves per SIMD), 1-3 MODE 0 up to 1,408 / 1,728 / threads*8 bins (26, 31, 37 KiB:
+    // the most bins that leave room for 6, 5, 4 workgroups per CU), 4 MODE 1, 5 MODE 2.
     constexpr int kClasses = 6;
     auto layout_of = [&](int width) {
         if (ctx->accumulation != 0) return 0;
@@ -1270,7 +1270,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         const size_t need = (size_t)16 * (w + (w - 1) / 8 + 1);
         if (need + 2048 > (size_t)ctx->lds_limit) return 5;
         if (w > threads * kChunk) return 4;
-        return w <= 1024 ? 0 : (w <= 1280 ? 1 : (w <= 1536 ? 2 : 3));
+        return w <= 1024 ? 0 : (w <= 1408 ? 1 : (w <= 1728 ? 2 : 3));
     };
     constexpr int kModeOf[kClasses] = {3, 0, 0, 0, 1, 2};
     struct Layout {

@@ -283,9 +283,9 @@ def _bls_search_hip_specs(
 
 #: Phase-bin spans (bins per period, padded) where the engine's launch classes change:
 #: registers and LDS with 8 waves per SIMD (1,024), registers and LDS sized for 6, 5 and 4
-#: workgroups per CU (1,280, 1,536, 256 threads x 8 bins), LDS only (160 KiB); wider
+#: workgroups per CU (1,408, 1,728, 256 threads x 8 bins), LDS only (160 KiB); wider
 #: spans go to global memory.
-_SPAN_LIMITS = (1024, 1280, 1536, 2048, 8960)
+_SPAN_LIMITS = (1024, 1408, 1728, 2048, 8960)
 
 
 def _width_of(p_max: float, bin_duration: float, oversample: int) -> int:
