@@ -67,3 +67,50 @@ def test_gapped_curve_full_grid():
     r = np.nan_to_num(ref["power"], neginf=-1.0)
     j = int(np.argmax(r))
     assert sub["power"][j] == pytest.approx(r.max(), rel=1e-12)
+
+
+def test_long_baseline_curve_full_grid():
+    # One curve of the long-baseline batch at its full size: N = 100,000 over 10,000 d
+    # with times offset by 2.45e6 d, 4,000,000 linspace frequencies in [1e-4, 10] /d, 16
+    # period-scaled durations per log segment (phases up to 1e5 cycles). The whole grid
+    # runs on the engine; the injected 0.7 d transit is its best period, and 21 windows
+    # of the grid (20 seeded ones and the best period's neighbourhood) match the oracle.
+    from cuperiod_amd.batch import engine
+    from cuperiod_amd.core.config import BLSSettings
+    from cuperiod_amd.workloads import long_baseline_batch
+
+    settings = BLSSettings(n_durations=16)
+    wl = long_baseline_batch(n_curves=1)
+    t, y, dy = wl.curves[0]
+    full = engine._linspace_plan(0, wl.grids[0], settings)
+    pb = engine.pack_plans(engine.pack_curves(wl.curves), [full], settings)
+    res = engine.finish(pb, *engine.run_packed(pb), spectra=True)
+    spec = res.spectra[0]
+    periods = full.periods()
+    p = np.nan_to_num(spec["power"], neginf=-1.0)
+    assert np.all(np.isfinite(spec["power"]) | np.isneginf(spec["power"]))
+    k_best = int(np.argmax(p))
+    assert periods[k_best] == pytest.approx(0.7, rel=1e-3)
+    rng = np.random.default_rng(5)
+    starts = np.sort(rng.choice(periods.size - 60, 20, replace=False))
+    ranges = sorted({(int(a), int(a) + 50) for a in starts}
+                    | {(max(0, k_best - 100), min(periods.size, k_best + 100))})
+    merged: list[tuple[int, int]] = []
+    for a, b in ranges:
+        if merged and a <= merged[-1][1]:
+            merged[-1] = (merged[-1][0], max(merged[-1][1], b))
+        else:
+            merged.append((a, b))
+    plan = engine.window_plan(full, merged)
+    idx = np.concatenate([np.arange(a, b) for a, b in merged])
+    parts = []
+    for sg in plan.segments:
+        sub = engine.CurvePlan(segments=[sg], n_periods=sg.n_periods)
+        per = sub.periods_at(np.arange(sg.n_periods))
+        parts.append(oracle.bls_power(t, y, dy, per, sg.durations, settings.bins_per_duration))
+    ref = {k: np.concatenate([r[k] for r in parts]) for k in parts[0]}
+    np.testing.assert_array_equal(ref["period"], periods[idx])
+    got = {k: np.asarray(spec[k])[idx] for k in KEYS}
+    errs = compare(got, ref, name="long-baseline curve")
+    print(f"long-baseline curve: {periods.size} periods on the engine, {idx.size} checked, "
+          f"best {periods[k_best]:.6f} d, {errs}", flush=True)
