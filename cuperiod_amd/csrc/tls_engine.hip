@@ -136,15 +136,15 @@ __global__ __launch_bounds__(kTlsThreads) void tls_kernel(
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     if (g < ng) {
-                        // q within 2^-51.4 |q| of tau/P through the reciprocal; where q lies
-                        // within 2^-49 |q| of an integer, or ph*n_bins within
-                        // (2^-49 |q| + 2^-50) n_bins of one, the division decides instead
+                        // q within 2^-51.4 |q| of tau/P through the reciprocal; where
+                        // ph*n_bins lies within (2^-49 |q| + 2^-50) n_bins of an integer
+                        // (q near an integer puts it near 0 or n_bins), the division
+                        // decides instead
                         double qq = x * invPg[g];
                         double ph = qq - floor(qq);
                         double r = ph * dn;
                         const double tq = fabs(qq) * 0x1p-49;
-                        if (ph < tq || 1.0 - ph <= tq ||
-                            fabs(r - rint(r)) <= (tq + 0x1p-50) * dn) {
+                        if (fabs(r - rint(r)) <= (tq + 0x1p-50) * dn) {
                             qq = x / Pg[g];
                             ph = qq - floor(qq);
                             r = ph * dn;
