@@ -2,7 +2,7 @@
 // src/cuperiod/methods/tls.py:1-26; kernels :270-355 and :491-595).
 //
 // One workgroup takes one (light curve, tile of trial periods) work item and, within it,
-// G trial periods at a time (G = 4 while their bins fit a quarter of the LDS):
+// G trial periods at a time (G = 8 while their bins fit a quarter of the LDS):
 //
 //   (a) fold    every point's phase bin from the reference's FP64 expressions
 //               q = tau/P, ph = q - floor(q), bin = int(ph*n_bins) (clipped), bit for
@@ -312,9 +312,10 @@ extern "C" int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau,
     const size_t tmpl_bytes = (size_t)8 * tmpl_len;
     if (per_period + tmpl_bytes + 1024 > (size_t)ctx->lds_limit)
         return fail(ctx, CPB_E_RESOURCE, "phase bins and templates exceed the LDS");
-    // periods per pass: 4 (or 2) while their bins and the templates stay within a quarter
-    // (a half) of the LDS, so that several workgroups share a CU
-    const int n_pass = per_period * 4 + tmpl_bytes <= (size_t)ctx->lds_limit / 4   ? 4
+    // periods per pass: 8 or 4 (2) while their bins and the templates stay within a
+    // quarter (a half) of the LDS, so that several workgroups share a CU
+    const int n_pass = per_period * 8 + tmpl_bytes <= (size_t)ctx->lds_limit / 4   ? 8
+                       : per_period * 4 + tmpl_bytes <= (size_t)ctx->lds_limit / 4   ? 4
                        : per_period * 2 + tmpl_bytes <= (size_t)ctx->lds_limit / 2 ? 2
                                                                                    : 1;
     const size_t lds = per_period * n_pass + tmpl_bytes;
@@ -393,7 +394,9 @@ extern "C" int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau,
                            reinterpret_cast<const double*>(db + o_tm), n_widths, max_width,
                            tmpl_len, den_rel, out, n_out);
     };
-    if (n_pass == 4)
+    if (n_pass == 8)
+        launch(tls_kernel<8>);
+    else if (n_pass == 4)
         launch(tls_kernel<4>);
     else if (n_pass == 2)
         launch(tls_kernel<2>);
