@@ -54,11 +54,13 @@ __device__ __forceinline__ bool tls_better(double a, uint32_t ka, double b, uint
 // (-1 for a lane past the curve's end), (a, b) its values and (pa, pb) the wave's
 // inclusive prefix sums of them.
 __device__ __forceinline__ void fold_runs(int64_t* bins, int bb, int64_t a, int64_t b,
-                                          int64_t pa, int64_t pb, int lane) {
-    const int prev = __shfl_up(bb, 1, kWave);
-    const bool head = lane == 0 || bb != prev;
-    const unsigned long long heads = __ballot(head);
-    const bool tail = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
+                                          int64_t pa, int64_t pb) {
+    // the neighbours' bins by wavefront shifts (DPP wave_shr:1 / wave_shl:1); lane 0 and
+    // lane 63 get INT_MIN, which no bin equals, so they open and close a run
+    const int prev = __builtin_amdgcn_update_dpp(INT_MIN, bb, 0x138, 0xF, 0xF, false);
+    const int next = __builtin_amdgcn_update_dpp(INT_MIN, bb, 0x130, 0xF, 0xF, false);
+    const bool head = bb != prev;
+    const bool tail = bb != next;
     if (bb >= 0 && (head || tail)) {
         using u64 = unsigned long long;
         const u64 da = (tail ? (u64)pa : 0ull) - (head ? (u64)pa - (u64)a : 0ull);
@@ -151,8 +153,7 @@ __global__ __launch_bounds__(kTlsThreads) void tls_kernel(
                         }
                         int bb = (int)r;
                         bb = bb > n_bins - 1 ? n_bins - 1 : (bb < 0 ? 0 : bb);
-                        fold_runs(P2 + 2 * (int64_t)g * n_ext, valid ? bb : -1, a, b, pa, pb,
-                                  lane);
+                        fold_runs(P2 + 2 * (int64_t)g * n_ext, valid ? bb : -1, a, b, pa, pb);
                     }
                 }
             }
