@@ -819,13 +819,13 @@ def batch_periodograms(
 
     size = int(chunk_size) if chunk_size is not None else DEFAULT_CHUNK_SIZE
     chunks: dict[int, list[InputItem]] = {}
-    configs: dict[int, _ChunkConfig] = {}
+    chunk_cfgs: dict[int, _ChunkConfig] = {}
     idx = next_index
     for missing, group in groups.items():
         group_cfg = cfg if missing == methods else dataclasses.replace(cfg, methods=missing)
         for chunk in _chunked(group, size):
             chunks[idx] = chunk
-            configs[idx] = group_cfg
+            chunk_cfgs[idx] = group_cfg
             idx += 1
 
     rows_by_chunk: dict[int, list[dict[str, Any]]] = {}
@@ -844,7 +844,7 @@ def batch_periodograms(
 
     runner = _ChunkRunner(absorb)
     for i in sorted(chunks):
-        runner.submit(i, chunks[i], configs[i])
+        runner.submit(i, chunks[i], chunk_cfgs[i])
     runner.drain()
 
     all_rows = [row for i in sorted(rows_by_chunk) for row in rows_by_chunk[i]]

@@ -30,7 +30,7 @@ MAX_GRID_SIZE: int = 250_000_000
 #: Automatic grids above this size warn (and are still built).
 AUTO_GRID_WARN_SIZE: int = 5_000_000
 
-_BASELINE_OUTLIER_RATIO = 10.0
+_SPAN_OUTLIER_RATIO = 10.0
 _CHECK_CHUNK = 1 << 20
 
 
@@ -191,7 +191,7 @@ def check_grid_size(
     if time is not None and np.size(time) > 0:
         span, central, ratio = _baseline_outlier_ratio(time)
         baseline = span if baseline is None else baseline
-        if ratio > _BASELINE_OUTLIER_RATIO:
+        if ratio > _SPAN_OUTLIER_RATIO:
             outlier = (
                 f" The baseline ({span:.6g} d) is {ratio:.3g}x the span of the central "
                 f"98% of the samples ({central:.6g} d), so a few outlying epochs set "

@@ -4,7 +4,7 @@
  * every coarse frequency cell as the location of the full-grid BLS best period.
  * cuPeriod has no counterpart: the proposer narrows a batch's fine search to a few
  * frequency windows per curve, and the plain full-grid search (cpb_bls_search_batch)
- * stays its fallback and the judge of its recall.
+ * stays its fallback and the reference its recall is measured against.
  *
  * Network, per curve, on L = 4096 positions (ascending log frequency):
  *   h1 = GELU(conv7(x, W1) + b1)          x: 2 channels (FP32), W1: [64][7][2] FP32

@@ -1,7 +1,7 @@
 """Host logic of ``batch_periodograms`` on the CPU.
 
 The device search is replaced by the oracle (the spectra a correct engine returns), so
-these tests check what the driver adds around it: per-curve checks and errors, input
+these tests check what the batch runner adds around it: per-curve checks and errors, input
 order, chunking, rows, sinks, resume and the directory manifest
 (src/cuperiod/batch/runner.py:546-1032, src/cuperiod/batch/io.py:507-667).
 """
