@@ -240,3 +240,18 @@ def test_batch_periodograms_rows_equal_single_curve_rows():
         for k, v in want.items():
             same = (v == row[k]) or (isinstance(v, float) and np.isnan(v) and np.isnan(row[k]))
             assert same, (k, row[k], v)
+
+
+def test_dense_sampling_matches_oracle():
+    # 20,000 points over 27 d (the TLS survey's curves): at the longer periods a wave's 64
+    # consecutive points share a few of the 256 phase bins, so the kernel's run sums (one
+    # atomic pair per run of equal bins) carry most of the fold.
+    from cuperiod_amd.workloads import tls_survey
+
+    curves, _ = tls_survey(n_curves=1)
+    t, flux, err = curves[0]
+    settings = cup.TLSSettings()
+    grid = TLS.default_grid(cup.LightCurve.from_arrays(t, flux, err, domain="flux"), settings)
+    periods = np.ascontiguousarray(grid.period[::13])
+    _check(tls_power(t, flux, err, periods, settings=settings),
+           _oracle(t, flux, err, periods, settings))
