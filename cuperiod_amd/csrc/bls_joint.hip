@@ -401,23 +401,43 @@ __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, i
         }
     }
     __syncthreads();
-    for (int kk = tid; kk < gk; kk += T) {
-        longlong2* B = S2 + (int64_t)kk * wl;
-        for (int o = 0; o < oversample; ++o) {
-            int dst = nb - oversample + o;
-            dst = dst < 0 ? 0 : (dst > width - 1 ? width - 1 : dst);
-            B[dst] = B[1 + o];
+    // Wrap copy (bins nb-os+o take bin 1+o): when the copied and the overwritten bins are
+    // disjoint the prefix reads the source bin in place of each overwritten one;
+    // otherwise the copies run in order first (a copy may read a bin an earlier one wrote).
+    const int wrap_lo = nb - oversample;
+    const bool fused_wrap = wrap_lo > oversample && oversample <= kWave;
+    if (!fused_wrap) {
+        for (int kk = tid; kk < gk; kk += T) {
+            longlong2* B = S2 + (int64_t)kk * wl;
+            for (int o = 0; o < oversample; ++o) {
+                int dst = nb - oversample + o;
+                dst = dst < 0 ? 0 : (dst > width - 1 ? width - 1 : dst);
+                B[dst] = B[1 + o];
+            }
         }
+        __syncthreads();
     }
-    __syncthreads();
     for (int kk = wave; kk < gk; kk += NW) {
         longlong2* B = S2 + (int64_t)kk * wl;
         double2* D = reinterpret_cast<double2*>(B);
         int64_t cy = 0, cw = 0;
+        // the copy's sources (bins 1..os), held in lanes 0..os-1 before the scan
+        // overwrites them with prefix sums
+        int64_t sy = 0, sw = 0;
+        if (fused_wrap && lane < oversample) {
+            const longlong2 v = B[1 + lane];
+            sy = v.x;
+            sw = v.y;
+        }
         for (int base = 0; base <= nb; base += kWave) {
             const int i = base + lane;
             longlong2 v = make_longlong2(0, 0);
             if (i <= nb) v = B[i];
+            if (fused_wrap && base + kWave > wrap_lo) {
+                const int src = (i - wrap_lo) & (kWave - 1);
+                const int64_t wy = shfl_i64(sy, src), ww = shfl_i64(sw, src);
+                if (i >= wrap_lo && i < nb) v = make_longlong2(wy, ww);
+            }
             const int64_t y = wave_incl_scan_dpp(v.x) + cy;
             const int64_t w = wave_incl_scan_dpp(v.y) + cw;
             if (i <= nb) D[i] = make_double2((double)y, (double)w);
