@@ -65,27 +65,6 @@ __device__ __forceinline__ int64_t shfl_i64(int64_t v, int src) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-__device__ __forceinline__ int64_t shfl_up_i64(int64_t v, int delta) {
-    const uint32_t lo = __shfl_up((uint32_t)(uint64_t)v, delta, kWave);
-    const uint32_t hi = __shfl_up((uint32_t)((uint64_t)v >> 32), delta, kWave);
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-__device__ __forceinline__ int64_t wave_scan_incl_i64(int64_t v, int lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int64_t o = shfl_up_i64(v, off);
-        if (lane >= off) v += o;
-    }
-    return v;
-}
-
-__device__ __forceinline__ float wave_max_f32(float v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
-    return v;
-}
-
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -106,12 +85,6 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
         v += (int64_t)(((uint64_t)hi << 32) | lo);
     }
     return v;
-}
-
-// First-wins order of two box candidates: the larger objective, then the smaller
-// (duration index, start) key.
-__device__ __forceinline__ bool better(double a, uint32_t ka, double b, uint32_t kb) {
-    return a > b || (a == b && ka < kb);
 }
 
 // Phase bin of one point: `int(w/bd) + 1` with `w = x - P*floor(x/P)`, clipped to
