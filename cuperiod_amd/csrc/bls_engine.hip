@@ -324,6 +324,10 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
     __shared__ int s_red_v[NW];
     __shared__ BoxKey s_red_b[NW];
     __shared__ uint32_t s_red_k[NW];
+    // each period's winner: exact in-box sums and (duration index, start) key
+    __shared__ int64_t s_win_y[kMaxTile];
+    __shared__ int64_t s_win_w[kMaxTile];
+    __shared__ uint32_t s_win_k[kMaxTile];
 
     const int tid = threadIdx.x;
     const int lane = tid % kWave;
@@ -606,9 +610,9 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
             }
             __syncthreads();
 
-            // (c) finalize in FP64 from the winner's exact sums and write the seven
-            // outputs. Nothing below reads the bins, so the next period's fold can start
-            // while thread 0 finishes.
+            // the period's winner (first wins across the waves), kept for (c). Nothing
+            // below reads the bins, so the next period's fold can start while thread 0
+            // finishes.
             if (tid == 0) {
                 BoxKey b = s_red_b[0];
                 uint32_t bk = s_red_k[0];
@@ -619,38 +623,52 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                         bk = s_red_k[w];
                     }
                 }
-                if (!(b.num2 >= 0.0)) {
-                    out[col] = -INFINITY;
-                    for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
-                } else {
-                    const int n = (int)(bk & 0xFFFFFFu);
-                    const int d = s_dur[bk >> 24];
-                    const double iv_in = ldexp((double)b.win, cv.w_exp);
-                    const double iv_out = ldexp((double)(cv.w_tot - b.win), cv.w_exp);
-                    const double y_in = ldexp((double)b.yin, cv.y_exp);
-                    const double y_rest = ldexp((double)(cv.y_tot - b.yin), cv.y_exp);
-                    const double yin = y_in / iv_in;
-                    const double yout = y_rest / iv_out;
-                    const double depth = yout - yin;
-                    const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
-                    const double dsnr = depth / derr;
-                    const double ll = 0.5 * iv_in * depth * depth;
-                    const double dur = (double)d * bd;
-                    // (n*bd + dur/2) lies in [0, 2P): fmod by one exact subtraction.
-                    const double a = (double)n * bd + 0.5 * dur;
-                    const double ph = a < P ? a : (a < 2.0 * P ? a - P : fmod(a, P));
-                    out[col] = OBJ == 0 ? dsnr : ll;
-                    out[1 * n_out + col] = depth;
-                    out[2 * n_out + col] = derr;
-                    out[3 * n_out + col] = dsnr;
-                    out[4 * n_out + col] = dur;
-                    out[5 * n_out + col] = ph + cv.t_min;
-                    out[6 * n_out + col] = ll;
-                }
+                s_win_y[q] = b.yin;
+                s_win_w[q] = b.win;
+                s_win_k[q] = b.num2 >= 0.0 ? bk : 0xFFFFFFFFu;
             }
             if constexpr (GLOBAL) {
                 __threadfence();
                 __syncthreads();
+            }
+        }
+
+        // (c) finalize in FP64 from each winner's exact sums and write the seven outputs:
+        // one lane per period of the item (coalesced rows).
+        __syncthreads();
+        if (!(cv.flags & 2) && tid < it.count) {
+            const int q = tid;
+            const double P = s_per[q];
+            const int64_t col = sg.out_offset + it.begin + q;
+            const uint32_t bk = s_win_k[q];
+            if (bk == 0xFFFFFFFFu) {
+                out[col] = -INFINITY;
+                for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
+            } else {
+                const int64_t byin = s_win_y[q], bwin = s_win_w[q];
+                const int n = (int)(bk & 0xFFFFFFu);
+                const int d = s_dur[bk >> 24];
+                const double iv_in = ldexp((double)bwin, cv.w_exp);
+                const double iv_out = ldexp((double)(cv.w_tot - bwin), cv.w_exp);
+                const double y_in = ldexp((double)byin, cv.y_exp);
+                const double y_rest = ldexp((double)(cv.y_tot - byin), cv.y_exp);
+                const double yin = y_in / iv_in;
+                const double yout = y_rest / iv_out;
+                const double depth = yout - yin;
+                const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
+                const double dsnr = depth / derr;
+                const double ll = 0.5 * iv_in * depth * depth;
+                const double dur = (double)d * bd;
+                // (n*bd + dur/2) lies in [0, 2P): fmod by one exact subtraction.
+                const double a = (double)n * bd + 0.5 * dur;
+                const double ph = a < P ? a : (a < 2.0 * P ? a - P : fmod(a, P));
+                out[col] = OBJ == 0 ? dsnr : ll;
+                out[1 * n_out + col] = depth;
+                out[2 * n_out + col] = derr;
+                out[3 * n_out + col] = dsnr;
+                out[4 * n_out + col] = dur;
+                out[5 * n_out + col] = ph + cv.t_min;
+                out[6 * n_out + col] = ll;
             }
         }
     }
