@@ -317,6 +317,10 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
     constexpr int NW = T / kWave;
     constexpr bool GLOBAL = MODE == 2;
     __shared__ int32_t s_dur[DMAX];
+    // Per period of the work item: P, 1/P and the bin count, read at the start of the
+    // period; once it is done, the same slots hold its winner (in-box sums, key) until
+    // the item's outputs are written. No LDS is added for the winners, so each launch
+    // class keeps its workgroups per CU.
     __shared__ double s_per[kMaxTile];
     __shared__ double s_invp[kMaxTile];
     __shared__ int32_t s_nb[kMaxTile];
@@ -324,10 +328,6 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
     __shared__ int s_red_v[NW];
     __shared__ BoxKey s_red_b[NW];
     __shared__ uint32_t s_red_k[NW];
-    // each period's winner: exact in-box sums and (duration index, start) key
-    __shared__ int64_t s_win_y[kMaxTile];
-    __shared__ int64_t s_win_w[kMaxTile];
-    __shared__ uint32_t s_win_k[kMaxTile];
 
     const int tid = threadIdx.x;
     const int lane = tid % kWave;
@@ -623,9 +623,10 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                         bk = s_red_k[w];
                     }
                 }
-                s_win_y[q] = b.yin;
-                s_win_w[q] = b.win;
-                s_win_k[q] = b.num2 >= 0.0 ? bk : 0xFFFFFFFFu;
+                // every thread read slot q at the start of this period
+                s_per[q] = __longlong_as_double((long long)b.yin);
+                s_invp[q] = __longlong_as_double((long long)b.win);
+                s_nb[q] = (int32_t)(b.num2 >= 0.0 ? bk : 0xFFFFFFFFu);
             }
             if constexpr (GLOBAL) {
                 __threadfence();
@@ -638,14 +639,15 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
         __syncthreads();
         if (!(cv.flags & 2) && tid < it.count) {
             const int q = tid;
-            const double P = s_per[q];
+            const double P = segment_period(sg, periods, it.begin + q);
             const int64_t col = sg.out_offset + it.begin + q;
-            const uint32_t bk = s_win_k[q];
+            const uint32_t bk = (uint32_t)s_nb[q];
             if (bk == 0xFFFFFFFFu) {
                 out[col] = -INFINITY;
                 for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
             } else {
-                const int64_t byin = s_win_y[q], bwin = s_win_w[q];
+                const int64_t byin = (int64_t)__double_as_longlong(s_per[q]);
+                const int64_t bwin = (int64_t)__double_as_longlong(s_invp[q]);
                 const int n = (int)(bk & 0xFFFFFFu);
                 const int d = s_dur[bk >> 24];
                 const double iv_in = ldexp((double)bwin, cv.w_exp);
