@@ -113,3 +113,26 @@ def test_survey_best_periods_match_oracle_up_to_ties():
         else:
             assert abs(rpow[gi] - top) <= 1e-12 * abs(top)
     print(f"  identical best period on {exact}/8 curves", flush=True)
+
+
+def test_non_finite_curve_leaves_its_neighbours_alone():
+    # A curve with a non-finite flux value gets no box at any period (power -inf, the
+    # other outputs 0); the curves packed beside it in the same launches, whose work
+    # items finalize their periods together at the item's end, are unchanged.
+    print("case batch non-finite", flush=True)
+    settings = cup.BLSSettings(n_durations=6)
+    lcs = _curves(4, seed=3)
+    grids = [LinspaceGrid(1.0 / (t.max() - t.min()), 2.0, 3000) for t, _, _ in lcs]
+    clean = bls_batch(lcs, grids, settings, spectra=True)
+    t, y, dy = lcs[1]
+    y = y.copy()
+    y[len(y) // 2] = np.nan
+    res = bls_batch([lcs[0], (t, y, dy), *lcs[2:]], grids, settings, spectra=True)
+    for c in (0, 2, 3):
+        for k in KEYS:
+            np.testing.assert_array_equal(res.spectra[c][k], clean.spectra[c][k],
+                                          err_msg=f"{c}/{k}")
+    assert res.spectra[1]["power"].size == grids[1].as_gridspec().period.size
+    assert np.all(res.spectra[1]["power"] == -np.inf)
+    for k in KEYS[1:]:
+        assert np.all(res.spectra[1][k] == 0.0), k
