@@ -98,6 +98,10 @@ class _SegPlan:
     lin_num: int = 0
     lin_k0: int = 0
     periods: np.ndarray | None = None
+    # cuts of a parametric segment where its phase span crosses the engine's span limits,
+    # by oversample: they depend only on the fields above, so every batch that packs this
+    # segment (the plain search, a learned search's fallback) computes them once
+    cuts: dict = field(default_factory=dict, repr=False, compare=False)
 
 
 @dataclass
@@ -479,9 +483,12 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
                 parametric.append((len(pieces), c, sg))
                 pieces.append((c, sg, -1, -1, -1))  # filled in below
     if parametric:
-        cut = _parametric_width_ranges([sg for _, _, sg in parametric], oversample)
+        todo = [sg for _, _, sg in parametric if oversample not in sg.cuts]
+        if todo:
+            for sg, rng in zip(todo, _parametric_width_ranges(todo, oversample)):
+                sg.cuts[oversample] = rng
         expanded: list[tuple[int, _SegPlan, int, int, int]] = []
-        slots = {pos: (c, sg, rng) for (pos, c, sg), rng in zip(parametric, cut)}
+        slots = {pos: (c, sg, sg.cuts[oversample]) for pos, c, sg in parametric}
         for pos, piece in enumerate(pieces):
             if pos in slots:
                 c, sg, rng = slots[pos]
