@@ -11,6 +11,7 @@ copies only); their raw pointers and the current torch stream go through the ABI
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import threading
 from pathlib import Path
@@ -51,6 +52,7 @@ EXPORTED_SYMBOLS = (
     "cpb_set_timing",
     "cpb_main_kernel_ms",
     "cpb_set_accumulation",
+    "cpb_set_peak_rounding",
     "cpb_set_launch",
     "cpb_graph_create",
     "cpb_graph_launch",
@@ -85,7 +87,7 @@ class Segment(ctypes.Structure):
         ("n_dur", ctypes.c_int32),
         ("curve", ctypes.c_int32),
         ("mode", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("group", ctypes.c_int32),
         ("lin_start", ctypes.c_double),
         ("lin_step", ctypes.c_double),
         ("lin_stop", ctypes.c_double),
@@ -102,6 +104,8 @@ class Curve(ctypes.Structure):
         ("n_points", ctypes.c_int64),
         ("t_min", ctypes.c_double),
         ("ivar_floor", ctypes.c_double),
+        ("sum_y", ctypes.c_double),
+        ("sum_ivar", ctypes.c_double),
     ]
 
 
@@ -160,7 +164,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.cpb_status_string.argtypes = [ctypes.c_int]
     lib.cpb_bls_search.restype = ctypes.c_int
     lib.cpb_bls_search.argtypes = [
-        vp, vp, vp, vp, vp, i64, f64, f64, vp, vp,
+        vp, vp, vp, vp, vp, i64, f64, f64, f64, f64, vp, vp,
         ctypes.POINTER(Segment), i32, i32, i32, vp, i64,
     ]
     lib.cpb_bls_search_batch.restype = ctypes.c_int
@@ -190,6 +194,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.cpb_set_timing.argtypes = [vp, ctypes.c_int]
     lib.cpb_main_kernel_ms.restype = ctypes.c_int
     lib.cpb_main_kernel_ms.argtypes = [vp, ctypes.POINTER(f64)]
+    lib.cpb_set_peak_rounding.restype = ctypes.c_int
+    lib.cpb_set_peak_rounding.argtypes = [vp, ctypes.c_int]
     lib.cpb_set_launch.restype = ctypes.c_int
     lib.cpb_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.cpb_graph_create.restype = ctypes.c_int
@@ -300,6 +306,19 @@ class Context:
             return
         detail = self.lib.cpb_last_error(self.handle).decode(errors="replace")
         raise status_error(status, f"BLS {what}: {detail}")
+
+    @contextlib.contextmanager
+    def peak_rounding(self, enabled: bool):
+        """Within the block, searches on this context do (or do not) search each
+        segment's near-maximum periods again with the reference's FP64 sequence
+        (``cpb_set_peak_rounding``; on by default)."""
+        with self.lock:
+            self.check(self.lib.cpb_set_peak_rounding(self.handle, int(bool(enabled))),
+                       "peak rounding")
+            try:
+                yield self
+            finally:
+                self.check(self.lib.cpb_set_peak_rounding(self.handle, 1), "peak rounding")
 
     def close(self) -> None:
         if getattr(self, "handle", None):

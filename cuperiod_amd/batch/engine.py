@@ -98,6 +98,9 @@ class _SegPlan:
     lin_num: int = 0
     lin_k0: int = 0
     periods: np.ndarray | None = None
+    # the reference segment of these periods (the engine takes the band of its peak
+    # rounding per reference segment); -1: the segment's position in its plan
+    group: int = -1
     # cuts of a parametric segment where its phase span crosses the engine's span limits,
     # by oversample: they depend only on the fields above, so every batch that packs this
     # segment (the plain search, a learned search's fallback) computes them once
@@ -372,10 +375,15 @@ def pack_curves(lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]]) -> Cur
         cv.point_offset = off
         cv.n_points = int(t_h.size)
         cv.t_min = t_min[c]
-        cv.ivar_floor = core._ivar_floor(int(t_h.size), float(iv_h.sum()))
+        sum_ivar = float(iv_h.sum())
+        cv.ivar_floor = core._ivar_floor(int(t_h.size), sum_ivar)
+        # the reference's host sums (numpy), read by the reference rounding of the peak
+        cv.sum_y = float(yws[-1].sum())
+        cv.sum_ivar = sum_ivar
         c32 = curves32[c]
         c32.point_offset, c32.n_points, c32.t_min = cv.point_offset, cv.n_points, cv.t_min
-        c32.ivar_floor = core._ivar_floor(int(t_h.size), float(iv_h.sum()), float32=True)
+        c32.ivar_floor = core._ivar_floor(int(t_h.size), sum_ivar, float32=True)
+        c32.sum_y, c32.sum_ivar = cv.sum_y, cv.sum_ivar
         n_points[c] = t_h.size
         point_base[c] = off
         off += int(t_h.size)
@@ -470,8 +478,10 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
     # where the width crosses the engine's span limits (outputs do not depend on it)
     pieces: list[tuple[int, _SegPlan, int, int, int]] = []
     parametric: list[tuple[int, int, _SegPlan]] = []
+    group_of: dict[int, int] = {}
     for c, plan in enumerate(plans):
-        for sg in plan.segments:
+        for pos, sg in enumerate(plan.segments):
+            group_of[id(sg)] = sg.group if sg.group >= 0 else pos
             bd = float(sg.durations.min()) / oversample
             width = int(np.ceil(sg.p_max / bd)) + oversample + 1
             if width <= core._SPAN_LIMITS[0] or sg.n_periods < 2:
@@ -528,6 +538,7 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
         d.n_dur = int(db.size)
         d.curve = c
         d.mode = sg.mode
+        d.group = group_of[id(sg)]
         d.lin_start, d.lin_step, d.lin_stop = sg.lin_start, sg.lin_step, sg.lin_stop
         d.lin_num, d.lin_k0 = sg.lin_num, sg.lin_k0 + a
         if sg.mode == 0:
@@ -568,7 +579,7 @@ def window_plan(base: CurvePlan, ranges: Sequence[tuple[int, int]]) -> CurvePlan
     segment's durations and binning, so its outputs equal the full search's."""
     plan = CurvePlan()
     start = 0
-    for sg in base.segments:
+    for pos, sg in enumerate(base.segments):
         if sg.mode == 0:
             raise ValueError("window_plan needs parametric segments")
         lo, hi = start, start + sg.n_periods
@@ -579,7 +590,8 @@ def window_plan(base: CurvePlan, ranges: Sequence[tuple[int, int]]) -> CurvePlan
             plan.segments.append(
                 _SegPlan(curve=sg.curve, n_periods=b - a, durations=sg.durations, p_max=sg.p_max,
                          mode=sg.mode, lin_start=sg.lin_start, lin_step=sg.lin_step,
-                         lin_stop=sg.lin_stop, lin_num=sg.lin_num, lin_k0=sg.lin_k0 + (a - lo))
+                         lin_stop=sg.lin_stop, lin_num=sg.lin_num, lin_k0=sg.lin_k0 + (a - lo),
+                         group=sg.group if sg.group >= 0 else pos)
             )
             plan.n_periods += b - a
         start = hi

@@ -632,14 +632,18 @@ class _ChunkRunner:
 
     def submit(self, idx: int, items: Sequence[InputItem], cfg: _ChunkConfig) -> None:
         launches = []
-        errors: dict[int, tuple[str, str]] = {}
-        for name in cfg.methods:
+        # position -> (index of the method that failed, key, message). As the
+        # reference's chunk loop (src/cuperiod/batch/runner.py:220-249): a light curve's
+        # methods run in order, a failure keeps the rows of the methods before it and
+        # runs none after it.
+        failed: dict[int, tuple[int, str, str]] = {}
+        for mi, name in enumerate(cfg.methods):
             settings = cfg.settings_map[name]
             prepare = _prepare_tls if name == "TLS" else _prepare
             prepared: list[_Prepared] = []
             mb_rows: list[tuple[int, dict[str, Any]]] = []
             for pos, (key, lc) in enumerate(items):
-                if pos in errors:
+                if pos in failed:
                     continue
                 try:
                     if isinstance(lc, MultiBandLightCurve):
@@ -647,24 +651,29 @@ class _ChunkRunner:
                     else:
                         prepared.append(prepare(pos, key, lc, settings, cfg.grid))
                 except Exception as exc:  # noqa: BLE001 - one bad light curve
-                    errors[pos] = (key, f"{type(exc).__name__}: {exc}")
+                    failed[pos] = (mi, key, f"{type(exc).__name__}: {exc}")
             if name == "TLS":
                 handle = ("tls", _tls_rows(prepared, settings, cfg)) if prepared else None
             else:
                 handle = _launch(prepared, settings, cfg.grid) if prepared else None
-            launches.append((settings, prepared, handle))
+            launches.append((mi, settings, prepared, handle))
             if mb_rows:
-                launches.append((settings, [], ("rows", mb_rows)))
+                launches.append((mi, settings, [], ("rows", mb_rows)))
         self._complete()
-        self.pending = (idx, cfg, launches, errors)
+        self.pending = (idx, cfg, launches, failed)
 
     def _complete(self) -> None:
         if self.pending is None:
             return
-        idx, cfg, launches, errors = self.pending
+        idx, cfg, launches, failed = self.pending
         self.pending = None
-        by_pos: dict[int, list[dict[str, Any]]] = {}
-        for settings, prepared, handle in launches:
+
+        def runs(pos: int, mi: int) -> bool:
+            # method mi ran on this light curve: no method before or at it failed
+            return pos not in failed or mi < failed[pos][0]
+
+        by_pos: dict[int, dict[int, dict[str, Any]]] = {}
+        for mi, settings, prepared, handle in launches:
             if handle is None:
                 continue
             if isinstance(handle[0], str):
@@ -673,11 +682,11 @@ class _ChunkRunner:
                 pairs = (zip((p.position for p in prepared), handle[1])
                          if handle[0] == "tls" else handle[1])
                 for pos, row in pairs:
-                    if pos not in errors:
-                        by_pos.setdefault(pos, []).append(row)
+                    if runs(pos, mi):
+                        by_pos.setdefault(pos, {})[mi] = row
                 continue
             for prep, res in zip(prepared, _collect(handle, candidates=not cfg.store_raw)):
-                if prep.position in errors:
+                if not runs(prep.position, mi):
                     continue
                 try:
                     row = None
@@ -692,12 +701,12 @@ class _ChunkRunner:
                                                  store_raw=cfg.store_raw,
                                                  extra_keys=cfg.extra_keys, source=prep.label)
                 except Exception as exc:  # noqa: BLE001 - recorded per light curve
-                    errors[prep.position] = (prep.key, f"{type(exc).__name__}: {exc}")
-                    by_pos.pop(prep.position, None)
+                    failed[prep.position] = (mi, prep.key, f"{type(exc).__name__}: {exc}")
                     continue
-                by_pos.setdefault(prep.position, []).append(row)
-        rows = [r for pos in sorted(by_pos) if pos not in errors for r in by_pos[pos]]
-        errs = [errors[pos] for pos in sorted(errors)]
+                by_pos.setdefault(prep.position, {})[mi] = row
+        rows = [by_pos[pos][mi] for pos in sorted(by_pos) for mi in sorted(by_pos[pos])
+                if runs(pos, mi)]
+        errs = [(key, msg) for _, (_, key, msg) in sorted(failed.items())]
         self.absorb(idx, rows, errs)
 
     def drain(self) -> None:

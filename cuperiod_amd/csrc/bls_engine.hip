@@ -34,6 +34,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "cuperiod_bls.h"
@@ -1128,6 +1129,244 @@ __global__ void fill_empty_kernel(double* __restrict__ out, int64_t n_out, int64
     for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + off + i] = 0.0;
 }
 
+// ---------------------------------------------------------------------------------
+// Reference rounding of the near-maximum periods (cpb_set_peak_rounding).
+//
+// The main kernel's box sums are exact: boxes that hold the same points tie exactly and
+// the first one in (duration, start) order wins, and the adjacent periods of a
+// flat-topped peak, which fold the best box's points identically, have equal powers.
+// The reference's FP64 kernel orders the same ties by the rounding of its sequential
+// bin sums and prefix sums (src/cuperiod/methods/_bls_core.py:369-442). The periods that
+// can hold a curve's best power are those within kPeakBand of its largest power (the
+// reference's rounding moves a power by about 1e-13 relative); they are searched again
+// with the reference's FP64 sequence, so the best period and its box are the ones it
+// reports. The band is taken per reference segment (a group of the engine's segments),
+// so a segment's outputs do not depend on the other segments searched with it; every
+// period within kPeakBand of the curve's largest power is within it of its own
+// segment's largest.
+
+constexpr double kPeakBand = 0x1p-30;
+constexpr int kPeakCap = 64;       // re-searched periods per reference segment, at most
+constexpr int kPeakThreads = 256;
+
+// One workgroup per reference segment g (engine segments seg_idx[seg_start[g] ..
+// seg_start[g+1]), all of one curve): the largest sanitised power, then every period
+// within kPeakBand of it, as (engine segment, index in segment) pairs.
+__global__ __launch_bounds__(kPeakThreads) void peak_band_kernel(
+    const double* __restrict__ out, const cpb_segment* __restrict__ segs,
+    const int32_t* __restrict__ seg_start, const int32_t* __restrict__ seg_idx,
+    const cpb_curve* __restrict__ curves, int2* __restrict__ cand,
+    int32_t* __restrict__ count) {
+    __shared__ double red[kPeakThreads / kWave];
+    __shared__ int32_t s_n;
+    const int c = blockIdx.x;  // the group
+    const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+    const int s0 = seg_start[c], s1 = seg_start[c + 1];
+    double m = 0.0;
+    for (int s = s0; s < s1; ++s) {
+        const cpb_segment& sg = segs[seg_idx[s]];
+        for (int64_t k = tid; k < sg.n_periods; k += kPeakThreads)
+            m = fmax(m, sanitize(out[sg.out_offset + k]));
+    }
+    m = wave_max_f64(m);
+    if (lane == 0) red[wave] = m;
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    m = red[0];
+    for (int w = 1; w < kPeakThreads / kWave; ++w) m = fmax(m, red[w]);
+    const cpb_curve cv = curves[segs[seg_idx[s0]].curve];
+    if (m > 0.0 && cv.sum_ivar > 0.0 && isfinite(cv.sum_y) && isfinite(cv.sum_ivar)) {
+        const double thr = m - m * kPeakBand;
+        for (int s = s0; s < s1; ++s) {
+            const int si = seg_idx[s];
+            const cpb_segment& sg = segs[si];
+            for (int64_t k = tid; k < sg.n_periods; k += kPeakThreads) {
+                if (sanitize(out[sg.out_offset + k]) >= thr) {
+                    const int i = atomicAdd(&s_n, 1);
+                    if (i < kPeakCap) cand[(int64_t)c * kPeakCap + i] = make_int2(si, (int)k);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) count[c] = s_n;
+}
+
+// One wave per (reference segment, candidate): the reference's FP64 search of that
+// period.
+//   bins:   my[ind] += yw[j], mi[ind] += wv[j] over the points in order (:369-378), with
+//           ind = int((x - P*floor(x/P))/bd) + 1 clipped, by IEEE divisions; lanes compute
+//           the bins of 64 points at once, lane 0 adds them in point order, keeping a run
+//           of points in one bin in registers (the same additions in the same order);
+//   wrap:   bin[n_bins-os+k] = bin[1+k] in order (:379-383);
+//   prefix: sequential inclusive sums over [0, n_bins] (:384-390);
+//   boxes:  every (duration, start) box's objective with the reference's formulas and
+//           admissibility tests (:398-442; its screen skips only boxes that cannot win),
+//           lanes taking strided starts; the first strict maximum in (duration index,
+//           start) order wins;
+//   finish: the seven outputs (:443-461).
+// GLOBAL: the bins live in a per-workgroup slice of global memory instead of LDS.
+template <bool GLOBAL>
+__global__ __launch_bounds__(kWave) void peak_rounding_kernel(
+    const double* __restrict__ tau, const double* __restrict__ yw,
+    const double* __restrict__ wv, const cpb_curve* __restrict__ curves,
+    const cpb_segment* __restrict__ segs, const double* __restrict__ periods,
+    const int32_t* __restrict__ dur_bins, int32_t oversample, int32_t obj_flag,
+    int32_t n_groups, const int2* __restrict__ cand, const int32_t* __restrict__ count,
+    int32_t wl, double* __restrict__ wide, double* __restrict__ out, int64_t n_out) {
+    extern __shared__ double peak_lds[];
+    const int lane = threadIdx.x;
+    double* my = GLOBAL ? wide + (int64_t)blockIdx.x * 2 * wl : peak_lds;
+    double* mi = my + wl;
+    const int64_t total = (int64_t)n_groups * kPeakCap;
+    for (int64_t i = blockIdx.x; i < total; i += gridDim.x) {
+        // slot-major order: every group's first candidate, then every second one, ...
+        const int slot = (int)(i / n_groups);
+        const int g = (int)(i % n_groups);
+        if (slot >= min(count[g], kPeakCap)) continue;
+        const int2 ck = cand[(int64_t)g * kPeakCap + slot];
+        const cpb_segment sg = segs[ck.x];
+        const cpb_curve cv = curves[sg.curve];
+        const double P = segment_period(sg, periods, ck.y);
+        const double bd = sg.bin_duration;
+        const int width = sg.width;
+        // a candidate has an admissible box, so its period is finite and positive
+        if (!(P > 0.0 && P / bd < (double)width)) continue;
+        int64_t nb64 = (int64_t)ceil(P / bd) + oversample;
+        if (nb64 > width - 1) nb64 = width - 1;
+        const int nb = (int)nb64;
+        __syncthreads();  // the previous candidate's bins are no longer read
+        for (int b = lane; b <= nb; b += kWave) {
+            my[b] = 0.0;
+            mi[b] = 0.0;
+        }
+        if constexpr (GLOBAL) __threadfence();
+        __syncthreads();
+        const int64_t off = cv.point_offset;
+        const int64_t n = cv.n_points;
+        int cur = -1;
+        double acc_y = 0.0, acc_w = 0.0;
+        for (int64_t j0 = 0; j0 < n; j0 += kWave) {
+            const int64_t j = j0 + lane;
+            int ind = 0;
+            double a = 0.0, b = 0.0;
+            if (j < n) {
+                const double x = tau[off + j];
+                const double w = x - P * floor(x / P);
+                int64_t q = (int64_t)(w / bd) + 1;
+                q = q < 0 ? 0 : (q > width - 1 ? width - 1 : q);
+                ind = (int)q;
+                a = yw[off + j];
+                b = wv[off + j];
+            }
+            const int m = (int)min<int64_t>(kWave, n - j0);
+            for (int q = 0; q < m; ++q) {
+                const int iq = __builtin_amdgcn_readlane(ind, q);
+                const double aq = readlane_f64(a, q);
+                const double bq = readlane_f64(b, q);
+                if (lane == 0) {
+                    if (iq != cur) {
+                        if (cur >= 0) {
+                            my[cur] = acc_y;
+                            mi[cur] = acc_w;
+                        }
+                        cur = iq;
+                        acc_y = my[cur];
+                        acc_w = mi[cur];
+                    }
+                    acc_y += aq;
+                    acc_w += bq;
+                }
+            }
+        }
+        if (lane == 0) {
+            if (cur >= 0) {
+                my[cur] = acc_y;
+                mi[cur] = acc_w;
+            }
+            for (int k = 0; k < oversample; ++k) {
+                const int dst = nb - oversample + k;
+                if (dst >= 0 && dst < width) {
+                    my[dst] = my[1 + k];
+                    mi[dst] = mi[1 + k];
+                }
+            }
+            double ay = 0.0, ai = 0.0;
+            for (int b = 0; b <= nb; ++b) {
+                ay += my[b];
+                my[b] = ay;
+                ai += mi[b];
+                mi[b] = ai;
+            }
+        }
+        if constexpr (GLOBAL) __threadfence();
+        __syncthreads();
+        const double sum_y = cv.sum_y, sum_ivar = cv.sum_ivar, floor_iv = cv.ivar_floor;
+        double loc = -INFINITY;
+        uint32_t key = 0xFFFFFFFFu;
+        for (int di = 0; di < sg.n_dur; ++di) {
+            const int d = dur_bins[sg.dur_offset + di];
+            if (d < 1 || d >= nb) continue;
+            const int mbox = nb - d + 1;
+            for (int s = lane; s < mbox; s += kWave) {
+                const double y_in = my[s + d] - my[s];
+                const double iv_in = mi[s + d] - mi[s];
+                const double iv_out = sum_ivar - iv_in;
+                if (iv_in < floor_iv || iv_out < floor_iv) continue;
+                const double yin = y_in / iv_in;
+                const double yout = (sum_y - y_in) / iv_out;
+                if (yout < yin) continue;
+                const double depth = yout - yin;
+                const double obj = obj_flag == 0 ? depth / sqrt(1.0 / iv_in + 1.0 / iv_out)
+                                                 : 0.5 * iv_in * depth * depth;
+                if (obj > loc) {
+                    loc = obj;
+                    key = ((uint32_t)di << 24) | (uint32_t)s;
+                }
+            }
+        }
+        // first strict maximum: the largest objective, the smallest key among equals
+#pragma unroll
+        for (int offx = kWave / 2; offx > 0; offx >>= 1) {
+            const double ov = __shfl_xor(loc, offx, kWave);
+            const uint32_t ok = (uint32_t)__shfl_xor((int)key, offx, kWave);
+            if (ov > loc || (ov == loc && ok < key)) {
+                loc = ov;
+                key = ok;
+            }
+        }
+        if (lane == 0) {
+            const int64_t col = sg.out_offset + ck.y;
+            if (!(loc > -INFINITY)) {
+                out[col] = -INFINITY;
+                for (int o = 1; o < CPB_N_OUTPUTS; ++o) out[o * n_out + col] = 0.0;
+            } else {
+                const int ln = (int)(key & 0xFFFFFFu);
+                const int ld = dur_bins[sg.dur_offset + (int)(key >> 24)];
+                const double y_in = my[ln + ld] - my[ln];
+                const double iv_in = mi[ln + ld] - mi[ln];
+                const double iv_out = sum_ivar - iv_in;
+                const double yin = y_in / iv_in;
+                const double yout = (sum_y - y_in) / iv_out;
+                const double depth = yout - yin;
+                const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
+                const double dsnr = depth / derr;
+                const double ll = 0.5 * iv_in * depth * depth;
+                const double dur = (double)ld * bd;
+                // (ln*bd + dur/2) >= 0 and P > 0: Python's float % is fmod here
+                const double ph = fmod((double)ln * bd + 0.5 * dur, P);
+                out[col] = obj_flag == 0 ? dsnr : ll;
+                out[1 * n_out + col] = depth;
+                out[2 * n_out + col] = derr;
+                out[3 * n_out + col] = dsnr;
+                out[4 * n_out + col] = dur;
+                out[5 * n_out + col] = ph + cv.t_min;
+                out[6 * n_out + col] = ll;
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------
@@ -1206,6 +1445,47 @@ hipError_t dispatch_threads(int threads, int mode, int dmax, const LaunchArgs& a
 }
 
 constexpr int kMaxDur = kMaxDurations;
+
+// The reference rounding of every reference segment's near-maximum periods, after the
+// main kernel: one workgroup per reference segment selects them, one wave per period
+// searches it again.
+int launch_peak_rounding(cpb_context* ctx, hipStream_t st, const double* tau, const double* yw,
+                         const double* wv, const cpb_curve* curves, int32_t n_groups,
+                         const cpb_segment* segs, const int32_t* g_start,
+                         const int32_t* g_idx, const double* periods,
+                         const int32_t* dur_bins, int32_t oversample, int32_t obj_flag,
+                         int wmax, double* out, int64_t n_out) {
+    if (n_groups == 0) return CPB_OK;
+    const size_t cand_bytes = sizeof(int2) * (size_t)n_groups * kPeakCap;
+    int rc;
+    if ((rc = ensure(ctx, ctx->peak, cand_bytes + sizeof(int32_t) * n_groups))) return rc;
+    auto* cand = static_cast<int2*>(ctx->peak.ptr);
+    auto* count = reinterpret_cast<int32_t*>(static_cast<char*>(ctx->peak.ptr) + cand_bytes);
+    hipLaunchKernelGGL(peak_band_kernel, dim3(n_groups), dim3(kPeakThreads), 0, st, out, segs,
+                       g_start, g_idx, curves, cand, count);
+    CPB_HIP(ctx, hipGetLastError());
+    const int wl = wmax;  // n_bins <= width - 1 < wmax
+    const size_t bins_bytes = (size_t)16 * wl;
+    const int64_t slots = (int64_t)n_groups * kPeakCap;
+    if (bins_bytes <= 65536) {
+        const int64_t grid = std::min<int64_t>(slots, (int64_t)ctx->n_cu * 8);
+        hipLaunchKernelGGL(peak_rounding_kernel<false>, dim3((unsigned)grid), dim3(kWave),
+                           bins_bytes, st, tau, yw, wv, curves, segs, periods, dur_bins,
+                           oversample, obj_flag, n_groups, cand, count, wl, nullptr, out,
+                           n_out);
+    } else {
+        // bins in global memory: at most 1 GiB of them
+        const int64_t fit = std::max<int64_t>(1, ((int64_t)1 << 30) / (int64_t)bins_bytes);
+        const int64_t grid = std::min<int64_t>({slots, (int64_t)ctx->n_cu * 2, fit});
+        if ((rc = ensure(ctx, ctx->peak_wide, (size_t)grid * bins_bytes))) return rc;
+        hipLaunchKernelGGL(peak_rounding_kernel<true>, dim3((unsigned)grid), dim3(kWave), 0, st,
+                           tau, yw, wv, curves, segs, periods, dur_bins, oversample, obj_flag,
+                           n_groups, cand, count, wl,
+                           static_cast<double*>(ctx->peak_wide.ptr), out, n_out);
+    }
+    CPB_HIP(ctx, hipGetLastError());
+    return CPB_OK;
+}
 
 int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double* yw,
                const double* wv, int64_t total_points, const cpb_curve* curves,
@@ -1325,11 +1605,37 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     }
     const int64_t n_items = (int64_t)ctx->h_items.size();
 
-    // Descriptor tables: [curves | segments | work items], 256-byte aligned parts.
+    // The searched segments of each reference segment (equal curve and group), in
+    // order of first appearance, for the reference rounding of its peak (CSR).
+    std::vector<int32_t> cs_start(1, 0), cs_idx;
+    {
+        std::vector<std::pair<int64_t, int32_t>> keyed;
+        keyed.reserve(n_segments);
+        for (int32_t s = 0; s < n_segments; ++s)
+            if (segments[s].n_periods > 0 && segments[s].n_dur > 0)
+                keyed.emplace_back(((int64_t)segments[s].curve << 32) |
+                                       (int64_t)(uint32_t)segments[s].group, s);
+        std::stable_sort(keyed.begin(), keyed.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (size_t i = 0; i < keyed.size(); ++i) {
+            if (i > 0 && keyed[i].first != keyed[i - 1].first)
+                cs_start.push_back((int32_t)i);
+            cs_idx.push_back(keyed[i].second);
+        }
+        cs_start.push_back((int32_t)keyed.size());
+        if (keyed.empty()) cs_start.assign(1, 0);
+    }
+    const int32_t n_groups = (int32_t)cs_start.size() - 1;
+    if (cs_idx.empty()) cs_idx.push_back(0);
+
+    // Descriptor tables: [curves | segments | work items | curve segment lists],
+    // 256-byte aligned parts.
     auto align = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t off_seg = align(sizeof(cpb_curve) * n_curves);
     const size_t off_item = off_seg + align(sizeof(cpb_segment) * std::max(1, n_segments));
-    const size_t tbytes = off_item + align(sizeof(WorkItem) * std::max<int64_t>(1, n_items));
+    const size_t off_cs = off_item + align(sizeof(WorkItem) * std::max<int64_t>(1, n_items));
+    const size_t off_ci = off_cs + align(sizeof(int32_t) * cs_start.size());
+    const size_t tbytes = off_ci + align(sizeof(int32_t) * cs_idx.size());
     if ((rc = ensure(ctx, ctx->tables, tbytes))) return rc;
     if (ctx->copy_pending) {
         CPB_HIP(ctx, hipEventSynchronize(ctx->copied));
@@ -1347,6 +1653,8 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     if (n_segments > 0) std::memcpy(hb + off_seg, segments, sizeof(cpb_segment) * n_segments);
     if (n_items > 0)
         std::memcpy(hb + off_item, ctx->h_items.data(), sizeof(WorkItem) * n_items);
+    std::memcpy(hb + off_cs, cs_start.data(), sizeof(int32_t) * cs_start.size());
+    std::memcpy(hb + off_ci, cs_idx.data(), sizeof(int32_t) * cs_idx.size());
     CPB_HIP(ctx, hipMemcpyAsync(ctx->tables.ptr, hb, tbytes, hipMemcpyHostToDevice, st));
     if (!ctx->capturing) {
         CPB_HIP(ctx, hipEventRecord(ctx->copied, st));
@@ -1356,6 +1664,8 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     const auto* curve_in = reinterpret_cast<const cpb_curve*>(db);
     const auto* sp = reinterpret_cast<const cpb_segment*>(db + off_seg);
     const auto* ip = reinterpret_cast<const WorkItem*>(db + off_item);
+    const auto* csp = reinterpret_cast<const int32_t*>(db + off_cs);
+    const auto* cip = reinterpret_cast<const int32_t*>(db + off_ci);
 
     hipLaunchKernelGGL(prep_curves_kernel, dim3(n_curves), dim3(kPrepThreads), 0, st, yw,
                        wv, curve_in, obj_flag,
@@ -1453,6 +1763,9 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
         ctx->timed = true;
     }
+    if (ctx->peak_rounding)
+        return launch_peak_rounding(ctx, st, tau, yw, wv, curve_in, n_groups, sp, csp, cip,
+                                    periods, dur_bins, oversample, obj_flag, wmax, out, n_out);
     return CPB_OK;
 }
 
@@ -1499,7 +1812,8 @@ void cpb_destroy(cpb_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->copy_pending) (void)hipEventSynchronize(ctx->copied);
-    for (DevBuf* b : {&ctx->tables, &ctx->curves, &ctx->yq, &ctx->wq, &ctx->wide})
+    for (DevBuf* b : {&ctx->tables, &ctx->curves, &ctx->yq, &ctx->wq, &ctx->wide, &ctx->peak,
+                      &ctx->peak_wide})
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->h_tables) (void)hipHostFree(ctx->h_tables);
     if (ctx->copied) (void)hipEventDestroy(ctx->copied);
@@ -1525,7 +1839,7 @@ const char* cpb_status_string(int status) {
 
 int cpb_bls_search(cpb_context* ctx, void* stream, const double* tau, const double* yw,
                    const double* wv, int64_t n_points, double t_min, double ivar_floor,
-                   const double* periods, const int32_t* dur_bins,
+                   double sum_y, double sum_ivar, const double* periods, const int32_t* dur_bins,
                    const cpb_segment* segments, int32_t n_segments, int32_t oversample,
                    int32_t obj_flag, double* out, int64_t n_out) {
     if (!ctx) return CPB_E_INVALID;
@@ -1537,6 +1851,8 @@ int cpb_bls_search(cpb_context* ctx, void* stream, const double* tau, const doub
     c.n_points = n_points;
     c.t_min = t_min;
     c.ivar_floor = ivar_floor;
+    c.sum_y = sum_y;
+    c.sum_ivar = sum_ivar;
     return run_search(ctx, static_cast<hipStream_t>(stream), tau, yw, wv, n_points, &c, 1,
                       periods, dur_bins, segments, n_segments, oversample, obj_flag, out,
                       n_out);
@@ -1670,6 +1986,7 @@ int cpb_graph_create(cpb_context* ctx, const double* tau, const double* yw, cons
     g->priv->threads = ctx->threads;
     g->priv->tile = ctx->tile;
     g->priv->grid_per_cu = ctx->grid_per_cu;
+    g->priv->peak_rounding = ctx->peak_rounding;
     g->priv->capturing = true;
     auto bail = [&](int status, const std::string& msg) {
         std::string m = msg;
@@ -1716,6 +2033,13 @@ int cpb_graph_launch(cpb_context* ctx, cpb_graph* g, void* stream) {
         CPB_HIP(ctx, hipEventRecord(ctx->t1, st));
         ctx->timed = true;
     }
+    return CPB_OK;
+}
+
+int cpb_set_peak_rounding(cpb_context* ctx, int enable) {
+    if (!ctx) return CPB_E_INVALID;
+    if (enable != 0 && enable != 1) return fail(ctx, CPB_E_INVALID, "peak rounding must be 0 or 1");
+    ctx->peak_rounding = enable != 0;
     return CPB_OK;
 }
 

@@ -215,6 +215,11 @@ struct cpb_context {
     // Joint multi-band search (cpb_set_joint_kernel): 0 automatic, 1 every band of a
     // period at once, 2 band groups with register-resident box totals.
     int joint_kernel = 0;
+    // Reference rounding of each curve's near-maximum periods (cpb_set_peak_rounding):
+    // `peak` holds the per-curve candidate lists and counts, `peak_wide` the bins of
+    // periods too wide for LDS.
+    bool peak_rounding = true;
+    cpb_engine::DevBuf peak, peak_wide;
 };
 
 namespace cpb_engine {

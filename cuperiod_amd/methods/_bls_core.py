@@ -258,6 +258,7 @@ def _bls_search_hip_specs(
         sg.n_dur = int(dur_all[k].shape[0])
         sg.curve = 0
         sg.mode = 0
+        sg.group = k
     if int(max(spec.width for spec in specs)) > (1 << 30):
         raise BackendUnavailableError("BLS: phase-bin count above 2^30 per period")
     out = torch.empty((_lib.N_OUTPUTS, total), dtype=torch.float64, device=device)
@@ -266,7 +267,8 @@ def _bls_search_hip_specs(
         ctx.check(ctx.lib.cpb_set_accumulation(ctx.handle, int(accumulation)), "accumulation")
         status = ctx.lib.cpb_bls_search(
             ctx.handle, _stream_handle(device), _ptr(data["tau"]), _ptr(data["yw"]),
-            _ptr(data["ivar"]), data["n"], data["t_min"], floor, _ptr(periods_d),
+            _ptr(data["ivar"]), data["n"], data["t_min"], floor, data["sum_y"],
+            data["sum_ivar"], _ptr(periods_d),
             _ptr(dur_d), segs, len(pieces), int(oversample), int(obj_flag), _ptr(out),
             total,
         )

@@ -49,7 +49,9 @@ typedef struct cpb_segment {
     int32_t n_dur;          /* box widths (bins) of this segment */
     int32_t curve;          /* light-curve index (0 for single-curve calls) */
     int32_t mode;           /* 0: explicit; 1: linspace; 2: arange (frequencies) */
-    int32_t reserved;
+    int32_t group;          /* the reference segment this block belongs to: blocks of one
+                               curve with equal `group` share the band of
+                               cpb_set_peak_rounding */
     double lin_start;       /* modes 1-2: first frequency */
     double lin_step;        /* modes 1-2: frequency step as numpy applies it */
     double lin_stop;        /* mode 1: last frequency (np.linspace sets it exactly) */
@@ -63,6 +65,8 @@ typedef struct cpb_curve {
     int64_t n_points;
     double t_min;           /* time origin subtracted from tau (added to transit_time) */
     double ivar_floor;      /* `_ivar_floor(N, sum(ivar))` (src/cuperiod/methods/_bls_core.py:75-94) */
+    double sum_y;           /* sum(y*ivar) and sum(ivar) as the reference's host forms them */
+    double sum_ivar;        /*   (numpy sums, src/cuperiod/methods/_bls_core.py:483-484) */
 } cpb_curve;
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -91,9 +95,12 @@ const char* cpb_status_string(int status);
  * `bls_block` kernel it launches per segment (:505-656): the same inputs (tau = t - t_min,
  * yw = y*ivar, wv = ivar; the segment binning; oversample; objective; ivar floor) and
  * the same seven outputs per period. Results are exact-arithmetic box statistics
- * (64-bit fixed-point phase bins), rounded once to float64.
+ * (64-bit fixed-point phase bins), rounded once to float64, except at the periods whose
+ * power lies within 2^-30 of the curve's largest (see cpb_set_peak_rounding).
  *
  * tau, yw, wv   device float64[n_points]
+ * sum_y, sum_ivar  the reference's host sums of yw and wv (`sum_y`, `sum_ivar` of the
+ *               kernel's arguments, :514)
  * periods       device float64[...], explicit periods of mode-0 segments (may be NULL)
  * dur_bins      device int32[...], box widths of every segment, concatenated
  * segments      HOST array of n_segments descriptors (curve must be 0)
@@ -102,7 +109,7 @@ const char* cpb_status_string(int status);
  */
 int cpb_bls_search(cpb_context* ctx, void* stream, const double* tau, const double* yw,
                    const double* wv, int64_t n_points, double t_min, double ivar_floor,
-                   const double* periods, const int32_t* dur_bins,
+                   double sum_y, double sum_ivar, const double* periods, const int32_t* dur_bins,
                    const cpb_segment* segments, int32_t n_segments, int32_t oversample,
                    int32_t obj_flag, double* out, int64_t n_out);
 
@@ -255,6 +262,21 @@ int cpb_main_kernel_ms(cpb_context* ctx, double* ms);
  * REAL=float). Modes 1 and 2 evaluate every box in FP32, serve the precision study, and
  * need the phase bins in LDS (CPB_E_RESOURCE otherwise). */
 int cpb_set_accumulation(cpb_context* ctx, int mode);
+
+/* Reference rounding of the near-maximum periods of later searches on `ctx` (1, the
+ * default) or none (0). The engine's box sums are exact, so boxes holding the same points
+ * tie exactly and the first one wins; the reference's FP64 kernel orders such ties (the
+ * samples of a flat-topped peak, boxes shifted across empty bins) by the rounding of its
+ * sequential bin and prefix sums (src/cuperiod/methods/_bls_core.py:369-442). With 1,
+ * every period whose power lies within 2^-30 (relative) of its segment's largest power is
+ * searched again with that FP64 sequence -- bins summed in point order, the wrap copy,
+ * sequential prefix sums, every box's objective, the first strict maximum -- and its
+ * seven outputs are the reference's. The band is taken per reference segment (the
+ * segments of a curve with equal `group`), so a segment's outputs do not depend on the
+ * other segments searched with it, and it holds every period within 2^-30 of the curve's
+ * largest power. Applies to the exact (accumulation 0) searches; at most 64 periods per
+ * reference segment are searched again. */
+int cpb_set_peak_rounding(cpb_context* ctx, int enable);
 
 /* Launch shape of later searches on `ctx` (chosen by the autotuner,
  * cuperiod_amd/batch/autotune.py): `threads` per workgroup (128, 256 or 512), `tile`

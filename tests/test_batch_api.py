@@ -59,13 +59,13 @@ def fake_engine(monkeypatch):
     return calls
 
 
-def _expected_row(key, lc, n_best=10, settings=None):
+def _expected_row(key, lc, n_best=10, settings=None, methods=("BLS",)):
     settings = settings or BLSSettings()
     prep = runner._prepare(0, key, lc, settings, None)
     (periods, spectrum), = _oracle_spectra([prep], settings, None)
     pg = runner._periodogram(prep, periods, spectrum, settings)
     return bio.periodogram_to_row(key, pg, n_best=n_best, store_raw=False,
-                                  extra_keys=bio.peak_extra_columns(["BLS"]), source="")
+                                  extra_keys=bio.peak_extra_columns(list(methods)), source="")
 
 
 def _same(a, b):
@@ -95,6 +95,43 @@ def test_rows_errors_and_order(fake_engine):
     assert {f"{e}_10" for e in ("period", "power", "depth", "depth_snr", "duration",
                                 "t0", "sde")} <= set(row)
     assert abs(row["best_period"] - 2.3) < 0.02
+
+
+def test_later_method_failure_keeps_earlier_rows(fake_engine, monkeypatch):
+    # The reference's chunk loop (src/cuperiod/batch/runner.py:220-249) runs a light
+    # curve's methods in order and appends each row as it is made: when a later method
+    # fails, the earlier methods' rows of that curve stay; the error is recorded once.
+    real_prepare_tls = runner._prepare_tls
+
+    def prepare_tls(position, key, lc, settings, grid):
+        if key == "b":
+            raise ValueError("TLS refuses this light curve")
+        return real_prepare_tls(position, key, lc, settings, grid)
+
+    def tls_rows(prepared, settings, cfg):
+        return [{"key": p.key, "method": "TLS"} for p in prepared]
+
+    monkeypatch.setattr(runner, "_prepare_tls", prepare_tls)
+    monkeypatch.setattr(runner, "_tls_rows", tls_rows)
+    lcs = [("a", _curve(1)), ("b", _curve(2)), ("c", _curve(3))]
+    res = runner.batch_periodograms(lcs, ["BLS", "TLS"], chunk_size=3)
+    assert [(r["key"], r["method"]) for r in res.rows] == [
+        ("a", "BLS"), ("a", "TLS"), ("b", "BLS"), ("c", "BLS"), ("c", "TLS")]
+    assert res.errors == [("b", "ValueError: TLS refuses this light curve")]
+    _same(res.rows[2], _expected_row("b", lcs[1][1], methods=("BLS", "TLS")))
+
+
+def test_earlier_method_failure_runs_no_later_method(fake_engine, monkeypatch):
+    # A failure of the first method stops the curve: no row from any method.
+    def tls_rows(prepared, settings, cfg):
+        return [{"key": p.key, "method": "TLS"} for p in prepared]
+
+    monkeypatch.setattr(runner, "_tls_rows", tls_rows)
+    lcs = [("a", _curve(1)), ("b", _curve(2, n=3)), ("c", _curve(3))]
+    res = runner.batch_periodograms(lcs, ["BLS", "TLS"], chunk_size=3)
+    assert [(r["key"], r["method"]) for r in res.rows] == [
+        ("a", "BLS"), ("a", "TLS"), ("c", "BLS"), ("c", "TLS")]
+    assert [k for k, _ in res.errors] == ["b"]
 
 
 def test_keys_from_meta_and_renamed(fake_engine):

@@ -123,7 +123,9 @@ def test_best_periods_match_oracle():
         k = int(np.argmin(np.abs(periods - row["best_period"])))
         assert periods[k] == row["best_period"]
         power = np.nan_to_num(power, nan=0.0, posinf=0.0, neginf=0.0)
-        assert abs(power[k] - power.max()) <= 1e-12 * abs(power.max()), key
+        # the oracle's own maximum, bit for bit (no tie tolerance)
+        assert power[k] == power.max(), key
+        assert np.count_nonzero(power == power.max()) == 1, key
 
 
 def test_parquet_sink_resume(tmp_path):

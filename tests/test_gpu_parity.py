@@ -17,7 +17,7 @@ import oracle
 from cuperiod_amd.methods import _bls_core as core
 from cuperiod_amd.methods.bls import BLSMethod, _grid_segments
 import cuperiod_amd as cup
-from parity import KEYS, compare
+from parity import KEYS, BoxInputs, compare
 from synth import adversarial_cases, synthetic_eclipser, transit_curve, two_runs
 
 pytestmark = pytest.mark.gpu
@@ -41,7 +41,8 @@ def test_eclipser_matches_oracle(objective):
     durations = np.array([0.05, 0.1, 0.2])
     got = _hip_power(t, flux, err, periods, durations, 10, objective)
     ref = oracle.bls_power(t, flux, err, periods, durations, 10, objective=objective)
-    errs = compare(got, ref, name=f"eclipser/{objective}")
+    errs = compare(got, ref, name=f"eclipser/{objective}",
+                   inputs=BoxInputs.for_specs(t, [(periods, durations)], 10))
     _say(f"  errors {errs}")
     assert errs["power"] < 1e-9
 
@@ -60,10 +61,10 @@ def test_adversarial_cases_match_oracle(obj_flag):
             got = _hip_power(t, y, dy, periods, durations, 3, objective)
         # Exact-tie data (constant / quantised flux, five points in 800 bins) has
         # boxes holding identical point sets at most periods; the oracle orders them
-        # by FP64 rounding, the engine's exact sums by first-wins. compare() still
-        # requires every differing box to tie in power.
-        box_frac = 0.0 if name in ("quantised ties", "constant", "five points") else 0.995
-        errs = compare(got, ref, name=name, box_frac=box_frac)
+        # by FP64 rounding, the engine's exact sums by first-wins. compare() proves
+        # that every differing box holds the same points as the oracle's.
+        errs = compare(got, ref, name=name,
+                       inputs=BoxInputs.for_specs(t, [(periods, durations)], 3))
         _say(f"  errors {errs}")
 
 
@@ -81,9 +82,9 @@ def test_default_grid_periodogram_matches_oracle():
     ref_freq = 1.0 / np.concatenate([p["period"] for p in parts])
     order = np.argsort(ref_freq, kind="stable")
     np.testing.assert_allclose(pg.power, ref_power[order], rtol=1e-9, atol=1e-12)
-    assert pg.best_period() == pytest.approx(
-        float(1.0 / ref_freq[order][np.argmax(ref_power[order])]), rel=1e-12
-    )
+    # identical best period (the segments' near-maximum periods carry the reference's
+    # FP64 rounding on the engine too)
+    assert pg.best_period() == float(1.0 / ref_freq[order][np.argmax(ref_power[order])])
     _say(f"  best period {pg.best_period()}")
 
 
@@ -132,7 +133,8 @@ def test_wide_histogram_path_matches_oracle():
     durations = np.array([0.05, 0.1, 0.3])
     got = _hip_power(t, flux, err, periods, durations, 10)
     ref = oracle.bls_power(t, flux, err, periods, durations, 10)
-    errs = compare(got, ref, name="wide")
+    errs = compare(got, ref, name="wide",
+                   inputs=BoxInputs.for_specs(t, [(periods, durations)], 10))
     _say(f"  errors {errs}")
 
 
@@ -150,7 +152,8 @@ def test_mid_span_histogram_path_matches_oracle():
     for objective in ("snr", "likelihood"):
         got = _hip_power(t, flux, err, periods, durations, 10, objective)
         ref = oracle.bls_power(t, flux, err, periods, durations, 10, objective=objective)
-        errs = compare(got, ref, name=f"mid-span/{objective}")
+        errs = compare(got, ref, name=f"mid-span/{objective}",
+                       inputs=BoxInputs.for_specs(t, [(periods, durations)], 10))
         _say(f"  errors {errs}")
 
 
@@ -172,7 +175,8 @@ def test_skewed_flux_and_wide_boxes_match_oracle(objective):
     durations = np.array([0.2, 0.6, 1.2, 2.0])
     got = _hip_power(t, flux, err, periods, durations, 5, objective)
     ref = oracle.bls_power(t, flux, err, periods, durations, 5, objective=objective)
-    errs = compare(got, ref, name=f"skewed/{objective}")
+    errs = compare(got, ref, name=f"skewed/{objective}",
+                   inputs=BoxInputs.for_specs(t, [(periods, durations)], 5))
     _say(f"  errors {errs}")
 
 

@@ -71,7 +71,8 @@ def test_library_host_entry_points():
 def test_segment_struct_layout_matches_header():
     assert ctypes.sizeof(_lib.Segment) == 96
     assert _lib.Segment.lin_start.offset == 56
-    assert ctypes.sizeof(_lib.Curve) == 32
+    assert ctypes.sizeof(_lib.Curve) == 48
+    assert _lib.Curve.sum_ivar.offset == 40
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
@@ -88,8 +89,9 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         "int main(void) {\n"
         '  printf("%zu %zu %zu %zu %zu\\n", sizeof(cpb_segment), sizeof(cpb_curve),\n'
         "         sizeof(cpb_band), sizeof(cpb_star), sizeof(cpb_tls_curve));\n"
-        '  printf("%zu %zu %zu\\n", offsetof(cpb_segment, lin_start),\n'
-        "         offsetof(cpb_star, t_min), offsetof(cpb_tls_curve, w_total));\n"
+        '  printf("%zu %zu %zu %zu\\n", offsetof(cpb_segment, lin_start),\n'
+        "         offsetof(cpb_star, t_min), offsetof(cpb_tls_curve, w_total),\n"
+        "         offsetof(cpb_curve, sum_y));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "layout"
     subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
@@ -99,7 +101,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         ctypes.sizeof(_lib.Segment), ctypes.sizeof(_lib.Curve), ctypes.sizeof(_lib.Band),
         ctypes.sizeof(_lib.Star), ctypes.sizeof(_lib.TlsCurve)]
     assert [int(v) for v in offsets.split()] == [
-        _lib.Segment.lin_start.offset, _lib.Star.t_min.offset, _lib.TlsCurve.w_total.offset]
+        _lib.Segment.lin_start.offset, _lib.Star.t_min.offset, _lib.TlsCurve.w_total.offset,
+        _lib.Curve.sum_y.offset]
 
 
 @pytest.mark.parametrize("start,stop,num", [(1 / 50, 2.0, 200000), (0.001, 5.0, 1000000),
