@@ -43,6 +43,7 @@ EXPORTED_SYMBOLS = (
     "cpb_status_string",
     "cpb_bls_search",
     "cpb_bls_search_batch",
+    "cpb_bls_search_batch_static",
     "cpb_bls_summarize",
     "cpb_peak_candidates",
     "cpb_bls_joint_search",
@@ -172,6 +173,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         vp, vp, vp, vp, vp, i64, ctypes.POINTER(Curve), i32, vp, vp,
         ctypes.POINTER(Segment), i32, i32, i32, vp, i64,
     ]
+    lib.cpb_bls_search_batch_static.restype = ctypes.c_int
+    lib.cpb_bls_search_batch_static.argtypes = lib.cpb_bls_search_batch.argtypes
     lib.cpb_bls_summarize.restype = ctypes.c_int
     lib.cpb_bls_summarize.argtypes = [vp, vp, vp, vp, vp, i32, vp]
     lib.cpb_peak_candidates.restype = ctypes.c_int

@@ -1458,9 +1458,11 @@ int launch_peak_rounding(cpb_context* ctx, hipStream_t st, const double* tau, co
     if (n_groups == 0) return CPB_OK;
     const size_t cand_bytes = sizeof(int2) * (size_t)n_groups * kPeakCap;
     int rc;
-    if ((rc = ensure(ctx, ctx->peak, cand_bytes + sizeof(int32_t) * n_groups))) return rc;
-    auto* cand = static_cast<int2*>(ctx->peak.ptr);
-    auto* count = reinterpret_cast<int32_t*>(static_cast<char*>(ctx->peak.ptr) + cand_bytes);
+    void* peak_buf = nullptr;
+    if ((rc = device_buffer(ctx, ctx->peak, cand_bytes + sizeof(int32_t) * n_groups, &peak_buf)))
+        return rc;
+    auto* cand = static_cast<int2*>(peak_buf);
+    auto* count = reinterpret_cast<int32_t*>(static_cast<char*>(peak_buf) + cand_bytes);
     hipLaunchKernelGGL(peak_band_kernel, dim3(n_groups), dim3(kPeakThreads), 0, st, out, segs,
                        g_start, g_idx, curves, cand, count);
     CPB_HIP(ctx, hipGetLastError());
@@ -1477,11 +1479,13 @@ int launch_peak_rounding(cpb_context* ctx, hipStream_t st, const double* tau, co
         // bins in global memory: at most 1 GiB of them
         const int64_t fit = std::max<int64_t>(1, ((int64_t)1 << 30) / (int64_t)bins_bytes);
         const int64_t grid = std::min<int64_t>({slots, (int64_t)ctx->n_cu * 2, fit});
-        if ((rc = ensure(ctx, ctx->peak_wide, (size_t)grid * bins_bytes))) return rc;
+        void* wide_bins = nullptr;
+        if ((rc = device_buffer(ctx, ctx->peak_wide, (size_t)grid * bins_bytes, &wide_bins)))
+            return rc;
         hipLaunchKernelGGL(peak_rounding_kernel<true>, dim3((unsigned)grid), dim3(kWave), 0, st,
                            tau, yw, wv, curves, segs, periods, dur_bins, oversample, obj_flag,
                            n_groups, cand, count, wl,
-                           static_cast<double*>(ctx->peak_wide.ptr), out, n_out);
+                           static_cast<double*>(wide_bins), out, n_out);
     }
     CPB_HIP(ctx, hipGetLastError());
     return CPB_OK;
@@ -1541,9 +1545,15 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
 
     CPB_HIP(ctx, hipSetDevice(ctx->device));
     int rc;
-    if ((rc = ensure(ctx, ctx->curves, sizeof(CurveParams) * n_curves))) return rc;
-    if ((rc = ensure(ctx, ctx->yq, sizeof(int64_t) * std::max<int64_t>(1, total_points)))) return rc;
-    if ((rc = ensure(ctx, ctx->wq, sizeof(int64_t) * std::max<int64_t>(1, total_points)))) return rc;
+    void *curves_buf = nullptr, *yq_buf = nullptr, *wq_buf = nullptr;
+    const size_t point_bytes = sizeof(int64_t) * std::max<int64_t>(1, total_points);
+    if ((rc = device_buffer(ctx, ctx->curves, sizeof(CurveParams) * n_curves, &curves_buf)))
+        return rc;
+    if ((rc = device_buffer(ctx, ctx->yq, point_bytes, &yq_buf))) return rc;
+    if ((rc = device_buffer(ctx, ctx->wq, point_bytes, &wq_buf))) return rc;
+    auto* cparams = static_cast<CurveParams*>(curves_buf);
+    auto* yq_d = static_cast<int64_t*>(yq_buf);
+    auto* wq_d = static_cast<int64_t*>(wq_buf);
 
     // Segments with no periods or no box widths: power -inf, the rest 0 (the
     // reference's `_empty_output`, src/cuperiod/methods/_bls_core.py:843-847).
@@ -1636,31 +1646,39 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     const size_t off_cs = off_item + align(sizeof(WorkItem) * std::max<int64_t>(1, n_items));
     const size_t off_ci = off_cs + align(sizeof(int32_t) * cs_start.size());
     const size_t tbytes = off_ci + align(sizeof(int32_t) * cs_idx.size());
-    if ((rc = ensure(ctx, ctx->tables, tbytes))) return rc;
+    void* tables_buf = nullptr;
+    if ((rc = device_buffer(ctx, ctx->tables, tbytes, &tables_buf))) return rc;
     if (ctx->copy_pending) {
         CPB_HIP(ctx, hipEventSynchronize(ctx->copied));
         ctx->copy_pending = false;
     }
-    if (ctx->h_tables_bytes < tbytes) {
-        if (ctx->h_tables) CPB_HIP(ctx, hipHostFree(ctx->h_tables));
-        ctx->h_tables = nullptr;
-        ctx->h_tables_bytes = 0;
-        CPB_HIP(ctx, hipHostMalloc(&ctx->h_tables, tbytes + tbytes / 4, hipHostMallocDefault));
-        ctx->h_tables_bytes = tbytes + tbytes / 4;
+    char* hb;
+    if (ctx->carve_base) {
+        // pageable staging: the copy below has read it when it returns
+        if (ctx->h_stage.size() < tbytes) ctx->h_stage.resize(tbytes);
+        hb = ctx->h_stage.data();
+    } else {
+        if (ctx->h_tables_bytes < tbytes) {
+            if (ctx->h_tables) CPB_HIP(ctx, hipHostFree(ctx->h_tables));
+            ctx->h_tables = nullptr;
+            ctx->h_tables_bytes = 0;
+            CPB_HIP(ctx, hipHostMalloc(&ctx->h_tables, tbytes + tbytes / 4, hipHostMallocDefault));
+            ctx->h_tables_bytes = tbytes + tbytes / 4;
+        }
+        hb = static_cast<char*>(ctx->h_tables);
     }
-    char* hb = static_cast<char*>(ctx->h_tables);
     std::memcpy(hb, curves, sizeof(cpb_curve) * n_curves);
     if (n_segments > 0) std::memcpy(hb + off_seg, segments, sizeof(cpb_segment) * n_segments);
     if (n_items > 0)
         std::memcpy(hb + off_item, ctx->h_items.data(), sizeof(WorkItem) * n_items);
     std::memcpy(hb + off_cs, cs_start.data(), sizeof(int32_t) * cs_start.size());
     std::memcpy(hb + off_ci, cs_idx.data(), sizeof(int32_t) * cs_idx.size());
-    CPB_HIP(ctx, hipMemcpyAsync(ctx->tables.ptr, hb, tbytes, hipMemcpyHostToDevice, st));
-    if (!ctx->capturing) {
+    CPB_HIP(ctx, hipMemcpyAsync(tables_buf, hb, tbytes, hipMemcpyHostToDevice, st));
+    if (!ctx->capturing && !ctx->carve_base) {
         CPB_HIP(ctx, hipEventRecord(ctx->copied, st));
         ctx->copy_pending = true;
     }
-    char* db = static_cast<char*>(ctx->tables.ptr);
+    char* db = static_cast<char*>(tables_buf);
     const auto* curve_in = reinterpret_cast<const cpb_curve*>(db);
     const auto* sp = reinterpret_cast<const cpb_segment*>(db + off_seg);
     const auto* ip = reinterpret_cast<const WorkItem*>(db + off_item);
@@ -1669,8 +1687,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
 
     hipLaunchKernelGGL(prep_curves_kernel, dim3(n_curves), dim3(kPrepThreads), 0, st, yw,
                        wv, curve_in, obj_flag,
-                       static_cast<CurveParams*>(ctx->curves.ptr),
-                       static_cast<int64_t*>(ctx->yq.ptr), static_cast<int64_t*>(ctx->wq.ptr));
+                       cparams, yq_d, wq_d);
     CPB_HIP(ctx, hipGetLastError());
 
     for (const cpb_segment& sg : empties) {
@@ -1691,7 +1708,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         // a launch (and a graph kernel node) holds fewer than 2^32 work-items; the
         // kernel loops over the items its grid does not cover
         const dim3 grid((unsigned)std::min<int64_t>(n_items, ((int64_t)1 << 31) / 256));
-        const auto* cp = static_cast<const CurveParams*>(ctx->curves.ptr);
+        const auto* cp = cparams;
         const bool timed = ctx->timing && !ctx->capturing;
         if (timed) CPB_HIP(ctx, hipEventRecord(ctx->t0, st));
         auto launch = [&](auto fn) {
@@ -1730,8 +1747,9 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         if (mode == 2) {
             const int64_t g = std::min<int64_t>(
                 lay.count, persistent > 0 ? persistent : (int64_t)ctx->n_cu * 4);
-            if ((rc = ensure(ctx, ctx->wide, (size_t)g * lds_need))) return rc;
-            wide_ptr = static_cast<int64_t*>(ctx->wide.ptr);
+            void* wide_buf = nullptr;
+            if ((rc = device_buffer(ctx, ctx->wide, (size_t)g * lds_need, &wide_buf))) return rc;
+            wide_ptr = static_cast<int64_t*>(wide_buf);
             la.grid = dim3((unsigned)g);
         } else {
             // fewer than 2^32 work-items per launch (a graph kernel node refuses more);
@@ -1742,9 +1760,9 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         }
         la.st = st;
         la.tau = tau;
-        la.yq = static_cast<const int64_t*>(ctx->yq.ptr);
-        la.wq = static_cast<const int64_t*>(ctx->wq.ptr);
-        la.curves = static_cast<const CurveParams*>(ctx->curves.ptr);
+        la.yq = yq_d;
+        la.wq = wq_d;
+        la.curves = cparams;
         la.segs = sp;
         la.items = ip + lay.begin;
         la.n_items = lay.count;
@@ -1767,6 +1785,43 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         return launch_peak_rounding(ctx, st, tau, yw, wv, curve_in, n_groups, sp, csp, cip,
                                     periods, dur_bins, oversample, obj_flag, wmax, out, n_out);
     return CPB_OK;
+}
+
+// Scratch of cpb_bls_search_batch_static, for callers that cannot allocate device
+// memory: the fixed-point points, curve parameters, descriptor tables and the peak
+// rounding's candidate lists of one search are carved from it. Every GPU the library
+// runs on holds its own copy until the process ends. Host code never passes the array's
+// name as an address: each device's copy is found once with hipGetSymbolAddress (with
+// that device current) and that pointer is what kernels and copies receive. Calls that
+// overlap on one GPU share the array and race. Nothing in it needs to start at zero:
+// every part is written by the call before it is read.
+constexpr size_t kStaticScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_static_mu;
+void* g_static_ptr[kMaxStaticDevices] = {};
+hipError_t g_static_err[kMaxStaticDevices] = {};
+bool g_static_done[kMaxStaticDevices] = {};
+
+// This device's copy of `scratch` (the device must be current); resolved once per device.
+void* static_scratch_address(int device, std::string& error) {
+    if (device < 0 || device >= kMaxStaticDevices) {
+        error = "device index beyond the static scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (!g_static_done[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_static_err[device] = e;
+        g_static_ptr[device] = e == hipSuccess ? p : nullptr;
+        g_static_done[device] = true;
+    }
+    if (!g_static_ptr[device])
+        error = std::string("hipGetSymbolAddress(scratch): ") +
+                hipGetErrorString(g_static_err[device]);
+    return g_static_ptr[device];
 }
 
 }  // namespace
@@ -1804,6 +1859,7 @@ int cpb_create(int device, cpb_context** out) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) ==
             hipSuccess && cus > 0)
         ctx->n_cu = cus;
+    ctx->static_scratch = static_scratch_address(device, ctx->static_error);
     *out = ctx;
     return CPB_OK;
 }
@@ -1868,6 +1924,27 @@ int cpb_bls_search_batch(cpb_context* ctx, void* stream, const double* tau,
     return run_search(ctx, static_cast<hipStream_t>(stream), tau, yw, wv, total_points,
                       curves, n_curves, periods, dur_bins, segments, n_segments,
                       oversample, obj_flag, out, n_out);
+}
+
+int cpb_bls_search_batch_static(cpb_context* ctx, void* stream, const double* tau,
+                                const double* yw, const double* wv, int64_t total_points,
+                                const cpb_curve* curves, int32_t n_curves,
+                                const double* periods, const int32_t* dur_bins,
+                                const cpb_segment* segments, int32_t n_segments,
+                                int32_t oversample, int32_t obj_flag, double* out,
+                                int64_t n_out) {
+    if (!ctx) return CPB_E_INVALID;
+    if (!ctx->static_scratch)
+        return fail(ctx, CPB_E_DEVICE, "static scratch unavailable: " + ctx->static_error);
+    ctx->carve_base = static_cast<char*>(ctx->static_scratch);
+    ctx->carve_cap = kStaticScratchBytes;
+    ctx->carve_used = 0;
+    const int rc = run_search(ctx, static_cast<hipStream_t>(stream), tau, yw, wv,
+                              total_points, curves, n_curves, periods, dur_bins, segments,
+                              n_segments, oversample, obj_flag, out, n_out);
+    ctx->carve_base = nullptr;
+    ctx->carve_cap = ctx->carve_used = 0;
+    return rc;
 }
 
 int cpb_bls_summarize(cpb_context* ctx, void* stream, const double* power,

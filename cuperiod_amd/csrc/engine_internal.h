@@ -220,6 +220,16 @@ struct cpb_context {
     // periods too wide for LDS.
     bool peak_rounding = true;
     cpb_engine::DevBuf peak, peak_wide;
+    // While cpb_bls_search_batch_static runs: device buffers are carved from the
+    // library's static scratch array at `carve_base` (carve_cap bytes) instead of being
+    // allocated, and descriptors are staged in pageable host memory (`h_stage`).
+    char* carve_base = nullptr;
+    size_t carve_cap = 0, carve_used = 0;
+    std::vector<char> h_stage;
+    // This device's copy of the static scratch array (hipGetSymbolAddress at creation;
+    // NULL, with the reason in `static_error`, when it could not be resolved).
+    void* static_scratch = nullptr;
+    std::string static_error;
 };
 
 namespace cpb_engine {
@@ -256,6 +266,25 @@ inline int ensure(cpb_context* ctx, DevBuf& b, size_t bytes) {
     b.ptr = p;
     b.bytes = want;
     return CPB_OK;
+}
+
+// A device buffer of at least `bytes` for this call: carved from the static scratch
+// (256-byte aligned) while a static search runs, else `b`, grown with hipMalloc.
+inline int device_buffer(cpb_context* ctx, DevBuf& b, size_t bytes, void** out) {
+    if (ctx->carve_base) {
+        const size_t off = (ctx->carve_used + 255) / 256 * 256;
+        if (off + bytes > ctx->carve_cap)
+            return fail(ctx, CPB_E_RESOURCE,
+                        "the search needs more scratch than the static scratch array holds "
+                        "(" + std::to_string(ctx->carve_cap >> 20) + " MiB); use the "
+                        "allocating entry point");
+        *out = ctx->carve_base + off;
+        ctx->carve_used = off + bytes;
+        return CPB_OK;
+    }
+    const int rc = ensure(ctx, b, bytes);
+    if (rc == CPB_OK) *out = b.ptr;
+    return rc;
 }
 
 // Fixed-point conversion of every curve's points and its parameters (one workgroup per

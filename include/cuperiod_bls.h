@@ -128,6 +128,27 @@ int cpb_bls_search_batch(cpb_context* ctx, void* stream, const double* tau,
                          int64_t n_out);
 
 /*
+ * cpb_bls_search_batch without allocating: no hipMalloc, hipHostMalloc or hipFree (nor
+ * any other allocating HIP routine) in the call. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation). Same arguments, kernels and
+ * outputs. The search's scratch (fixed-point copies of the points, 16 bytes per point;
+ * curve parameters; descriptor tables; the peak rounding's candidate lists) is carved
+ * from a static device array of 64 MiB per GPU, held until the process ends; a search
+ * that needs more (about 4 million points, or phase spans kept in global memory)
+ * returns CPB_E_RESOURCE and should use cpb_bls_search_batch. Descriptors are copied
+ * from pageable host memory. Every call on one GPU shares the array: calls that overlap
+ * on one GPU (other streams, other contexts, other threads) race.
+ */
+int cpb_bls_search_batch_static(cpb_context* ctx, void* stream, const double* tau,
+                                const double* yw, const double* wv, int64_t total_points,
+                                const cpb_curve* curves, int32_t n_curves,
+                                const double* periods, const int32_t* dur_bins,
+                                const cpb_segment* segments, int32_t n_segments,
+                                int32_t oversample, int32_t obj_flag, double* out,
+                                int64_t n_out);
+
+/*
  * Per-curve summary of power rows left on the device by a batched search (the SDE
  * statistics and best period of `_assemble` / `Periodogram.best_period`,
  * src/cuperiod/methods/bls.py:274-301, src/cuperiod/core/result.py:285-438):
