@@ -1,30 +1,40 @@
-"""Benchmark: batched plain BLS on one MI355X (and the CPU oracle beside it).
+"""Benchmark: batched plain BLS on one MI355X, with the oracle and the other legs beside it.
 
-Workload (default): the 4,096-curve survey batch -- N log-uniform in [100, 5,000],
-baselines 50-2,000 d, 200,000 frequencies per curve in [1/baseline, 2] /d, 6
-period-scaled durations per log segment (``cuperiod_amd.workloads.survey_batch``).
+Headline workload (the survey batch, ``cuperiod_amd.workloads.survey_batch``): 4,096
+curves, N log-uniform in [100, 5,000], baselines 50-2,000 d, 200,000 linspace frequencies
+per curve in [1/baseline, 2] /d, 6 period-scaled durations per log segment. A step is one
+replay of the captured batched search (descriptor upload, curve preparation, the main
+kernel's launch classes, the reference rounding of each segment's peak, per-curve
+summaries) with the inputs resident in HBM. W untimed warm-up steps, then K timed steps
+between two device synchronisations; ``value`` = curves/s. One JSON line is printed.
 
-A step is one batched search of every curve (fold, box scan, argmax, per-curve
-summary with the best period) with the inputs already resident in HBM. W untimed warm-up
-steps, then K timed steps between two device synchronisations. One JSON line is printed.
+``roofline``: the main kernel runs on the vector ALUs (no MFMA, HBM far from busy), so the
+bound is ``valu``: algorithmic FLOP per launch (4 FP64 per point-fold, 8 FP32 per box
+evaluation) over its mean HIP-event duration, against the vector FMA peak (78.6 TFLOP/s:
+256 CUs x 4 SIMDs x 16 lanes x 2 FLOP x 2.4 GHz, the published FP64 vector peak, at which
+non-packed FP32 also issues). ``issue`` adds the measured VALU instruction rate against
+the issue limit (256 x 4 x 16 lanes x 2.4 GHz), from rocprofv3 SQ_INSTS_VALU counts of the
+same workload.
 
-``roofline``: the main kernel's algorithmic FLOP (4 FP64 per point-fold, 8 FP32 per box
-evaluation) per launch over its mean HIP-event duration, against the 157.3 TFLOP/s FP32
-peak; ``detail`` adds point-folds/s, box-evaluations/s and the LDS bytes they imply
-(16 B each) against 256 CUs x 128 B/clk x 2.4 GHz.
-
-``cpu_baseline``: the FP64 C++ oracle (the reference's numba kernel restated, with its
-screen) on the first ``--cpu-curves`` curves of the same batch, all host threads.
-
-``next_rows``: the joint-ephemeris multi-band search (``multiband_survey``: 6 bands x 150
-points over 1,095 d per star, the default BLS grid) and the TLS matched filter
-(``tls_survey``: 20,000 points over 27 d per curve, the default TLS grid), each timed over
-one batched launch and beside its own CPU oracle.
+Legs beside the headline (each skippable):
+* ``parity``: the first ``--cpu-curves`` curves (64) on the engine and the FP64 oracle at
+  full grids: max relative power error, best periods identical, and every differing best
+  period proven an exact equal-point tie of the oracle's own maximum;
+* ``cpu_baseline``: the oracle's time on that sample (all host threads), in curves/s,
+  point-folds/s and box-evaluations/s, extrapolated to the whole batch; plus a
+  calibration run on the published single-curve shape against the published numba rate;
+* ``end_to_end``: ``batch_periodograms`` on the whole batch (host checks, packing,
+  uploads, search, peak candidates, rows) in curves/s;
+* ``long_baseline``: the long-baseline batch (256 curves x 100,000 points x 4,000,000
+  frequencies, 16 durations) timed on a seeded subset and extrapolated in sum(N x F), with
+  the precision modes' errors against the oracle;
+* ``next_rows``: joint multi-band BLS and TLS, each one batched launch beside its oracle.
 """
 
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -38,15 +48,19 @@ if ROOT not in sys.path:
 
 METRIC = ("light curves/s, 200k-freq BLS, 1 GPU: learned+refine vs plain, best-period "
           "match %")
-PEAK_FP32 = 157.3e12
+PEAK_VALU_FLOPS = 78.6e12
+VALU_ISSUE = 256 * 4 * 16 * 2.4e9
 PEAK_LDS = 256 * 128 * 2.4e9
-PEAK_HBM = 8.0e12
 COPY_BYTES = 256 << 20
-#: HBM bytes per step of bls_main_kernel (its launch classes), by (curves, frequencies
-#: per curve) of the survey batch: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of
-#: this script (--steps 1 --warmup 0 --no-cpu --no-learned --calibrate-copy), each counter
-#: scaled by what it reads for the plain 256 MiB copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).
-MEASURED_TRAFFIC = {(4096, 200_000): 7.539e9 + 58.69e9}
+#: Published single-curve numba BLS rate on 16 cores / 32 threads (891 points x 76,124
+#: periods in 39.5 ms, cuPeriod benchmarks/REPORT.md:206).
+PUBLISHED_NUMBA_FOLDS_PER_S = 891 * 76124 / 0.039467
+PUBLISHED_NUMBA_CORES = 16
+#: Per step of the survey batch (4,096 curves x 200,000 frequencies): HBM bytes of
+#: bls_main_kernel (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a plain copy's
+#: reading) and its VALU wave instructions (--pmc SQ_INSTS_VALU), summed over the launch
+#: classes; the commands are in profiles/bls_main_kernel_survey_batch.md.
+MEASURED = {(4096, 200_000): {"traffic": 66.23e9, "valu_wave_instr": 3.17e12}}
 
 
 def log(msg: str) -> None:
@@ -54,24 +68,14 @@ def log(msg: str) -> None:
 
 
 def count_work(plans, n_points, oversample) -> tuple[float, float]:
-    """Exact point-folds and box evaluations of one launch over a batch."""
+    """Exact point-folds and box evaluations of a search over ``plans``."""
     from cuperiod_amd.methods._bls_core import _duration_bins
 
-    folds = 0.0
-    boxes = 0.0
+    folds = boxes = 0.0
     for plan, n in zip(plans, n_points):
         for sg in plan.segments:
-            if sg.mode == 0:
-                periods = sg.periods
-            else:
-                k = np.arange(sg.n_periods, dtype=np.int64)
-                i = sg.lin_num - 1 - (sg.lin_k0 + k)
-                f = i.astype(np.float64) * sg.lin_step + sg.lin_start
-                if sg.mode == 1:
-                    f = np.where(i == sg.lin_num - 1, sg.lin_stop, f)
-                else:
-                    f = np.where(i == 0, sg.lin_start, f)
-                periods = 1.0 / f
+            sub = type(plan)(segments=[sg], n_periods=sg.n_periods)
+            periods = sub.periods()
             bd = float(sg.durations.min()) / oversample
             width = int(np.ceil(sg.p_max / bd)) + oversample + 1
             nb = np.minimum(np.ceil(periods / bd).astype(np.int64) + oversample, width - 1)
@@ -82,94 +86,280 @@ def count_work(plans, n_points, oversample) -> tuple[float, float]:
     return folds, boxes
 
 
-def cpu_baseline(wl, settings, n_curves: int):
-    """The oracle on the first ``n_curves`` curves (same grids, same settings)."""
-    import oracle
-    from cuperiod_amd.methods.bls import _grid_segments
-
-    threads = oracle.default_threads()
-    jobs = []
-    for c in range(n_curves):
-        t, y, dy = wl.curves[c]
-        segs = _grid_segments(wl.grids[c].as_gridspec(), settings)
-        jobs.append((t, y, dy, segs))
-    oracle.bls_power(jobs[0][0][:50], jobs[0][1][:50], jobs[0][2][:50],
-                     np.array([1.0, 2.0]), np.array([0.1]), 10)  # load + warm
-    results = []
-    t0 = time.perf_counter()
-    for t, y, dy, segs in jobs:
-        parts = [oracle.bls_power(t, y, dy, p, d, settings.bins_per_duration,
-                                  objective=settings.objective, threads=threads)
-                 for p, d in segs]
-        results.append(parts)
-    elapsed = time.perf_counter() - t0
-    best, powers = [], []
-    from cuperiod_amd.core.result import Periodogram
-
-    for parts in results:
-        power = np.nan_to_num(np.concatenate([p["power"] for p in parts]), nan=0.0,
-                              posinf=0.0, neginf=0.0)
-        period = np.concatenate([p["period"] for p in parts])
-        pg = Periodogram.from_spectrum(method="BLS", backend="oracle", frequency=1.0 / period,
-                                       power=power, objective_sense="max", n_samples=1,
-                                       baseline=1.0)
-        best.append(pg.best_period())
-        powers.append(power)
-    return elapsed, threads, np.asarray(best), powers
-
-
-def learned_leg(pb, wl, settings, plain, plain_s: float, steps: int) -> dict:
-    """The learned proposer + refinement on the same batch, against the plain search:
-    speed, and agreement of the best period (identical, or the same best power, i.e. an
-    exact tie the plain search resolved to another sample)."""
-    import torch
-
-    from cuperiod_amd.learned.proposer import DEFAULT_WEIGHTS, LearnedProposer
-
-    prop = LearnedProposer()
-    t0 = time.perf_counter()
-    prep = prop.prepare(pb.curve_set, wl.grids, settings, full=pb.plans)
-    prep_s = time.perf_counter() - t0
-    prop.search(prep, settings)  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        r = prop.search(prep, settings)
-    torch.cuda.synchronize()
-    el = (time.perf_counter() - t0) / steps
-    identical = r.best_period == plain.best_period
-    same_power = r.best_power == plain.best_power
-    keep = ~r.fallback
-    return {
-        "value": round(len(pb.plans) / el, 3),
-        "unit": "light curves/s",
-        "ms_per_step": round(1000.0 * el, 3),
-        "speedup_vs_plain": round(plain_s / el, 3),
-        "best_period_match": float(np.mean(identical | same_power)),
-        "best_period_identical": float(np.mean(identical)),
-        "fallback_frac": float(np.mean(r.fallback)),
-        "match_without_fallback": float(np.mean((identical | same_power)[keep])) if keep.any() else None,
-        "fine_freqs_per_refined_curve": float(np.mean(r.n_fine[keep])) if keep.any() else None,
-        "coarse_freqs": 4096,
-        "windows": {"max": prop.max_windows, "mass": prop.window_mass,
-                    "threshold": prop.threshold, "fine_per_window": prop.fine_per_window},
-        "timings_s": {k: round(v, 4) for k, v in r.timings.items()},
-        "prepare_s": round(prep_s, 3),
-        "weights": DEFAULT_WEIGHTS.name,
-    }
-
-
 def _kernel_ms(ctx) -> float:
-    import ctypes
-
     ms = ctypes.c_double()
     ctx.check(ctx.lib.cpb_main_kernel_ms(ctx.handle, ctypes.byref(ms)), "timing")
     return ms.value
 
 
+def _oracle_curve(t, y, dy, segs, settings, threads):
+    import oracle
+
+    parts = [oracle.bls_power(t, y, dy, p, d, settings.bins_per_duration,
+                              objective=settings.objective, threads=threads)
+             for p, d in segs]
+    return {k: np.concatenate([p[k] for p in parts]) for k in ("period", *oracle.OUTPUT_NAMES)}
+
+
+def parity_and_cpu_leg(wl, settings, pb, n: int) -> tuple[dict, dict]:
+    """The oracle on the first ``n`` curves: its time (the CPU baseline) and the engine's
+    agreement with it on the same curves and grids."""
+    import oracle
+    from cuperiod_amd.batch.engine import bls_batch
+    from cuperiod_amd.core.result import Periodogram
+    from cuperiod_amd.methods.bls import _grid_segments
+    from oracle.ties import BoxInputs, prove_cross_period_tie
+
+    threads = oracle.default_threads()
+    segs = [_grid_segments(wl.grids[c].as_gridspec(), settings) for c in range(n)]
+    t0, y0, e0 = wl.curves[0]
+    oracle.bls_power(t0[:50], y0[:50], e0[:50], np.array([1.0, 2.0]), np.array([0.1]), 10)
+    start = time.perf_counter()
+    refs = [_oracle_curve(*wl.curves[c], segs[c], settings, threads) for c in range(n)]
+    elapsed = time.perf_counter() - start
+    eng = bls_batch(wl.curves[:n], wl.grids[:n], settings, spectra=True)
+    worst = 0.0
+    identical = ties = 0
+    for c in range(n):
+        ref, got = refs[c], eng.spectra[c]
+        finite = np.isfinite(ref["power"]) & np.isfinite(got["power"])
+        rel = np.abs(got["power"][finite] - ref["power"][finite]) / np.maximum(
+            np.abs(ref["power"][finite]), 1e-300)
+        worst = max(worst, float(rel.max()) if rel.size else 0.0)
+        rpow = np.nan_to_num(ref["power"], neginf=0.0)
+        pg = Periodogram.from_spectrum(method="BLS", backend="oracle",
+                                       frequency=1.0 / ref["period"], power=rpow,
+                                       objective_sense="max", n_samples=1, baseline=1.0)
+        ref_best = pg.best_period()
+        if eng.best_period[c] == ref_best:
+            identical += 1
+        else:
+            gi = int(eng.best_index[c])
+            ri = int(np.flatnonzero(ref["period"] == ref_best)[0])
+            inputs = BoxInputs.for_specs(wl.curves[c][0], segs[c], settings.bins_per_duration)
+            prove_cross_period_tie(ref, inputs, gi, ri, name=f"curve {c}")
+            ties += 1
+    folds, boxes = count_work(pb.plans[:n], [wl.curves[c][0].size for c in range(n)],
+                              settings.bins_per_duration)
+    parity = {
+        "sample": f"the first {n} curves of the batch, every frequency of their grids",
+        "max_rel_power_error": worst,
+        "power_tolerance": 1e-6,
+        "best_period_identical": identical / n,
+        "best_period_equal_point_ties": ties,
+        "best_period_match": (identical + ties) / n,
+        "tie_rule": ("a differing best period counts only if the oracle's own powers at "
+                     "both periods are bit-identical and both boxes hold the same points"),
+    }
+    cpu = {
+        "value": round(n / elapsed, 5),
+        "unit": "light curves/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"the first {n} curves of the same batch, same grids and settings, one "
+                  f"timed pass ({elapsed:.1f} s)",
+        "point_folds_per_s": folds / elapsed,
+        "box_evals_per_s": boxes / elapsed,
+        "sample_point_folds": folds,
+    }
+    return parity, cpu
+
+
+def calibration_leg() -> dict:
+    """The oracle and the engine on the published single-curve shape."""
+    import oracle
+    from cuperiod_amd.core.lightcurve import LightCurve
+    from cuperiod_amd.methods.bls import BLSMethod, _grid_segments
+    from cuperiod_amd.workloads import published_curve
+
+    t, y, dy, settings = published_curve()
+    lc = LightCurve.from_arrays(t, y, dy, domain="flux")
+    method = BLSMethod()
+    grid = method.default_grid(lc, settings)
+    segs = _grid_segments(grid, settings)
+    threads = oracle.default_threads()
+    runs = []
+    _oracle_curve(t, y, dy, segs, settings, threads)  # warm-up
+    for _ in range(3):
+        s0 = time.perf_counter()
+        _oracle_curve(t, y, dy, segs, settings, threads)
+        runs.append(time.perf_counter() - s0)
+    cpu_s = min(runs)
+    method.power(grid, lc, settings, "hip")  # warm-up
+    import torch
+
+    runs = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        method.power(grid, lc, settings, "hip")
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - s0)
+    gpu_s = min(runs)
+    folds = float(t.size) * float(grid.values.size)
+    rate = folds / cpu_s
+    expected = PUBLISHED_NUMBA_FOLDS_PER_S * threads / PUBLISHED_NUMBA_CORES
+    return {
+        "workload": f"N={t.size} over 2,175 d, {grid.values.size} default-grid periods in "
+                    "0.5-4 d (grid_oversample 1): the published single-curve shape",
+        "oracle_s": round(cpu_s, 5),
+        "oracle_point_folds_per_s": rate,
+        "published_numba_point_folds_per_s": PUBLISHED_NUMBA_FOLDS_PER_S,
+        "published_numba_cores": PUBLISHED_NUMBA_CORES,
+        "published_scaled_to_these_cores": expected,
+        "oracle_over_published_scaled": round(rate / expected, 3),
+        "engine_single_curve_s": round(gpu_s, 5),
+        "published_numba_s": 0.039467,
+        "published_cupy_s": 0.0898,
+        "method": "warm-up, then best of 3 (benchmarks/benchmark.py:62-64)",
+        "host_cpu": _cpu_model(),
+        "published_host_cpu": "AMD Ryzen 9 9950X3D, 16 cores / 32 threads",
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def end_to_end_leg(wl, settings, chunk: int) -> dict:
+    """``batch_periodograms`` on the whole batch: checks, packing, uploads, search, peak
+    candidates and rows, with each curve's 200,000-frequency linspace grid."""
+    import torch
+
+    from cuperiod_amd.batch.engine import LinspaceGrid
+    from cuperiod_amd.batch.runner import batch_periodograms
+    from cuperiod_amd.core.lightcurve import LightCurve
+
+    items = [(f"c{c:05d}", LightCurve.from_arrays(t, y, dy, domain="flux"))
+             for c, (t, y, dy) in enumerate(wl.curves)]
+    n_freq = wl.grids[0].num
+
+    def rule(lc):
+        return LinspaceGrid(1.0 / lc.baseline, 2.0, n_freq)
+
+    batch_periodograms(items[:8], "BLS", grid=rule, settings=settings, chunk_size=chunk)
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    res = batch_periodograms(items, "BLS", grid=rule, settings=settings, chunk_size=chunk)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - s0
+    return {
+        "workload": f"batch_periodograms, {len(items)} survey curves, each on linspace(1/"
+                    f"baseline, 2, {n_freq}) /d, {settings.n_durations} segment durations, "
+                    f"chunks of {chunk}, rows with 10 peaks",
+        "value": round(len(items) / el, 3),
+        "unit": "light curves/s",
+        "wall_s": round(el, 3),
+        "rows": res.n_done,
+        "failed": res.n_failed,
+        "includes": "per-curve checks, host preparation and packing, H2D, the search, "
+                    "peak candidates and their D2H, host peak selection into rows",
+    }
+
+
+def long_baseline_leg(n_timed: int, cpu: bool) -> dict:
+    """The long-baseline batch: a seeded subset of its 256 curves timed at the full
+    4,000,000-frequency grid and extrapolated in sum(N x F); the precision modes against
+    the oracle on windows of one curve's grid."""
+    import torch
+
+    from cuperiod_amd import _lib
+    from cuperiod_amd.batch import engine
+    from cuperiod_amd.core.config import BLSSettings
+    from cuperiod_amd.workloads import long_baseline_batch
+
+    settings = BLSSettings(n_durations=16)
+    wl = long_baseline_batch(n_curves=n_timed)
+    ctx = _lib.context(0)
+    ctx.lib.cpb_set_timing(ctx.handle, 1)
+    pb = engine.pack(wl.curves, wl.grids, settings)
+    out, summary = engine.run_packed(pb)  # warm-up
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    out, summary = engine.run_packed(pb, out=out, summary=summary)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - s0
+    k_ms = _kernel_ms(ctx)
+    res = engine.finish(pb, out, summary)
+    folds = float(sum(c[0].size for c in wl.curves)) * float(wl.grids[0].num)
+    full_folds = 256.0 * 100_000 * float(wl.grids[0].num)
+    leg = {
+        "workload": f"long-baseline batch: {n_timed} of its 256 curves timed (N=100,000 over "
+                    "10,000 d offset by 2.45e6 d, 4,000,000 frequencies in [1e-4, 10] /d, "
+                    "16 segment durations); extrapolated to 256 curves in sum(N x F)",
+        "value": round(n_timed / el, 4),
+        "unit": "light curves/s",
+        "timed_curves": n_timed,
+        "wall_s": round(el, 3),
+        "main_kernel_ms": round(k_ms, 3),
+        "point_folds_per_s": folds / el,
+        "extrapolated_batch_s": round(el * full_folds / folds, 2),
+        "best_periods": [float(p) for p in res.best_period],
+        "injected_period": 0.7,
+        "precision": None,
+    }
+    if cpu:
+        leg["precision"] = precision_study()
+    return leg
+
+
+def precision_study(windows: int = 4000) -> dict:
+    """The engine's precision modes against the FP64 oracle on one long-baseline curve:
+    two windows of its grid (the top at 10 /d, phases ~1e5 cycles; around 1/0.7 d)."""
+    import oracle
+    from cuperiod_amd.batch import engine
+    from cuperiod_amd.core.config import BLSSettings
+    from cuperiod_amd.workloads import long_baseline_batch
+
+    base = BLSSettings(n_durations=16)
+    wl = long_baseline_batch(n_curves=1)
+    grid = wl.grids[0]
+    full = engine._linspace_plan(0, grid, base)
+    step = (grid.stop - grid.start) / (grid.num - 1)
+    k_tr = grid.num - 1 - int(round((1.0 / 0.7 - grid.start) / step))
+    plan = engine.window_plan(full, [(0, windows), (k_tr - windows // 2, k_tr + windows // 2)])
+    t, y, dy = wl.curves[0]
+    threads = oracle.default_threads()
+    parts = []
+    s0 = time.perf_counter()
+    for sg in plan.segments:
+        sub = engine.CurvePlan(segments=[sg], n_periods=sg.n_periods)
+        parts.append(oracle.bls_power(t, y, dy, sub.periods(), sg.durations,
+                                      base.bins_per_duration, threads=threads))
+    cpu_s = time.perf_counter() - s0
+    ref = {k: np.concatenate([r[k] for r in parts]) for k in parts[0]}
+    ok = np.isfinite(ref["power"]) & (np.abs(ref["power"]) > 0)
+    best_ref = ref["period"][int(np.argmax(np.where(ok, ref["power"], -np.inf)))]
+    cs = engine.pack_curves(wl.curves)
+    modes = {}
+    for acc in ("fixed64", "kahan32", "fp32"):
+        settings = BLSSettings(n_durations=16, accumulation=acc)
+        pb = engine.pack_plans(cs, [plan], settings)
+        res = engine.finish(pb, *engine.run_packed(pb), spectra=True)
+        p = res.spectra[0]["power"]
+        rel = np.abs(p[ok] - ref["power"][ok]) / np.abs(ref["power"][ok])
+        best = plan.periods()[int(np.argmax(np.where(ok, p, -np.inf)))]
+        modes[acc] = {"max_rel_power_error": float(rel.max()),
+                      "median_rel_power_error": float(np.median(rel)),
+                      "best_period_identical": bool(best == best_ref)}
+    folds = float(t.size) * float(plan.n_periods)
+    return {"curve": "long-baseline curve 0 (N=100,000, t offset 2.45e6 d)",
+            "frequencies": int(plan.n_periods), "oracle_best_period": float(best_ref),
+            "modes": modes,
+            "cpu_oracle": {"seconds": round(cpu_s, 3), "point_folds_per_s": folds / cpu_s,
+                           "cores": int(threads)}}
+
+
 def multiband_leg(n_stars: int, cpu: bool) -> dict:
     """Joint-ephemeris multi-band BLS: stars/s over one batched launch, the sweep's
-    box-band evaluations, and the CPU oracle on one star."""
+    box-band evaluations, and the CPU oracle on two stars."""
     from cuperiod_amd import _lib
     from cuperiod_amd.core.config import BLSSettings
     from cuperiod_amd.core.lightcurve import LightCurve
@@ -193,16 +383,15 @@ def multiband_leg(n_stars: int, cpu: bool) -> dict:
                         spec.width - 1)
         for d in spec.dur_bins:
             boxes += float(np.sum(np.where(nb > d, nb - d + 1, 0)))
-    k_bands = [p.n_bands for p in prepared]
-    box_band = boxes * float(sum(k_bands))
+    box_band = boxes * float(sum(p.n_bands for p in prepared))
     folds = float(grid.values.size) * float(sum(p.tau.size for p in prepared))
     ctx = _lib.context(0)
     ctx.lib.cpb_set_timing(ctx.handle, 1)
     joint_search_stars(prepared[:2], [segs] * 2, os_, objective="snr", backend="hip")
-    t0 = time.perf_counter()
+    s0 = time.perf_counter()
     found = joint_search_stars(prepared, [segs] * len(prepared), os_, objective="snr",
                                backend="hip")
-    wall = time.perf_counter() - t0
+    wall = time.perf_counter() - s0
     k_ms = _kernel_ms(ctx)
     periods = np.concatenate([p for p, _ in segs])
     hits = 0
@@ -215,7 +404,7 @@ def multiband_leg(n_stars: int, cpu: bool) -> dict:
     out = {
         "workload": f"joint-ephemeris multi-band BLS, {len(stars)} stars x 6 bands x 150 "
                     f"points over 1095 d, {grid.values.size} default-grid periods per star",
-        "kernel": "bls_joint_kernel",
+        "kernel": "bls_joint_group_kernel",
         "value": round(len(stars) / (k_ms * 1e-3), 3),
         "unit": "stars/s",
         "kernel_ms": round(k_ms, 3),
@@ -226,14 +415,14 @@ def multiband_leg(n_stars: int, cpu: bool) -> dict:
         "cpu_baseline": None,
     }
     if cpu:
-        from oracle import multiband as omb
         from oracle import default_threads
+        from oracle import multiband as omb
 
         n = 2
-        t0 = time.perf_counter()
+        s0 = time.perf_counter()
         refs = [omb.joint_search(omb.joint_bands(*zip(*s)), segs, os_,
                                  threads=default_threads()) for s in stars[:n]]
-        el = time.perf_counter() - t0
+        el = time.perf_counter() - s0
         diff = 0.0
         for f, ref in zip(found, refs):
             got = np.concatenate([r["power"] for r in f])
@@ -270,9 +459,9 @@ def tls_leg(n_curves: int, cpu: bool) -> dict:
     ctx = _lib.context(0)
     ctx.lib.cpb_set_timing(ctx.handle, 1)
     tls_mod.tls_power_batch(curves[:2], periods[:2], settings=settings)
-    t0 = time.perf_counter()
+    s0 = time.perf_counter()
     res = tls_mod.tls_power_batch(curves, periods, settings=settings)
-    wall = time.perf_counter() - t0
+    wall = time.perf_counter() - s0
     k_ms = _kernel_ms(ctx)
     hits = 0
     for r, p, p_true in zip(res, periods, truth):
@@ -303,10 +492,10 @@ def tls_leg(n_curves: int, cpu: bool) -> dict:
         from oracle import tls as otls
 
         n = min(64, len(curves))
-        t0 = time.perf_counter()
+        s0 = time.perf_counter()
         refs = [otls.tls_power(t, y, e, p, threads=default_threads())
                 for (t, y, e), p in zip(curves[:n], periods[:n])]
-        el = time.perf_counter() - t0
+        el = time.perf_counter() - s0
         diff = max(float(np.max(np.abs(r["sr"] - g["sr"]) / np.maximum(1e-300, np.abs(r["sr"]))))
                    for r, g in zip(refs, res[:n]))
         out["cpu_baseline"] = {
@@ -325,7 +514,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--curves", type=int, default=4096)
     ap.add_argument("--freqs", type=int, default=200_000)
-    ap.add_argument("--cpu-curves", type=int, default=12)
+    ap.add_argument("--cpu-curves", type=int, default=64,
+                    help="curves of the batch searched by the oracle (parity and CPU rate)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape instead of autotuning it")
@@ -335,6 +525,10 @@ def main() -> int:
                     help="skip the learned proposer + refinement leg")
     ap.add_argument("--no-next", action="store_true",
                     help="skip the multi-band and TLS legs")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the batch_periodograms leg")
+    ap.add_argument("--no-long", action="store_true", help="skip the long-baseline leg")
+    ap.add_argument("--long-curves", type=int, default=8)
+    ap.add_argument("--e2e-chunk", type=int, default=512)
     ap.add_argument("--mb-stars", type=int, default=32)
     ap.add_argument("--tls-curves", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -366,8 +560,9 @@ def main() -> int:
         del src, dst
     t0 = time.perf_counter()
     pb = engine.pack(wl.curves, wl.grids, settings)
-    log(f"packed in {time.perf_counter() - t0:.1f} s: {pb.n_segments} segments, "
-        f"{pb.n_out} periods, {pb.total_points} points")
+    pack_s = time.perf_counter() - t0
+    log(f"packed in {pack_s:.1f} s: {pb.n_segments} segments, {pb.n_out} periods, "
+        f"{pb.total_points} points")
     folds, boxes = count_work(pb.plans, [c[0].size for c in wl.curves], pb.oversample)
     log(f"work per step: {folds:.3e} point-folds, {boxes:.3e} box evaluations")
     device = torch.device("cuda", ctx.device)
@@ -401,13 +596,9 @@ def main() -> int:
     t_start = time.perf_counter()
     for s in range(args.steps):
         step()
-        import ctypes
-
-        ms = ctypes.c_double()
-        ctx.check(ctx.lib.cpb_main_kernel_ms(ctx.handle, ctypes.byref(ms)), "timing")
-        kernel_ms.append(ms.value)
+        kernel_ms.append(_kernel_ms(ctx))
         log(f"step {s + 1}/{args.steps}: {'graph' if not args.no_graph else 'main kernel'} "
-            f"{ms.value:.1f} ms")
+            f"{kernel_ms[-1]:.1f} ms")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     res = engine.finish(pb, out, summary)
@@ -416,6 +607,14 @@ def main() -> int:
     k_ms = float(np.mean(kernel_ms))
     flops = 4.0 * folds + 8.0 * boxes
     achieved = flops / (k_ms * 1e-3)
+    measured = MEASURED.get((len(pb.plans), args.freqs), {})
+    issue = None
+    if "valu_wave_instr" in measured:
+        lane_rate = 64.0 * measured["valu_wave_instr"] / (k_ms * 1e-3)
+        issue = {"valu_wave_instructions_per_step": measured["valu_wave_instr"],
+                 "lane_instructions_per_s": lane_rate, "issue_limit": VALU_ISSUE,
+                 "frac": round(lane_rate / VALU_ISSUE, 4),
+                 "per_box_eval": round(64.0 * measured["valu_wave_instr"] / boxes, 2)}
     line = {
         "metric": METRIC,
         "value": round(curves_per_s, 3),
@@ -436,14 +635,16 @@ def main() -> int:
             "freqs_per_curve": args.freqs,
             "point_folds_per_step": folds,
             "box_evals_per_step": boxes,
+            "host_pack_s": round(pack_s, 3),
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "achieved": round(achieved / 1e12, 4),
-            "peak": PEAK_FP32 / 1e12,
+            "peak": PEAK_VALU_FLOPS / 1e12,
             "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32, 5),
-            "traffic": MEASURED_TRAFFIC.get((len(pb.plans), args.freqs)),
+            "frac": round(achieved / PEAK_VALU_FLOPS, 5),
+            "traffic": measured.get("traffic"),
+            "issue": issue,
         },
         "detail": {
             "kernel": "bls_main_kernel",
@@ -455,18 +656,32 @@ def main() -> int:
             "best_period_sample": [float(x) for x in res.best_period[:4]],
         },
         "launch": launch,
+        "parity": None,
+        "cpu_baseline": None,
+        "end_to_end": None,
+        "long_baseline": None,
         "learned": None,
         "next_rows": None,
-        "cpu_baseline": None,
     }
-    from cuperiod_amd.learned.proposer import DEFAULT_WEIGHTS
-
-    if not args.no_learned and DEFAULT_WEIGHTS.exists():
-        log("learned proposer + refinement ...")
-        line["learned"] = learned_leg(pb, wl, settings, res, elapsed / args.steps,
-                                      max(1, min(args.steps, 2)))
-        log(f"learned: {line['learned']['value']} curves/s, match "
-            f"{line['learned']['best_period_match']:.4f}")
+    if not args.no_cpu and args.cpu_curves > 0:
+        n = min(args.cpu_curves, len(pb.plans))
+        log(f"oracle on {n} curves (parity and CPU rate) ...")
+        parity, cpu = parity_and_cpu_leg(wl, settings, pb, n)
+        cpu["extrapolated_batch_s"] = round(folds / cpu["sample_point_folds"] * n / cpu["value"], 1)
+        cpu["gpu_over_cpu"] = round(curves_per_s / cpu["value"], 1)
+        cpu["calibration"] = calibration_leg()
+        line["parity"], line["cpu_baseline"] = parity, cpu
+        log(f"parity: identical {parity['best_period_identical']:.4f}, ties "
+            f"{parity['best_period_equal_point_ties']}, max rel err "
+            f"{parity['max_rel_power_error']:.3g}; CPU {cpu['value']} curves/s")
+    if not args.no_e2e:
+        log("end-to-end batch_periodograms ...")
+        line["end_to_end"] = end_to_end_leg(wl, settings, args.e2e_chunk)
+        log(f"end-to-end: {line['end_to_end']['value']} curves/s")
+    if not args.no_long:
+        log("long-baseline batch ...")
+        line["long_baseline"] = long_baseline_leg(args.long_curves, not args.no_cpu)
+        log(f"long-baseline: {line['long_baseline']['value']} curves/s")
     if not args.no_next:
         log("multi-band joint search ...")
         mb = multiband_leg(args.mb_stars, not args.no_cpu)
@@ -475,27 +690,6 @@ def main() -> int:
         tl = tls_leg(args.tls_curves, not args.no_cpu)
         log(f"TLS: {tl['value']} curves/s ({tl['kernel_ms']} ms)")
         line["next_rows"] = {"multiband": mb, "tls": tl}
-    if not args.no_cpu and args.cpu_curves > 0:
-        n = min(args.cpu_curves, len(pb.plans))
-        log(f"CPU oracle on {n} curves ...")
-        el, threads, best_cpu, cpu_powers = cpu_baseline(wl, settings, n)
-        identical = best_cpu == res.best_period[:n]
-        # A different sample of a flat-topped peak is an exact tie: the oracle's power
-        # at the GPU's choice equals the oracle's maximum to 1e-12.
-        tie = np.array([abs(p[int(res.best_index[c])] - p.max()) <= 1e-12 * abs(p.max())
-                        for c, p in enumerate(cpu_powers)])
-        match = float(np.mean(identical | tie))
-        line["cpu_baseline"] = {
-            "value": round(n / el, 5),
-            "unit": "light curves/s",
-            "cores": int(threads),
-            "kind": "port",
-            "sample": f"first {n} curves of the same batch, same grids and settings "
-                      f"({el:.1f} s)",
-            "best_period_match": match,
-            "best_period_identical": float(np.mean(identical)),
-            "gpu_over_cpu": round(curves_per_s / (n / el), 1),
-        }
     print(json.dumps(line), flush=True)
     return 0
 

@@ -385,6 +385,8 @@ class _Prepared:
     label: str
     finite: LightCurve
     periods: np.ndarray | None = None
+    # the light curve's own grid when ``batch_periodograms`` was given a grid rule
+    grid: Any = None
 
 
 def _prepare(position: int, key: str, lc: LightCurve, settings: Any,
@@ -403,6 +405,16 @@ def _prepare(position: int, key: str, lc: LightCurve, settings: Any,
         )
     if finite.baseline <= 0.0:
         raise InsufficientDataError("BLS: no usable time baseline")
+    own = None
+    if callable(grid):
+        # a grid rule: this light curve's grid (a GridSpec or a parametric LinspaceGrid)
+        own = grid(finite)
+        if not isinstance(own, (GridSpec, engine.LinspaceGrid)):
+            raise TypeError("a grid rule must return a GridSpec or a LinspaceGrid, got "
+                            f"{type(own).__name__}")
+        grid = own if isinstance(own, GridSpec) else None
+        if grid is None:
+            return _Prepared(position, key, source_label(lc), finite, grid=own)
     if grid is None:
         plan = engine._default_plan(0, finite.baseline, finite.time, settings)
         if not plan.segments:
@@ -411,7 +423,7 @@ def _prepare(position: int, key: str, lc: LightCurve, settings: Any,
             )
     elif not _grid_segments(grid, settings):
         raise InsufficientDataError("BLS: empty trial grid")
-    return _Prepared(position, key, source_label(lc), finite)
+    return _Prepared(position, key, source_label(lc), finite, grid=own)
 
 
 def _prepare_tls(position: int, key: str, lc: LightCurve, settings: Any,
@@ -494,8 +506,8 @@ def _launch(prepared: Sequence[_Prepared], settings: Any, grid: GridSpec | None)
     """Pack the curves and enqueue one batched search (returns without waiting)."""
     from cuperiod_amd.batch import engine
 
-    pb = engine.pack([_curve_arrays(p.finite) for p in prepared],
-                     [grid] * len(prepared), settings)
+    grids = [p.grid if p.grid is not None else grid for p in prepared]
+    pb = engine.pack([_curve_arrays(p.finite) for p in prepared], grids, settings)
     out, summary = engine.run_packed(pb)
     return pb, out, summary
 
@@ -713,6 +725,12 @@ class _ChunkRunner:
         self._complete()
 
 
+def names_of(method: str | Sequence[str]) -> tuple[str, ...]:
+    """The requested method names, upper-cased."""
+    names = (method,) if isinstance(method, str) else tuple(method)
+    return tuple(str(n).upper() for n in names)
+
+
 def _chunked(items: Sequence[Any], size: int) -> list[list[Any]]:
     return [list(items[i:i + size]) for i in range(0, len(items), size)]
 
@@ -742,7 +760,10 @@ def batch_periodograms(
     call per chunk of ``chunk_size`` light curves (default 256); ``device="cpu"``
     raises :class:`BackendUnavailableError`, since this package has no CPU backend.
     ``workers`` is accepted for signature compatibility: the batched engine fills the
-    device from one process. ``grid``, ``settings``, ``sink``, ``n_best``,
+    device from one process. ``grid`` may also be a grid rule, a callable that takes a
+    light curve (its finite points) and returns its own grid -- a ``GridSpec`` or a
+    parametric :class:`~cuperiod_amd.batch.engine.LinspaceGrid` (BLS only; not with a
+    directory sink). ``grid``, ``settings``, ``sink``, ``n_best``,
     ``store_raw``, ``resume`` and ``chunk_size`` mean what they mean in cuPeriod
     (src/cuperiod/batch/runner.py:764-859). ``columns`` and ``domain`` only concern
     file and table inputs, which this package does not read; ``band_column``
@@ -760,8 +781,15 @@ def batch_periodograms(
         raise ValueError("device must be 'cpu', 'gpu', or 'hybrid'")
     if device == "hybrid":
         raise NotImplementedError("hybrid device is planned; use device='gpu'")
-    if grid is not None and not isinstance(grid, GridSpec):
-        raise TypeError(f"grid must be a GridSpec or None, got {type(grid).__name__}")
+    if grid is not None and not isinstance(grid, GridSpec) and not callable(grid):
+        raise TypeError(f"grid must be a GridSpec, a grid rule or None, got "
+                        f"{type(grid).__name__}")
+    if callable(grid) and not isinstance(grid, GridSpec):
+        if set(names_of(method)) - {"BLS"}:
+            raise ValueError("a grid rule (a callable grid) is supported for BLS only")
+        if sink is not None and _classify_sink(sink)[0] == "dir":
+            raise ValueError("a grid rule cannot be fingerprinted for a resumable directory "
+                             "sink; pass a GridSpec")
     if chunk_size is not None and int(chunk_size) < 1:
         raise ValueError(f"chunk_size must be a positive integer, got {chunk_size}")
     if band_column is not None:

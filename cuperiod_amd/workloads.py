@@ -1,6 +1,6 @@
 """Seeded synthetic survey workloads used by the benchmark and the tests.
 
-Seven workloads, each generated with ``np.random.default_rng(20261203 + k)`` for the k-th
+Eight workloads, each generated with ``np.random.default_rng(20261203 + k)`` for the k-th
 of the list below:
 
 1. ``small_curve``: one curve, N = 2,000 over 100 d, box transit P = 3.7 d, depth 2e-3, duration 0.1 d,
@@ -22,6 +22,10 @@ of the list below:
    chromatic box transits (P log-uniform in [0.5, 20] d, depth 0.7-1.3 x a
    log-uniform 2e-3 .. 1e-2, duration 0.02-0.05 of P) in every other star; searched
    jointly over the default BLS grid of a 1,095 d baseline;
+8. ``published_curve`` (seed k = 8): the shape of cuPeriod's published single-curve BLS
+   timing (benchmarks/benchmark.py:36-47, 149-156): N = 891 over a 2,175 d baseline,
+   sigma 1e-3, a 1.3 d box transit of depth 3e-3 and duration 0.08 d, periods 0.5-4 d on
+   the default grid with grid_oversample 1 (76,125 periods here, 76,124 published);
 7. ``tls_survey``: TESS-sector-like curves, 20,000 points over 27 d (offset 2.459e6 d),
    sigma log-uniform in [2e-4, 2e-3], limb-darkened transits (u1 0.4, u2 0.3; P
    log-uniform in [0.6, 12] d, depth log-uniform in [3e-4, 5e-3], duration 0.02-0.06 of
@@ -185,5 +189,22 @@ def tls_survey(n_curves: int = 256, n_points: int = 20_000, span: float = 27.0):
     return curves, truth
 
 
+
+
+
+def published_curve():
+    """``(t, flux, dy, settings)`` of the published single-curve BLS timing's shape."""
+    from cuperiod_amd.core.config import BLSSettings
+
+    rng = np.random.default_rng(SEED + 8)
+    t = np.sort(rng.uniform(0.0, 2175.0, 891))
+    t[0], t[-1] = 0.0, 2175.0
+    y = 1.0 + rng.normal(0.0, 1e-3, t.size)
+    y[_box_dip(t, 1.3, 0.4, 0.08)] -= 3e-3
+    settings = BLSSettings(min_period_days=0.5, max_period_days=4.0, grid_oversample=1)
+    return t, y, np.full(t.size, 1e-3), settings
+
+
 __all__ = ["SEED", "Workload", "gapped_curve", "long_baseline_batch", "multiband_survey",
+           "published_curve",
            "proposer_batch", "small_curve", "survey_batch", "survey_curves", "tls_survey"]

@@ -34,7 +34,8 @@ def _curve(seed: int, n: int = 160, baseline: float = 24.0, period: float = 2.3)
 def _oracle_spectra(prepared, settings, grid):
     out = []
     for p in prepared:
-        g = grid if grid is not None else BLSMethod().default_grid(p.finite, settings)
+        g = p.grid if p.grid is not None else grid
+        g = g if g is not None else BLSMethod().default_grid(p.finite, settings)
         t, y = p.finite.time, p.finite.value
         err = p.finite.error if p.finite.error is not None else np.ones_like(y)
         parts = [oracle.bls_power(t, y, err, per, dur, settings.bins_per_duration,
@@ -132,6 +133,24 @@ def test_earlier_method_failure_runs_no_later_method(fake_engine, monkeypatch):
     assert [(r["key"], r["method"]) for r in res.rows] == [
         ("a", "BLS"), ("a", "TLS"), ("c", "BLS"), ("c", "TLS")]
     assert [k for k, _ in res.errors] == ["b"]
+
+
+def test_grid_rule_gives_each_curve_its_grid(fake_engine):
+    # A grid rule (a callable) gives each light curve its own grid; rows equal those of
+    # the same grid passed directly.
+    from cuperiod_amd.core.grid import GridSpec
+
+    def rule(lc):
+        return GridSpec(kind="frequency", values=np.linspace(1.0 / lc.baseline, 2.0, 700))
+
+    lcs = [("a", _curve(1)), ("b", _curve(3, baseline=30.0))]
+    res = runner.batch_periodograms(lcs, "BLS", grid=rule)
+    assert res.n_done == 2
+    for row, (key, lc) in zip(res.rows, lcs):
+        one = runner.batch_periodograms([(key, lc)], "BLS", grid=rule(lc.finite()))
+        _same(row, one.rows[0])
+    with pytest.raises(ValueError, match="BLS only"):
+        runner.batch_periodograms(lcs, ["BLS", "TLS"], grid=rule)
 
 
 def test_keys_from_meta_and_renamed(fake_engine):

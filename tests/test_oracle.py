@@ -192,3 +192,57 @@ def test_threads_do_not_change_results():
     b = oracle.bls_power(t, flux, err, periods, np.array([0.05, 0.1]), 5, threads=4)
     for k in oracle.OUTPUT_NAMES:
         assert np.array_equal(a[k], b[k])
+
+
+def test_box_members_reproduce_the_oracle_power():
+    # The tie proofs list a box's points by folding with the reference's formula; the
+    # members of the oracle's own winning box must give back its power by direct sums.
+    from oracle.ties import BoxInputs, box_members, recover_boxes
+
+    rng = np.random.default_rng(1)
+    t = np.sort(rng.uniform(0, 30, 300))
+    y = 1 + 1e-3 * rng.standard_normal(300)
+    y[np.mod(t, 2.3) < 0.1] -= 3e-3
+    dy = np.full(300, 1e-3)
+    periods = np.linspace(1.5, 3.0, 60)
+    durations = np.array([0.05, 0.1])
+    ref = oracle.bls_power(t, y, dy, periods, durations, 3)
+    inp = BoxInputs.for_specs(t, [(periods, durations)], 3)
+    _, yh, ivh = oracle.host_inputs(t, y, dy)
+    yw = yh * ivh
+    sy, si = math.fsum(yw), math.fsum(ivh)
+    for i in range(periods.size):
+        boxes = recover_boxes(ref, inp, i)
+        assert boxes, i
+        m = box_members(inp, i, *boxes[0])
+        yin, ivin = math.fsum(yw[m]), math.fsum(ivh[m])
+        depth = (sy - yin) / (si - ivin) - yin / ivin
+        snr = depth / math.sqrt(1 / ivin + 1 / (si - ivin))
+        assert snr == pytest.approx(ref["power"][i], rel=1e-9)
+
+
+def test_tie_proof_refuses_a_box_with_other_points():
+    # A planted mistake on the oracle's side: its box moved by one bin onto a different
+    # point set must fail the proof; the unmoved box passes.
+    from oracle.ties import BoxInputs, box_members, prove_same_points, recover_boxes
+
+    rng = np.random.default_rng(2)
+    t = np.sort(rng.uniform(0, 20, 400))
+    y = 1 + 1e-3 * rng.standard_normal(400)
+    dy = np.full(400, 1e-3)
+    periods = np.array([1.7, 2.9])
+    durations = np.array([0.1])
+    ref = oracle.bls_power(t, y, dy, periods, durations, 4)
+    inp = BoxInputs.for_specs(t, [(periods, durations)], 4)
+    assert prove_same_points(ref, ref, inp, np.arange(2)) == 2
+    n, d = recover_boxes(ref, inp, 0)[0]
+    bd, P = float(inp.bin_duration[0]), float(inp.period[0])
+    for shift in (1, 2, 3, -1, -2):
+        n2 = n + shift
+        if 0 <= n2 <= inp.n_bins(0) - d and not np.array_equal(
+                box_members(inp, 0, n2, d), box_members(inp, 0, n, d)):
+            break
+    moved = {k: np.array(v, copy=True) for k, v in ref.items()}
+    moved["transit_time"][0] = math.fmod(n2 * bd + 0.5 * d * bd, P) + inp.t_min
+    with pytest.raises(AssertionError, match="different points"):
+        prove_same_points(ref, moved, inp, np.array([0]))
