@@ -215,43 +215,85 @@ def _default_plan(curve: int, baseline: float, time: np.ndarray, settings: BLSSe
 def _linspace_plan(curve: int, grid: LinspaceGrid, settings: BLSSettings) -> CurvePlan:
     """A linspace frequency grid split into log period segments from its shortest
     period, as ``_log_segments`` splits the materialised grid."""
-    num = int(grid.num)
-    p0 = float(grid.period_at(0))
-    factor = math.log(settings.segment_factor)
+    return _linspace_plans([curve], [grid], settings)[0]
 
-    def seg_index(k: int) -> int:
-        p = float(grid.period_at(k))
-        return int(np.floor(np.log(np.float64(p) / np.float64(p0)) / factor + 1e-9))
 
-    plan = CurvePlan()
-    k0 = 0
-    last = seg_index(num - 1)
-    step = (float(grid.stop) - float(grid.start)) / (num - 1)
-    while k0 < num:
-        s = seg_index(k0)
-        lo, hi = k0, num - 1
-        if seg_index(hi) == s:
-            k1 = num
-        else:
-            while hi - lo > 1:
-                mid = (lo + hi) // 2
-                if seg_index(mid) == s:
-                    lo = mid
-                else:
-                    hi = mid
-            k1 = hi
-        plan.segments.append(
-            _SegPlan(curve=curve, n_periods=k1 - k0,
-                     durations=_segment_durations(p0 * settings.segment_factor**s, settings),
-                     p_max=float(grid.period_at(k1 - 1)), mode=1,
-                     lin_start=float(grid.start), lin_step=step, lin_stop=float(grid.stop),
-                     lin_num=num, lin_k0=k0)
-        )
-        plan.n_periods += k1 - k0
-        k0 = k1
-        if s == last:
-            break
-    return plan
+def _linspace_periods(start: np.ndarray, stop: np.ndarray, num: np.ndarray,
+                      k: np.ndarray) -> np.ndarray:
+    """Ascending-period index ``k`` of linspace grids, by numpy's (and the device's)
+    recipe: ``f_i = i*step + start`` with ``f_{num-1} = stop``, ``i = num-1-k``."""
+    i = num - 1 - k
+    step = (stop - start) / (num - 1).astype(np.float64)
+    f = i.astype(np.float64) * step + start
+    return 1.0 / np.where(i == num - 1, stop, f)
+
+
+def _linspace_plans(curves: Sequence[int], grids: Sequence[LinspaceGrid],
+                    settings: BLSSettings) -> list[CurvePlan]:
+    """:func:`_linspace_plan` of many grids at once: every segment boundary is found by
+    a bisection on the period index run for all (grid, segment) pairs together; the
+    segment index of a period is ``_log_segment_index``'s array expression."""
+    n = len(grids)
+    if n == 0:
+        return []
+    start = np.array([g.start for g in grids], dtype=np.float64)
+    stop = np.array([g.stop for g in grids], dtype=np.float64)
+    num = np.array([g.num for g in grids], dtype=np.int64)
+    p0 = 1.0 / stop  # period of ascending index 0 (f = stop)
+    log_factor = math.log(settings.segment_factor)
+
+    def seg_index(rows: np.ndarray, k: np.ndarray) -> np.ndarray:
+        per = _linspace_periods(start[rows], stop[rows], num[rows], k)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return np.floor(np.log(per / p0[rows]) / log_factor + 1e-9).astype(np.int64)
+
+    rows = np.arange(n)
+    last = seg_index(rows, num - 1)
+    # (grid, s) pairs for s = 1 .. last: the first ascending index with index >= s
+    pair_rows = np.repeat(rows, last)
+    pair_s = np.concatenate([np.arange(1, m + 1) for m in last]) if last.sum() else \
+        np.zeros(0, dtype=np.int64)
+    lo = np.zeros(pair_rows.size, dtype=np.int64)
+    hi = num[pair_rows] - 1
+    while np.any(lo < hi):
+        mid = (lo + hi) // 2
+        ge = seg_index(pair_rows, mid) >= pair_s
+        open_ = lo < hi
+        hi = np.where(open_ & ge, mid, hi)
+        lo = np.where(open_ & ~ge, mid + 1, lo)
+    firsts = np.split(lo, np.cumsum(last)[:-1]) if n > 1 else [lo]
+    # every segment's longest period (its last index), all grids at once
+    seg_rows = np.repeat(rows, last + 1)
+    seg_end = np.concatenate([np.concatenate([f, [num[r]]]) for r, f in enumerate(firsts)])
+    p_last = _linspace_periods(start[seg_rows], stop[seg_rows], num[seg_rows],
+                               np.maximum(seg_end - 1, 0))
+    p_last_of = np.split(p_last, np.cumsum(last + 1)[:-1]) if n > 1 else [p_last]
+    durations: dict[float, np.ndarray] = {}
+    plans: list[CurvePlan] = []
+    for r in range(n):
+        grid = grids[r]
+        g_num = int(num[r])
+        step = (float(grid.stop) - float(grid.start)) / (g_num - 1)
+        # segment s covers [first(s), first(s+1)); indices never reached are skipped, as
+        # np.unique skips them in _log_segments
+        bounds = [0, *firsts[r].tolist(), g_num]
+        plan = CurvePlan()
+        p_ends = p_last_of[r].tolist()
+        for s, k0, k1 in zip(range(int(last[r]) + 1), bounds[:-1], bounds[1:]):
+            if k1 <= k0:
+                continue
+            p_lo = float(p0[r]) * settings.segment_factor**s
+            dur = durations.get(p_lo)
+            if dur is None:
+                dur = durations[p_lo] = _segment_durations(p_lo, settings)
+            p_max = p_ends[s]
+            plan.segments.append(
+                _SegPlan(curve=curves[r], n_periods=k1 - k0, durations=dur, p_max=p_max,
+                         mode=1, lin_start=float(grid.start), lin_step=step,
+                         lin_stop=float(grid.stop), lin_num=g_num, lin_k0=k0))
+            plan.n_periods += k1 - k0
+        plans.append(plan)
+    return plans
 
 
 def _explicit_plan(curve: int, grid: GridSpec, settings: BLSSettings) -> CurvePlan:
@@ -303,6 +345,8 @@ class CurveSet:
     n_points: np.ndarray
     point_base: np.ndarray
     times: list[np.ndarray]
+    # the CURVE_DTYPE tables behind ``curves`` / ``curves32`` (kept alive with them)
+    tables: tuple = ()
 
 
 @dataclass
@@ -328,6 +372,8 @@ class PackedBatch:
     obj_flag: int
     curve_set: CurveSet | None = None
     accumulation: int = 0
+    # the SEG_DTYPE table behind ``segments``
+    segment_table: np.ndarray | None = None
 
 
 def _device():
@@ -348,19 +394,33 @@ def _upload(a: np.ndarray, dtype, device) -> Any:
     return torch.from_numpy(a).to(device)
 
 
+#: numpy layouts of the descriptors (field for field the ctypes structures, which
+#: mirror cpb_curve / cpb_segment): descriptor tables are filled column-wise and handed
+#: to the library as ctypes arrays over the same memory.
+CURVE_DTYPE = np.dtype(_lib.Curve)
+SEG_DTYPE = np.dtype(_lib.Segment)
+
+
+def curve_array(table: np.ndarray):
+    """A ctypes ``Curve`` array over a ``CURVE_DTYPE`` table (the table must stay alive)."""
+    return (_lib.Curve * table.size).from_buffer(table)
+
+
 def pack_curves(lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]]) -> CurveSet:
     """Host preparation (median-centred flux, ``ivar``, time origin, floor) and one
     upload of a chunk's curves ``(t, flux, dy)``."""
     device = _device()
     taus, yws, ivs, times = [], [], [], []
-    curves = (_lib.Curve * max(1, len(lcs)))()
-    curves32 = (_lib.Curve * max(1, len(lcs)))()
     n = len(lcs)
+    table = np.zeros(max(1, n), dtype=CURVE_DTYPE)
+    table32 = np.zeros(max(1, n), dtype=CURVE_DTYPE)
     t_min = np.zeros(n)
     baseline = np.zeros(n)
     n_points = np.zeros(n, dtype=np.int64)
-    point_base = np.zeros(n, dtype=np.int64)
-    off = 0
+    floor64 = np.zeros(n)
+    floor32 = np.zeros(n)
+    sum_y = np.zeros(n)
+    sum_iv = np.zeros(n)
     for c, (t, y, dy) in enumerate(lcs):
         t_h, y_h, iv_h = core._host_inputs(t, y, dy)
         if t_h.size == 0:
@@ -371,28 +431,29 @@ def pack_curves(lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]]) -> Cur
         yws.append(y_h * iv_h)
         ivs.append(iv_h)
         times.append(t_h)
-        cv = curves[c]
-        cv.point_offset = off
-        cv.n_points = int(t_h.size)
-        cv.t_min = t_min[c]
-        sum_ivar = float(iv_h.sum())
-        cv.ivar_floor = core._ivar_floor(int(t_h.size), sum_ivar)
-        # the reference's host sums (numpy), read by the reference rounding of the peak
-        cv.sum_y = float(yws[-1].sum())
-        cv.sum_ivar = sum_ivar
-        c32 = curves32[c]
-        c32.point_offset, c32.n_points, c32.t_min = cv.point_offset, cv.n_points, cv.t_min
-        c32.ivar_floor = core._ivar_floor(int(t_h.size), sum_ivar, float32=True)
-        c32.sum_y, c32.sum_ivar = cv.sum_y, cv.sum_ivar
         n_points[c] = t_h.size
-        point_base[c] = off
-        off += int(t_h.size)
+        sum_iv[c] = float(iv_h.sum())
+        # the reference's host sums (numpy), read by the reference rounding of the peak
+        sum_y[c] = float(yws[-1].sum())
+        floor64[c] = core._ivar_floor(int(t_h.size), sum_iv[c])
+        floor32[c] = core._ivar_floor(int(t_h.size), sum_iv[c], float32=True)
+    point_base = np.concatenate([[0], np.cumsum(n_points)[:-1]]).astype(np.int64) if n else \
+        np.zeros(0, dtype=np.int64)
+    for tab, floor in ((table, floor64), (table32, floor32)):
+        tab["point_offset"][:n] = point_base
+        tab["n_points"][:n] = n_points
+        tab["t_min"][:n] = t_min
+        tab["ivar_floor"][:n] = floor
+        tab["sum_y"][:n] = sum_y
+        tab["sum_ivar"][:n] = sum_iv
+    off = int(n_points.sum())
     return CurveSet(
         tau=_upload(np.concatenate(taus) if taus else np.zeros(0), np.float64, device),
         yw=_upload(np.concatenate(yws) if yws else np.zeros(0), np.float64, device),
         ivar=_upload(np.concatenate(ivs) if ivs else np.zeros(0), np.float64, device),
-        total_points=off, curves=curves, curves32=curves32, t_min=t_min, baseline=baseline, n_points=n_points,
-        point_base=point_base, times=times,
+        total_points=off, curves=curve_array(table), curves32=curve_array(table32),
+        t_min=t_min, baseline=baseline, n_points=n_points, point_base=point_base,
+        times=times, tables=(table, table32),
     )
 
 
@@ -404,6 +465,19 @@ def plan_grid(cs: CurveSet, c: int, grid: LinspaceGrid | GridSpec | None,
     if isinstance(grid, LinspaceGrid):
         return _linspace_plan(c, grid, settings)
     return _explicit_plan(c, grid, settings)
+
+
+def plan_grids(cs: CurveSet, grids: Sequence[LinspaceGrid | GridSpec | None],
+               settings: BLSSettings) -> list[CurvePlan]:
+    """:func:`plan_grid` of every curve (linspace grids segmented together)."""
+    plans: list[CurvePlan | None] = [None] * len(grids)
+    lin = [c for c, g in enumerate(grids) if isinstance(g, LinspaceGrid)]
+    for c, plan in zip(lin, _linspace_plans(lin, [grids[c] for c in lin], settings)):
+        plans[c] = plan
+    for c, g in enumerate(grids):
+        if plans[c] is None:
+            plans[c] = plan_grid(cs, c, g, settings)
+    return plans
 
 
 def _parametric_need(par: dict[str, np.ndarray], idx: np.ndarray, k: np.ndarray,
@@ -479,6 +553,7 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
     pieces: list[tuple[int, _SegPlan, int, int, int]] = []
     parametric: list[tuple[int, int, _SegPlan]] = []
     group_of: dict[int, int] = {}
+    explicit_cuts: dict[int, list] = {}
     for c, plan in enumerate(plans):
         for pos, sg in enumerate(plan.segments):
             group_of[id(sg)] = sg.group if sg.group >= 0 else pos
@@ -487,8 +562,10 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
             if width <= core._SPAN_LIMITS[0] or sg.n_periods < 2:
                 pieces.append((c, sg, 0, sg.n_periods, width))
             elif sg.mode == 0:
-                for a, b, w in core._width_ranges(sg.periods, bd, oversample):
-                    pieces.append((c, sg, a, b, w))
+                rng = explicit_cuts.get(id(sg))
+                if rng is None:
+                    rng = explicit_cuts[id(sg)] = core._width_ranges(sg.periods, bd, oversample)
+                pieces.extend((c, sg, a, b, w) for a, b, w in rng)
             else:
                 parametric.append((len(pieces), c, sg))
                 pieces.append((c, sg, -1, -1, -1))  # filled in below
@@ -506,52 +583,82 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
             else:
                 expanded.append(piece)
         pieces = expanded
-    segs = (_lib.Segment * max(1, len(pieces)))()
+    n_pieces = len(pieces)
+    table = np.zeros(max(1, n_pieces), dtype=SEG_DTYPE)
     dur_parts: list[np.ndarray] = []
     per_parts: list[np.ndarray] = []
-    out_off = p_off = d_off = 0
-    curve_base = np.zeros(len(plans), dtype=np.int64)
+    p_off = d_off = 0
     curve_count = np.array([plan.n_periods for plan in plans], dtype=np.int64)
-    dur_cache: dict[tuple[float, bytes], tuple[np.ndarray, int]] = {}
-    last_curve = -1
+    # per segment, once: its bin duration, box widths (deduplicated by value) and group
+    seg_info: dict[int, tuple[float, int, int, int]] = {}
+    dur_cache: dict[tuple[float, bytes], tuple[int, int]] = {}
+    per_cache: dict[tuple[int, int, int], int] = {}
+    col_c = np.empty(n_pieces, dtype=np.int64)
+    col_a = np.empty(n_pieces, dtype=np.int64)
+    col_b = np.empty(n_pieces, dtype=np.int64)
+    col_w = np.empty(n_pieces, dtype=np.int64)
+    col_s = np.empty(n_pieces, dtype=np.int64)
+    col_p = np.zeros(n_pieces, dtype=np.int64)
+    seg_list: list[_SegPlan] = []
+    seg_pos: dict[int, int] = {}
     for j, (c, sg, a, b, width) in enumerate(pieces):
-        if c != last_curve:
-            for cc in range(last_curve + 1, c + 1):
-                curve_base[cc] = out_off
-            last_curve = c
-        bd = float(sg.durations.min()) / oversample
-        key = (bd, sg.durations.tobytes())
-        hit = dur_cache.get(key)
-        if hit is None:
-            db = np.asarray(core._duration_bins(sg.durations, bd), dtype=np.int32)
-            hit = (db, d_off)
-            dur_cache[key] = hit
-            dur_parts.append(db)
-            d_off += int(db.size)
-        db, db_off = hit
-        d = segs[j]
-        d.bin_duration = bd
-        d.n_periods = b - a
-        d.out_offset = out_off
-        d.width = int(width)
-        d.dur_offset = db_off
-        d.n_dur = int(db.size)
-        d.curve = c
-        d.mode = sg.mode
-        d.group = group_of[id(sg)]
-        d.lin_start, d.lin_step, d.lin_stop = sg.lin_start, sg.lin_step, sg.lin_stop
-        d.lin_num, d.lin_k0 = sg.lin_num, sg.lin_k0 + a
+        key = id(sg)
+        pos = seg_pos.get(key)
+        if pos is None:
+            pos = seg_pos[key] = len(seg_list)
+            seg_list.append(sg)
+            bd = float(sg.durations.min()) / oversample
+            dkey = (bd, sg.durations.tobytes())
+            hit = dur_cache.get(dkey)
+            if hit is None:
+                db = np.asarray(core._duration_bins(sg.durations, bd), dtype=np.int32)
+                hit = dur_cache[dkey] = (d_off, int(db.size))
+                dur_parts.append(db)
+                d_off += int(db.size)
+            seg_info[key] = (bd, hit[0], hit[1], group_of[key])
+        col_c[j], col_a[j], col_b[j], col_w[j], col_s[j] = c, a, b, width, pos
         if sg.mode == 0:
-            d.period_offset = p_off
-            per_parts.append(sg.periods[a:b])
-            p_off += b - a
-        out_off += b - a
-    for cc in range(last_curve + 1, len(plans)):
-        curve_base[cc] = out_off
+            # the same explicit periods (one array shared by several plans) go up once
+            pkey = (id(sg.periods), a, b)
+            off = per_cache.get(pkey)
+            if off is None:
+                off = per_cache[pkey] = p_off
+                per_parts.append(sg.periods[a:b])
+                p_off += b - a
+            col_p[j] = off
+    if n_pieces:
+        info = np.array([seg_info[id(sg)] for sg in seg_list], dtype=np.float64)
+        lin = np.array([(sg.mode, sg.lin_start, sg.lin_step, sg.lin_stop, sg.lin_num,
+                         sg.lin_k0) for sg in seg_list], dtype=np.float64)
+        n_per = col_b - col_a
+        out_off = np.concatenate([[0], np.cumsum(n_per)[:-1]])
+        table["bin_duration"][:n_pieces] = info[col_s, 0]
+        table["period_offset"][:n_pieces] = col_p
+        table["n_periods"][:n_pieces] = n_per
+        table["out_offset"][:n_pieces] = out_off
+        table["width"][:n_pieces] = col_w
+        table["dur_offset"][:n_pieces] = info[col_s, 1].astype(np.int64)
+        table["n_dur"][:n_pieces] = info[col_s, 2].astype(np.int64)
+        table["curve"][:n_pieces] = col_c
+        table["group"][:n_pieces] = info[col_s, 3].astype(np.int64)
+        table["mode"][:n_pieces] = lin[col_s, 0].astype(np.int64)
+        table["lin_start"][:n_pieces] = lin[col_s, 1]
+        table["lin_step"][:n_pieces] = lin[col_s, 2]
+        table["lin_stop"][:n_pieces] = lin[col_s, 3]
+        # lin_num and lin_k0 are integers below 2^53: exact through float64
+        table["lin_num"][:n_pieces] = lin[col_s, 4].astype(np.int64)
+        table["lin_k0"][:n_pieces] = lin[col_s, 5].astype(np.int64) + col_a
+        out_off = int(n_per.sum())
+    else:
+        out_off = 0
+    # a curve's first column: where its first piece starts (its outputs are contiguous)
+    curve_base = np.concatenate([[0], np.cumsum(curve_count)[:-1]]).astype(np.int64) \
+        if len(plans) else np.zeros(0, dtype=np.int64)
+    segs = (_lib.Segment * table.size).from_buffer(table)
     return PackedBatch(
         tau=cs.tau, yw=cs.yw, ivar=cs.ivar, total_points=cs.total_points,
         curves=cs.curves32 if accumulation else cs.curves,
-        segments=segs, n_segments=len(pieces),
+        segments=segs, n_segments=n_pieces, segment_table=table,
         periods=_upload(np.concatenate(per_parts) if per_parts else np.zeros(1), np.float64, device),
         dur_bins=_upload(np.concatenate(dur_parts) if dur_parts else np.zeros(1), np.int32, device),
         n_out=out_off, curve_base=curve_base, curve_count=curve_count,
@@ -569,8 +676,7 @@ def pack(
 ) -> PackedBatch:
     """Host preparation of a chunk: curves uploaded once, every curve's segment plan."""
     cs = pack_curves(lcs)
-    plans = [plan_grid(cs, c, grid, settings) for c, grid in enumerate(grids)]
-    return pack_plans(cs, plans, settings)
+    return pack_plans(cs, plan_grids(cs, grids, settings), settings)
 
 
 def window_plan(base: CurvePlan, ranges: Sequence[tuple[int, int]]) -> CurvePlan:
