@@ -64,6 +64,7 @@ EXPORTED_SYMBOLS = (
 #: Every symbol the proposer header (include/cuperiod_proposer.h) declares.
 PROPOSER_SYMBOLS = (
     "cpp_features",
+    "cpp_stack_features",
     "cpp_layer1_forward",
     "cpp_conv_forward",
     "cpp_conv_wgrad",
@@ -223,7 +224,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "cpp_layer1_wgrad": [vp, vp, vp, i32, vp, vp],
         "cpp_flip_weights": [vp, vp, vp],
         "cpp_head_train": [vp, vp, vp, vp, vp, vp, i32, f32, vp, vp, vp, vp],
-        "cpp_head_windows": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp],
+        "cpp_head_windows": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp],
+        "cpp_stack_features": [vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, vp],
         "cpp_adam": [vp, vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32],
     }
     for name, args in proposer.items():

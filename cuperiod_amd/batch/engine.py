@@ -347,6 +347,8 @@ class CurveSet:
     times: list[np.ndarray]
     # the CURVE_DTYPE tables behind ``curves`` / ``curves32`` (kept alive with them)
     tables: tuple = ()
+    # every point's ivar on the host, concatenated (sums over runs of points)
+    host_ivar: np.ndarray | None = None
 
 
 @dataclass
@@ -374,6 +376,10 @@ class PackedBatch:
     accumulation: int = 0
     # the SEG_DTYPE table behind ``segments``
     segment_table: np.ndarray | None = None
+    # explicit periods on the host (what ``periods`` holds on the device)
+    periods_host: np.ndarray | None = None
+    # window batches: the full batch's column of each piece's first period
+    source_col: np.ndarray | None = None
 
 
 def _device():
@@ -454,7 +460,37 @@ def pack_curves(lcs: Sequence[tuple[np.ndarray, np.ndarray, np.ndarray]]) -> Cur
         total_points=off, curves=curve_array(table), curves32=curve_array(table32),
         t_min=t_min, baseline=baseline, n_points=n_points, point_base=point_base,
         times=times, tables=(table, table32),
+        host_ivar=np.concatenate(ivs) if ivs else np.zeros(0),
     )
+
+
+def sub_curves(cs: CurveSet, parent: np.ndarray, start: np.ndarray,
+               count: np.ndarray) -> CurveSet:
+    """Curves made of runs of ``cs``'s points, sharing its device arrays: curve ``j`` is
+    points ``start[j] .. start[j] + count[j] - 1`` of curve ``parent[j]`` (runs must not
+    overlap). Each is searched as those points alone (times measured from the parent's
+    origin, the parent's centred flux: BLS powers do not depend on a flux offset); the
+    reference rounding of the peak is not applied to them."""
+    parent = np.asarray(parent, dtype=np.int64)
+    start = np.asarray(start, dtype=np.int64)
+    count = np.asarray(count, dtype=np.int64)
+    n = parent.size
+    off = cs.point_base[parent] + start
+    csum = np.concatenate([[0.0], np.cumsum(cs.host_ivar)])
+    sum_iv = csum[off + count] - csum[off]
+    floor = 4.0 * count * core._F64_EPS * sum_iv
+    floor = np.where(floor > 0.0, floor, np.finfo(np.float64).tiny)
+    table = np.zeros(max(1, n), dtype=CURVE_DTYPE)
+    table["point_offset"][:n] = off
+    table["n_points"][:n] = count
+    table["t_min"][:n] = cs.t_min[parent]
+    table["ivar_floor"][:n] = floor
+    # sum_y = sum_ivar = 0: no reference rounding of these curves' peaks
+    curves = curve_array(table)
+    return CurveSet(tau=cs.tau, yw=cs.yw, ivar=cs.ivar, total_points=cs.total_points,
+                    curves=curves, curves32=curves, t_min=cs.t_min[parent],
+                    baseline=cs.baseline[parent], n_points=count, point_base=off,
+                    times=[], tables=(table,), host_ivar=None)
 
 
 def plan_grid(cs: CurveSet, c: int, grid: LinspaceGrid | GridSpec | None,
@@ -659,6 +695,7 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
         tau=cs.tau, yw=cs.yw, ivar=cs.ivar, total_points=cs.total_points,
         curves=cs.curves32 if accumulation else cs.curves,
         segments=segs, n_segments=n_pieces, segment_table=table,
+        periods_host=np.concatenate(per_parts) if per_parts else np.zeros(1),
         periods=_upload(np.concatenate(per_parts) if per_parts else np.zeros(1), np.float64, device),
         dur_bins=_upload(np.concatenate(dur_parts) if dur_parts else np.zeros(1), np.int32, device),
         n_out=out_off, curve_base=curve_base, curve_count=curve_count,
@@ -677,6 +714,80 @@ def pack(
     """Host preparation of a chunk: curves uploaded once, every curve's segment plan."""
     cs = pack_curves(lcs)
     return pack_plans(cs, plan_grids(cs, grids, settings), settings)
+
+
+def window_batch(pb: PackedBatch, curve: np.ndarray, k0: np.ndarray,
+                 k1: np.ndarray) -> PackedBatch:
+    """The part of packed batch ``pb`` at ascending-period indices ``[k0, k1)`` of
+    curve ``curve`` (one row per window; a curve's windows must not overlap), cut from
+    its descriptor table: every kept period keeps its piece's binning, box widths and
+    reference segment, so its outputs are those of the full search (up to the reference
+    rounding of a segment's peak, which follows the periods searched). ``source_col``
+    maps each new piece back to the full batch's columns."""
+    import dataclasses as dc
+
+    table = pb.segment_table[:pb.n_segments]
+    p_start = table["out_offset"].astype(np.int64)
+    p_end = p_start + table["n_periods"].astype(np.int64)
+    g0 = pb.curve_base[curve] + np.asarray(k0, dtype=np.int64)
+    g1 = pb.curve_base[curve] + np.asarray(k1, dtype=np.int64)
+    ok = g1 > g0
+    g0, g1 = g0[ok], g1[ok]
+    order = np.argsort(g0, kind="stable")
+    g0, g1 = g0[order], g1[order]
+    first = np.searchsorted(p_start, g0, side="right") - 1
+    last = np.searchsorted(p_start, g1 - 1, side="right") - 1
+    cnt = last - first + 1
+    w = np.repeat(np.arange(g0.size), cnt)
+    piece = first[w] + (np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+    lo = np.maximum(g0[w], p_start[piece])
+    hi = np.minimum(g1[w], p_end[piece])
+    keep = hi > lo
+    piece, lo, hi = piece[keep], lo[keep], hi[keep]
+    new = np.zeros(max(1, piece.size), dtype=SEG_DTYPE)
+    m = piece.size
+    if m:
+        new[:m] = table[piece]
+        shift = lo - p_start[piece]
+        n_per = hi - lo
+        explicit = new["mode"][:m] == 0
+        new["period_offset"][:m] += np.where(explicit, shift, 0)
+        new["lin_k0"][:m] += np.where(explicit, 0, shift)
+        new["n_periods"][:m] = n_per
+        new["out_offset"][:m] = np.concatenate([[0], np.cumsum(n_per)[:-1]])
+    n_curves = len(pb.plans)
+    counts = np.bincount(new["curve"][:m], weights=new["n_periods"][:m],
+                         minlength=n_curves).astype(np.int64)
+    bases = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    device = pb.curve_base_d.device
+    return dc.replace(
+        pb, segments=(_lib.Segment * new.size).from_buffer(new), n_segments=m,
+        segment_table=new, n_out=int(counts.sum()), curve_base=bases, curve_count=counts,
+        curve_base_d=_upload(bases, np.int64, device),
+        curve_count_d=_upload(counts, np.int64, device), source_col=lo.astype(np.int64))
+
+
+def periods_of_columns(pb: PackedBatch, cols: np.ndarray) -> np.ndarray:
+    """Periods of output columns ``cols`` of a packed batch, by the device's recipes."""
+    cols = np.asarray(cols, dtype=np.int64)
+    table = pb.segment_table[:pb.n_segments]
+    r = np.searchsorted(table["out_offset"], cols, side="right") - 1
+    row = table[r]
+    i_in = cols - row["out_offset"]
+    out = np.empty(cols.shape, dtype=np.float64)
+    explicit = row["mode"] == 0
+    if explicit.any():
+        out[explicit] = pb.periods_host[row["period_offset"][explicit] + i_in[explicit]]
+    par = ~explicit
+    if par.any():
+        rw = row[par]
+        num = rw["lin_num"].astype(np.int64)
+        i = num - 1 - (rw["lin_k0"].astype(np.int64) + i_in[par])
+        f = i.astype(np.float64) * rw["lin_step"] + rw["lin_start"]
+        f = np.where((rw["mode"] == 1) & (i == num - 1), rw["lin_stop"], f)
+        f = np.where((rw["mode"] == 2) & (i == 0), rw["lin_start"], f)
+        out[par] = 1.0 / f
+    return out
 
 
 def window_plan(base: CurvePlan, ranges: Sequence[tuple[int, int]]) -> CurvePlan:
@@ -887,6 +998,8 @@ def bls_batch(
     return finish(pb, out, summary, spectra=spectra)
 
 
-__all__ = ["BatchGraph", "BatchResult", "CurvePlan", "CurveSet", "LinspaceGrid", "PackedBatch",
-           "PeakCandidates", "bls_batch", "capture", "peak_candidates",
-           "finish", "pack", "pack_curves", "pack_plans", "plan_grid", "run_packed", "window_plan"]
+__all__ = ["BatchGraph", "BatchResult", "CURVE_DTYPE", "CurvePlan", "CurveSet", "LinspaceGrid",
+           "PackedBatch", "PeakCandidates", "SEG_DTYPE", "bls_batch", "capture", "curve_array",
+           "finish", "pack", "pack_curves", "pack_plans", "peak_candidates",
+           "periods_of_columns", "plan_grid", "plan_grids", "run_packed", "sub_curves",
+           "window_batch", "window_plan"]

@@ -110,6 +110,63 @@ __global__ __launch_bounds__(256) void features_kernel(
     }
 }
 
+// Stacked features: the coarse power of each curve summed over its time chunks, one
+// workgroup a curve. Chunk k of curve c has its coarse outputs at out_base[k] (ascending
+// period), so cell x reads power[out_base[k] + L-1-x]; S[x] = sum_k snr_k(x)^2 over the
+// chunks (a power that is not finite and positive counts 0). Channel 0 is S standardised
+// over the curve's cells and clipped; channel 1 the cadence histogram, as features_kernel.
+__global__ __launch_bounds__(256) void stack_features_kernel(
+    const double* __restrict__ power, const int32_t* __restrict__ chunk_start,
+    const int64_t* __restrict__ out_base, const double* __restrict__ tau,
+    const int64_t* __restrict__ point_base, const int32_t* __restrict__ n_points,
+    const double* __restrict__ baseline, float clip, float* __restrict__ x) {
+    __shared__ int hist[CPP_HIST_BINS];
+    __shared__ float sv[kL];
+    __shared__ double red[4];
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < CPP_HIST_BINS; i += blockDim.x) hist[i] = 0;
+    const int k0 = chunk_start[c], k1 = chunk_start[c + 1];
+    double part = 0.0;
+    for (int i = tid; i < kL; i += blockDim.x) {
+        double acc = 0.0;
+        for (int k = k0; k < k1; ++k) {
+            const double v = power[out_base[k] + (kL - 1 - i)];
+            if (isfinite(v) && v > 0.0) acc += v * v;
+        }
+        sv[i] = (float)acc;
+        part += acc;
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if (lane == 0) red[wave] = part;
+    __syncthreads();
+    const double mean = (red[0] + red[1] + red[2] + red[3]) / kL;
+    __syncthreads();
+    double q = 0.0;
+    for (int i = tid; i < kL; i += blockDim.x) q += ((double)sv[i] - mean) * ((double)sv[i] - mean);
+    for (int o = 32; o > 0; o >>= 1) q += __shfl_xor(q, o);
+    if (lane == 0) red[wave] = q;
+    const int n = n_points[c];
+    const double base = baseline[c];
+    const double scale = base > 0.0 ? CPP_HIST_BINS / base : 0.0;
+    const double* t = tau + point_base[c];
+    for (int j = tid; j < n; j += blockDim.x) {
+        int b = (int)(t[j] * scale);
+        b = b < 0 ? 0 : (b >= CPP_HIST_BINS ? CPP_HIST_BINS - 1 : b);
+        atomicAdd(&hist[b], 1);
+    }
+    __syncthreads();
+    const double sd = sqrt((red[0] + red[1] + red[2] + red[3]) / kL);
+    const float hs = n > 0 ? (float)CPP_HIST_BINS / (float)n : 0.0f;
+    float* xc = x + (size_t)c * kL * 2;
+    for (int i = tid; i < kL; i += blockDim.x) {
+        float z = sd > 0.0 ? (float)(((double)sv[i] - mean) / sd) : 0.0f;
+        z = fminf(fmaxf(z, -clip), clip);
+        xc[2 * i] = z;
+        xc[2 * i + 1] = (float)hist[i >> 3] * hs;
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Layer 1 (2 -> 64 channels) on the vector ALUs: one thread per position.
 
@@ -439,7 +496,7 @@ __global__ __launch_bounds__(256) void head_windows_kernel(
     const uint16_t* __restrict__ h4, const float* __restrict__ wh, const float* __restrict__ bh,
     const double* __restrict__ f_lo, const double* __restrict__ dlog,
     const double* __restrict__ half_width, int n_windows, int radius,
-    int32_t* __restrict__ centre, float* __restrict__ mass) {
+    int32_t* __restrict__ centre, float* __restrict__ mass, int32_t* __restrict__ span) {
     __shared__ float swh[kC];
     __shared__ float p[kL];
     __shared__ unsigned char avail[kL];
@@ -501,6 +558,10 @@ __global__ __launch_bounds__(256) void head_windows_kernel(
             if (tid == 0) {
                 centre[c * n_windows + w] = -1;
                 mass[c * n_windows + w] = 0.0f;
+                if (span) {
+                    span[2 * (c * n_windows + w)] = -1;
+                    span[2 * (c * n_windows + w) + 1] = -1;
+                }
             }
             continue;
         }
@@ -525,6 +586,10 @@ __global__ __launch_bounds__(256) void head_windows_kernel(
         if (tid == 0) {
             centre[c * n_windows + w] = gi;
             mass[c * n_windows + w] = part;
+            if (span) {
+                span[2 * (c * n_windows + w)] = lo;
+                span[2 * (c * n_windows + w) + 1] = hi;
+            }
         }
         __syncthreads();
     }
@@ -561,6 +626,19 @@ int cpp_features(void* stream, const double* power, const int64_t* out_base,
         return CPB_E_INVALID;
     hipLaunchKernelGGL(features_kernel, dim3(n_curves), dim3(256), 0, (hipStream_t)stream,
                        power, out_base, summary, tau, point_base, n_points, baseline, clip, x);
+    return launch_status();
+}
+
+int cpp_stack_features(void* stream, const double* power, const int32_t* chunk_start,
+                       const int64_t* out_base, const double* tau, const int64_t* point_base,
+                       const int32_t* n_points, const double* baseline, int32_t n_curves,
+                       float clip, float* x) {
+    if (n_curves <= 0 || !power || !chunk_start || !out_base || !tau || !point_base ||
+        !n_points || !baseline || !x || !(clip > 0.0f))
+        return CPB_E_INVALID;
+    hipLaunchKernelGGL(stack_features_kernel, dim3(n_curves), dim3(256), 0, (hipStream_t)stream,
+                       power, chunk_start, out_base, tau, point_base, n_points, baseline, clip,
+                       x);
     return launch_status();
 }
 
@@ -634,12 +712,12 @@ int cpp_head_train(void* stream, const uint16_t* h4, const uint16_t* z4, const f
 int cpp_head_windows(void* stream, const uint16_t* h4, const float* wh, const float* bh,
                      const double* f_lo, const double* dlog, const double* half_width,
                      int32_t n_curves, int32_t n_windows, int32_t radius, int32_t* centre,
-                     float* mass) {
+                     float* mass, int32_t* span) {
     if (n_curves <= 0 || n_windows <= 0 || n_windows > kL || radius < 0 || radius >= kL || !h4 ||
         !wh || !bh || !f_lo || !dlog || !half_width || !centre || !mass)
         return CPB_E_INVALID;
     hipLaunchKernelGGL(head_windows_kernel, dim3(n_curves), dim3(256), 0, (hipStream_t)stream, h4,
-                       wh, bh, f_lo, dlog, half_width, n_windows, radius, centre, mass);
+                       wh, bh, f_lo, dlog, half_width, n_windows, radius, centre, mass, span);
     return launch_status();
 }
 

@@ -147,20 +147,23 @@ class ProposerNet:
         return a
 
     def windows(self, x, n_windows: int, radius: int, f_lo, dlog, half_width):
-        """``(centre [n, W] int32, mass [n, W] FP32)`` device tensors for inputs ``x``;
-        ``f_lo``, ``dlog``, ``half_width`` are per-curve FP64 device tensors (cell x has
-        frequency ``f_lo * exp(x * dlog)``; a window spans ``+-half_width``)."""
+        """``(centre [n, W] int32, mass [n, W] FP32, span [n, W, 2] int32)`` device tensors
+        for inputs ``x``; ``f_lo``, ``dlog``, ``half_width`` are per-curve FP64 device
+        tensors (cell x has frequency ``f_lo * exp(x * dlog)``; a window claims the cells
+        within ``half_width`` of its centre, at least ``radius`` each side, and ``span``
+        holds the first and last of them)."""
         import torch
 
         n = int(x.shape[0])
         a = self.forward(x)
         centre = torch.empty((n, n_windows), dtype=torch.int32, device=self.device)
         mass = torch.empty((n, n_windows), dtype=torch.float32, device=self.device)
+        span = torch.empty((n, n_windows, 2), dtype=torch.int32, device=self.device)
         _check(self.lib.cpp_head_windows(_stream(self.device), _ptr(a.h[3]), _ptr(self.p["wh"]),
                                          _ptr(self.p["bh"]), _ptr(f_lo), _ptr(dlog),
                                          _ptr(half_width), n, int(n_windows), int(radius),
-                                         _ptr(centre), _ptr(mass)), "head")
-        return centre, mass
+                                         _ptr(centre), _ptr(mass), _ptr(span)), "head")
+        return centre, mass, span
 
     def backward(self, x, a: Activations, labels, scale: float) -> None:
         """Accumulate the gradients of ``scale * sum(cross-entropy)`` into ``self.g``;

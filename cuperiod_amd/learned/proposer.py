@@ -2,29 +2,42 @@
 
 For each light curve on a linspace frequency grid (the plain, full-grid search):
 
-1. a coarse BLS spectrum at 4,096 log-spaced frequencies spanning the grid (the same
-   engine, period segments and durations as the plain search);
-2. features: the coarse power standardised per curve and a 512-bin cadence histogram
-   (``cpp_features``);
-3. the network scores every coarse cell; greedy windows are taken around the most
-   probable cells (``cpp_head_windows``) and kept, most probable first, until their
-   probability mass reaches ``window_mass`` or ``max_windows`` are kept;
+1. a coarse BLS spectrum at L = 4,096 log-spaced frequencies spanning the grid, stacked
+   over time chunks. A BLS peak of a curve spanning T days is about duration/T wide in
+   relative frequency, far narrower than the coarse grid's spacing ``dlog`` for the
+   baselines the plain grid is sized for, so a whole-curve coarse spectrum misses most
+   peaks between its samples. The curve is cut into runs of points at most
+   ``chunk_duration / dlog`` days long, each run is searched on the coarse grid (same
+   engine, the plain search's segment durations, ``coarse_oversample`` bins per
+   duration), and the chunks' SNR^2 are summed per coarse frequency: each chunk's peak
+   spans about one coarse cell, and the stack keeps most of the transit's signal;
+2. features: the stacked power standardised per curve and a 512-bin cadence histogram
+   (``cpp_stack_features``);
+3. the network scores every coarse cell as the one nearest the plain search's best
+   frequency (it is trained on that cell +-1); greedy windows are taken around the most
+   probable cells (``cpp_head_windows``), each claiming its centre +-``radius`` cells,
+   and kept, most probable first, until their probability mass reaches ``window_mass``
+   or ``max_windows`` are kept;
 4. a curve whose kept mass is below ``threshold`` falls back to the full grid; every
-   other curve is searched only at the full grid's frequencies within
-   ``fine_per_window`` grid steps around each kept window (2,000 by default).
+   other curve is searched at the full grid's frequencies within ``fine_per_window / 2``
+   grid steps of every cell its kept windows claim. That half-width exceeds half a coarse
+   cell on the grids the proposer serves, so a claimed cell that is the nearest one to
+   the best frequency puts the best frequency inside the refined range.
 
 Refinement evaluates the plain grid's own frequencies with the plain search's segment
-binning, so a refined curve's best period is the plain best period whenever a window
-covers it. Refined and fallback curves share one engine launch.
+binning (the full search's descriptors, cut to the windows), so a refined curve's best
+period is the plain best period whenever a window covers it. Refined and fallback curves
+share one engine launch.
 """
 
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 from dataclasses import dataclass, field
 from pathlib import Path
-from typing import Any, Sequence
+from typing import Sequence
 
 import numpy as np
 
@@ -34,10 +47,13 @@ from cuperiod_amd.core.config import BLSSettings
 from cuperiod_amd.learned.network import L, ProposerNet, load_parameters
 from cuperiod_amd.methods.bls import _log_segments
 
-#: Weights trained on synthetic survey curves (see ``cuperiod_amd.learned.train``).
-DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "weights" / "proposer_survey.npz"
-#: Coarse standardised power is clipped to +-FEATURE_CLIP.
+#: Weights trained on synthetic gapped curves labelled by the plain search
+#: (``cuperiod_amd.learned.train``).
+DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "weights" / "proposer_gapped.npz"
+#: Stacked coarse power is standardised per curve and clipped to +-FEATURE_CLIP.
 FEATURE_CLIP = 30.0
+#: Chunks with fewer points are left out of the coarse stack.
+MIN_CHUNK_POINTS = 16
 
 
 def coarse_frequencies(f_lo: float, f_hi: float) -> np.ndarray:
@@ -76,6 +92,15 @@ def window_geometry(grids, fine_per_window: int):
     return f_lo, dlog, half
 
 
+def label_cells(freq: np.ndarray, f_lo: np.ndarray, dlog: np.ndarray,
+                radius: int = 1) -> np.ndarray:
+    """``[n, 2]`` training targets: the coarse cell nearest each frequency, +-``radius``
+    cells (clipped to the grid)."""
+    near = np.rint(np.log(freq / f_lo) / dlog)
+    out = np.stack([np.clip(near - radius, 0, L - 1), np.clip(near + radius, 0, L - 1)], axis=1)
+    return out.astype(np.int32)
+
+
 def label_ranges(freq: np.ndarray, f_lo: np.ndarray, dlog: np.ndarray,
                  half: np.ndarray) -> np.ndarray:
     """``[n, 2]`` cell ranges whose frequencies lie within ``half`` of ``freq`` (at least
@@ -90,56 +115,28 @@ def label_ranges(freq: np.ndarray, f_lo: np.ndarray, dlog: np.ndarray,
     return out.astype(np.int32)
 
 
-def features(pb: engine.PackedBatch, out, summary):
-    """``[n, L, 2]`` FP32 device features of a coarse search's outputs."""
-    import torch
-
-    cs = pb.curve_set
-    device = out.device
-    n = len(pb.plans)
-    x = torch.empty((n, L, 2), dtype=torch.float32, device=device)
-    if np.any(pb.curve_count != L):
-        raise ValueError("features need exactly L coarse periods per curve")
-    pbase = torch.from_numpy(cs.point_base).to(device)
-    npts = torch.from_numpy(cs.n_points.astype(np.int32)).to(device)
-    base = torch.from_numpy(cs.baseline).to(device)
-    import ctypes
-
-    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
-    lib = _lib.load_library()
-    st = lib.cpp_features(stream, ctypes.c_void_p(out.data_ptr()),
-                          ctypes.c_void_p(pb.curve_base_d.data_ptr()),
-                          ctypes.c_void_p(summary.data_ptr()), ctypes.c_void_p(cs.tau.data_ptr()),
-                          ctypes.c_void_p(pbase.data_ptr()), ctypes.c_void_p(npts.data_ptr()),
-                          ctypes.c_void_p(base.data_ptr()), n, FEATURE_CLIP,
-                          ctypes.c_void_p(x.data_ptr()))
-    if st != _lib.CPB_OK:
-        raise _lib.status_error(st, f"proposer features: status {st}")
-    return x
-
-
-def window_ranges(centres: np.ndarray, grid: engine.LinspaceGrid, f_lo: float, f_hi: float,
-                  half: int) -> list[tuple[int, int]]:
-    """Merged ascending-period index ranges of the full grid within ``half`` grid steps
-    of each coarse window centre."""
-    if centres.size == 0:
-        return []
-    freq = coarse_frequencies(f_lo, f_hi)[centres]
-    step = (grid.stop - grid.start) / (grid.num - 1)
-    i = np.rint((freq - grid.start) / step).astype(np.int64)
-    k = grid.num - 1 - i  # ascending-period index
-    lo = np.clip(k - half, 0, grid.num)
-    hi = np.clip(k + half, 0, grid.num)
-    order = np.argsort(lo)
-    ranges: list[tuple[int, int]] = []
-    for a, b in zip(lo[order], hi[order]):
-        if b <= a:
+def time_chunks(cs: engine.CurveSet, max_span: np.ndarray):
+    """``(parent, start, count)`` of the runs of each curve's points: consecutive points
+    spanning at most ``max_span[c]`` days, runs of fewer than MIN_CHUNK_POINTS dropped.
+    A curve whose times are not sorted is one run."""
+    parents, starts, counts = [], [], []
+    for c, t in enumerate(cs.times):
+        tau = t - t[0]
+        if tau.size > 1 and np.any(np.diff(tau) < 0):
+            parents.append(np.array([c], dtype=np.int64))
+            starts.append(np.array([0], dtype=np.int64))
+            counts.append(np.array([tau.size], dtype=np.int64))
             continue
-        if ranges and a <= ranges[-1][1]:
-            ranges[-1] = (ranges[-1][0], max(ranges[-1][1], int(b)))
-        else:
-            ranges.append((int(a), int(b)))
-    return ranges
+        edges = [0]
+        while edges[-1] < tau.size:
+            edges.append(int(np.searchsorted(tau, tau[edges[-1]] + max_span[c], side="right")))
+        e = np.asarray(edges, dtype=np.int64)
+        n = np.diff(e)
+        keep = n >= MIN_CHUNK_POINTS
+        parents.append(np.full(int(keep.sum()), c, dtype=np.int64))
+        starts.append(e[:-1][keep])
+        counts.append(n[keep])
+    return np.concatenate(parents), np.concatenate(starts), np.concatenate(counts)
 
 
 @dataclass
@@ -156,39 +153,113 @@ class LearnedResult:
 
 @dataclass
 class PreparedGrids:
-    """What depends only on the curves and grids: full and coarse plans, the coarse
-    search's packed descriptors."""
+    """What depends only on the curves and grids: the plain search's packed batch (the
+    fallback, and the descriptors refinement is cut from), the chunked coarse search and
+    the window geometry."""
 
     cs: engine.CurveSet
     grids: list
-    full: list
+    full: engine.PackedBatch
     coarse: engine.PackedBatch
+    chunk_start: object = None
+    chunk_base: object = None
     geometry: tuple = ()
+    geometry_host: tuple = ()
+    n_chunks: int = 0
 
 
 def prepare_grids(cs: engine.CurveSet, grids: Sequence[engine.LinspaceGrid],
-                  settings: BLSSettings, *, full: list | None = None,
-                  fine_per_window: int = 2000) -> PreparedGrids:
-    """Full-grid plans (built here unless given), the coarse search's packed
-    descriptors and the window geometry of a batch."""
+                  settings: BLSSettings, *, full: engine.PackedBatch | None = None,
+                  fine_per_window: int = 2000, chunk_duration: float = 0.2,
+                  coarse_oversample: int = 3) -> PreparedGrids:
+    """The plain search's packed batch (built here unless given), the chunked coarse
+    search's packed descriptors and the window geometry of a batch."""
     import torch
 
     grids = list(grids)
     if full is None:
-        full = [engine.plan_grid(cs, c, g, settings) for c, g in enumerate(grids)]
-    coarse = engine.pack_plans(cs, [coarse_plan(c, g, settings) for c, g in enumerate(grids)],
-                               settings)
+        full = engine.pack_plans(cs, engine.plan_grids(cs, grids, settings), settings)
+    f_lo, dlog, half = window_geometry(grids, fine_per_window)
+    parent, start, count = time_chunks(cs, chunk_duration / dlog)
+    chunks = engine.sub_curves(cs, parent, start, count)
+    coarse_settings = settings.model_copy(update={"bins_per_duration": int(coarse_oversample)})
+    # one coarse plan per distinct grid, shared by its curves' chunks (the packer then
+    # sends each plan's periods once)
+    by_grid: dict[tuple[float, float], engine.CurvePlan] = {}
+    plans = []
+    for c, g in enumerate(grids):
+        key = (float(g.start), float(g.stop))
+        if key not in by_grid:
+            by_grid[key] = coarse_plan(c, g, settings)
+        plans.append(by_grid[key])
+    coarse = engine.pack_plans(chunks, [plans[p] for p in parent.tolist()], coarse_settings)
+    chunk_start = np.searchsorted(parent, np.arange(len(grids) + 1)).astype(np.int32)
     dev = cs.tau.device
-    geometry = tuple(torch.from_numpy(a).to(dev) for a in window_geometry(grids, fine_per_window))
-    return PreparedGrids(cs=cs, grids=grids, full=full, coarse=coarse, geometry=geometry)
+    # windows claim their centre +-radius cells (no claim width); refinement adds the
+    # half-width around the claimed cells (refine_ranges)
+    geometry = tuple(torch.from_numpy(a).to(dev) for a in (f_lo, dlog, np.zeros_like(half)))
+    return PreparedGrids(cs=cs, grids=grids, full=full, coarse=coarse,
+                         chunk_start=torch.from_numpy(chunk_start).to(dev),
+                         chunk_base=coarse.curve_base_d, geometry=geometry,
+                         geometry_host=(f_lo, dlog, half), n_chunks=int(parent.size))
+
+
+def stacked_features(prep: PreparedGrids, out):
+    """``[n, L, 2]`` FP32 device features from a coarse search's outputs."""
+    import torch
+
+    cs = prep.cs
+    device = out.device
+    n = len(prep.grids)
+    x = torch.empty((n, L, 2), dtype=torch.float32, device=device)
+    pbase = torch.from_numpy(cs.point_base).to(device)
+    npts = torch.from_numpy(cs.n_points.astype(np.int32)).to(device)
+    base = torch.from_numpy(cs.baseline).to(device)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    lib = _lib.load_library()
+    st = lib.cpp_stack_features(stream, ctypes.c_void_p(out.data_ptr()),
+                                ctypes.c_void_p(prep.chunk_start.data_ptr()),
+                                ctypes.c_void_p(prep.chunk_base.data_ptr()),
+                                ctypes.c_void_p(cs.tau.data_ptr()),
+                                ctypes.c_void_p(pbase.data_ptr()),
+                                ctypes.c_void_p(npts.data_ptr()),
+                                ctypes.c_void_p(base.data_ptr()), n, FEATURE_CLIP,
+                                ctypes.c_void_p(x.data_ptr()))
+    if st != _lib.CPB_OK:
+        raise _lib.status_error(st, f"proposer features: status {st}")
+    return x
+
+
+def coarse_features(prep: PreparedGrids):
+    """Run the chunked coarse search and return the network's input features."""
+    out, _ = engine.run_packed(prep.coarse)
+    return stacked_features(prep, out)
+
+
+def refine_ranges(prep: PreparedGrids, span_lo: np.ndarray, span_hi: np.ndarray,
+                  curve: np.ndarray):
+    """Ascending-period index ranges ``[k0, k1)`` of the full grid of ``curve`` that hold
+    every frequency within the refinement half-width of the cells ``span_lo ..
+    span_hi`` (one window per row)."""
+    f_lo, dlog, half = (a[curve] for a in prep.geometry_host)
+    start = np.array([g.start for g in prep.grids])[curve]
+    stop = np.array([g.stop for g in prep.grids])[curve]
+    num = np.array([g.num for g in prep.grids], dtype=np.int64)[curve]
+    step = (stop - start) / (num - 1)
+    fa = f_lo * np.exp(span_lo * dlog) - half
+    fb = f_lo * np.exp(span_hi * dlog) + half
+    i_lo = np.clip(np.floor((fa - start) / step) - 1, 0, num - 1).astype(np.int64)
+    i_hi = np.clip(np.ceil((fb - start) / step) + 1, 0, num - 1).astype(np.int64)
+    return num - 1 - i_hi, num - i_lo
 
 
 class LearnedProposer:
-    """Coarse search, network windows, refinement and fallback over a batch."""
+    """Chunked coarse search, network windows, refinement and fallback over a batch."""
 
     def __init__(self, params: dict[str, np.ndarray] | None = None, *, max_windows: int = 32,
                  radius: int = 1, fine_per_window: int = 2000, window_mass: float = 0.995,
-                 threshold: float = 0.9) -> None:
+                 threshold: float = 0.9, chunk_duration: float = 0.2,
+                 coarse_oversample: int = 3) -> None:
         if params is None:
             params = load_parameters(DEFAULT_WEIGHTS)
         self.net = ProposerNet(params)
@@ -197,65 +268,95 @@ class LearnedProposer:
         self.fine_per_window = int(fine_per_window)
         self.window_mass = float(window_mass)
         self.threshold = float(threshold)
+        self.chunk_duration = float(chunk_duration)
+        self.coarse_oversample = int(coarse_oversample)
 
     def prepare(self, cs: engine.CurveSet, grids: Sequence[engine.LinspaceGrid],
-                settings: BLSSettings, full: list | None = None) -> PreparedGrids:
+                settings: BLSSettings, full: engine.PackedBatch | None = None) -> PreparedGrids:
         """Plans that do not depend on the network's output (built once per batch)."""
         return prepare_grids(cs, grids, settings, full=full,
-                             fine_per_window=self.fine_per_window)
+                             fine_per_window=self.fine_per_window,
+                             chunk_duration=self.chunk_duration,
+                             coarse_oversample=self.coarse_oversample)
 
     def propose(self, prep: PreparedGrids):
-        """Coarse search, features and windows: ``(centre, mass)`` host arrays."""
-        out, summary = engine.run_packed(prep.coarse)
-        x = features(prep.coarse, out, summary)
-        centre, mass = self.net.windows(x, self.max_windows, self.radius, *prep.geometry)
-        return centre.cpu().numpy(), mass.cpu().numpy()
+        """Coarse search, features and windows: ``(centre, mass, span)`` host arrays."""
+        x = coarse_features(prep)
+        centre, mass, span = self.net.windows(x, self.max_windows, self.radius, *prep.geometry)
+        return centre.cpu().numpy(), mass.cpu().numpy(), span.cpu().numpy()
 
     def select(self, centre: np.ndarray, mass: np.ndarray):
-        """Windows kept per curve (most probable first) and each curve's kept mass."""
-        kept: list[np.ndarray] = []
-        conf = np.zeros(centre.shape[0])
-        for c in range(centre.shape[0]):
-            ok = centre[c] >= 0
-            cen, m = centre[c][ok], mass[c][ok]
-            order = np.argsort(-m, kind="stable")
-            cum = np.cumsum(m[order])
-            n_keep = int(np.searchsorted(cum, self.window_mass) + 1) if cum.size else 0
-            n_keep = min(n_keep, cum.size)
-            kept.append(cen[order[:n_keep]])
-            conf[c] = float(cum[n_keep - 1]) if n_keep else 0.0
-        return kept, conf
+        """Per window, whether it is kept (most probable first until ``window_mass``), and
+        each curve's kept mass."""
+        m = np.where(centre >= 0, mass, 0.0).astype(np.float64)
+        order = np.argsort(-m, axis=1, kind="stable")
+        ms = np.take_along_axis(m, order, axis=1)
+        cum = np.cumsum(ms, axis=1)
+        # windows before the one that reaches window_mass, plus that one
+        before = np.concatenate([np.zeros((m.shape[0], 1)), cum[:, :-1]], axis=1)
+        keep_sorted = (before < self.window_mass) & (ms > 0.0)
+        keep = np.zeros_like(keep_sorted)
+        np.put_along_axis(keep, order, keep_sorted, axis=1)
+        conf = np.where(keep, m, 0.0).sum(axis=1)
+        return keep, conf
 
     def search(self, prep: PreparedGrids, settings: BLSSettings) -> LearnedResult:
         """Learned proposal + refinement, fallback below ``threshold``."""
         import torch
 
         t0 = time.perf_counter()
-        centre, mass = self.propose(prep)
+        centre, mass, span = self.propose(prep)
         t1 = time.perf_counter()
-        kept, conf = self.select(centre, mass)
+        keep, conf = self.select(centre, mass)
         fallback = conf < self.threshold
-        plans = []
-        for c, grid in enumerate(prep.grids):
-            if fallback[c]:
-                plans.append(prep.full[c])
-                continue
-            ranges = window_ranges(kept[c], grid, grid.start, grid.stop, self.fine_per_window // 2)
-            plans.append(engine.window_plan(prep.full[c], ranges))
-        pb = engine.pack_plans(prep.cs, plans, settings)
+        n = len(prep.grids)
+        rows, cols = np.nonzero(keep & ~fallback[:, None])
+        k0, k1 = refine_ranges(prep, span[rows, cols, 0], span[rows, cols, 1], rows)
+        fb = np.flatnonzero(fallback)
+        curve = np.concatenate([rows, fb])
+        k0 = np.concatenate([k0, np.zeros(fb.size, dtype=np.int64)])
+        k1 = np.concatenate([k1, prep.full.curve_count[fb]])
+        curve, k0, k1 = _merge_ranges(curve, k0, k1)
+        wb = engine.window_batch(prep.full, curve, k0, k1)
         t2 = time.perf_counter()
-        out, summary = engine.run_packed(pb)
-        res = engine.finish(pb, out, summary)
+        out, summary = engine.run_packed(wb)
+        s = summary.cpu().numpy()
         torch.cuda.synchronize()
+        best_col = wb.curve_base + s[:, 2].astype(np.int64)
+        r = np.searchsorted(wb.segment_table["out_offset"][:wb.n_segments], best_col,
+                            side="right") - 1
+        full_col = wb.source_col[r] + (best_col - wb.segment_table["out_offset"][r])
+        best_period = engine.periods_of_columns(prep.full, full_col)
+        best_period = np.where(wb.curve_count > 0, best_period, np.nan)
         t3 = time.perf_counter()
         return LearnedResult(
-            best_period=res.best_period, best_power=res.best_power, fallback=fallback,
-            confidence=conf, n_fine=pb.curve_count.copy(),
+            best_period=best_period, best_power=s[:, 3].copy(), fallback=fallback,
+            confidence=conf, n_fine=wb.curve_count.copy(),
             timings={"propose": t1 - t0, "plan": t2 - t1, "refine": t3 - t2},
         )
 
 
+def _merge_ranges(curve: np.ndarray, k0: np.ndarray, k1: np.ndarray):
+    """Per curve, the union of ranges ``[k0, k1)`` as disjoint sorted ranges:
+    ``(curve, k0, k1)``."""
+    order = np.lexsort((k0, curve))
+    curve, k0, k1 = curve[order], k0[order], k1[order]
+    if curve.size == 0:
+        return curve, k0, k1
+    # running maximum of the range ends within each curve
+    ends = k1.copy()
+    start_of = np.concatenate([[True], curve[1:] != curve[:-1]])
+    grp = np.cumsum(start_of) - 1
+    for_max = ends + grp.astype(np.int64) * (int(k1.max()) + 1)
+    run_max = np.maximum.accumulate(for_max) - grp.astype(np.int64) * (int(k1.max()) + 1)
+    prev_end = np.concatenate([[-1], run_max[:-1]])
+    new = start_of | (k0 > prev_end)
+    idx = np.flatnonzero(new)
+    last = np.concatenate([idx[1:], [curve.size]]) - 1
+    return curve[idx], k0[idx], run_max[last]
+
+
 __all__ = ["DEFAULT_WEIGHTS", "LearnedProposer", "LearnedResult", "PreparedGrids",
-           "coarse_frequencies", "coarse_plan", "coarse_position", "features", "label_ranges",
-           "prepare_grids",
-           "window_geometry", "window_ranges"]
+           "coarse_features", "coarse_frequencies", "coarse_plan", "coarse_position",
+           "label_cells", "label_ranges", "prepare_grids", "refine_ranges", "stacked_features",
+           "time_chunks", "window_geometry"]

@@ -14,7 +14,10 @@ of the list below:
    log-uniform in [0.5, 20] d, depth log-uniform in [5e-4, 1e-2], duration/period
    uniform in [0.01, 0.05]); 200,000 frequencies in [1/baseline, 2] /d per curve;
    6 period-scaled durations per log segment;
-4. ``proposer_batch``: like 3 with 16,384 curves (the learned proposer's workload);
+4. ``learned_batch``: 16,384 curves of the gapped-curve recipe (2.), each with its own
+   transit period (log-uniform in [2, 20] d) and phase, depth 1.5e-3 and duration 0.2 d;
+   200,000 frequencies in [1/1,000, 2] /d; 6 period-scaled durations per log segment
+   (the learned proposer's workload; its training curves use other seeds);
 5. ``long_baseline_batch``: 256 curves, N = 100,000 over 10,000 d offset by 2.45e6 d, P = 0.7 d, depth 1e-3,
    duration 0.05 d; 4,000,000 frequencies in [1e-4, 10] /d; 16 durations;
 6. ``multiband_survey``: stars of 6 bands x 150 points each over 1,095 d (random
@@ -118,9 +121,30 @@ def survey_batch(n_curves: int = 4096, n_freq: int = 200_000) -> Workload:
     return Workload("survey_batch", curves, grids, n_durations=6, truth=truth)
 
 
-def proposer_batch(n_curves: int = 16384, n_freq: int = 200_000) -> Workload:
-    curves, grids, truth = survey_curves(n_curves, SEED + 4, n_freq)
-    return Workload("proposer_batch", curves, grids, n_durations=6, truth=truth)
+def gapped_curves(n_curves: int, seed: int, n_freq: int = 200_000):
+    """``(curves, grids, periods)``: gapped-curve recipe curves, N = 20,000 uniform over
+    1,000 d with the first 30% of every 365.25 d season removed, sigma 2e-3, a 0.2 d box
+    transit of depth 1.5e-3 at a period log-uniform in [2, 20] d and a uniform phase;
+    ``np.linspace(1/1000, 2, n_freq)`` frequencies each."""
+    from cuperiod_amd.batch.engine import LinspaceGrid
+
+    rng = np.random.default_rng(seed)
+    curves, grids, periods = [], [], []
+    for _ in range(n_curves):
+        t = np.sort(rng.uniform(0.0, 1000.0, 20_000))
+        t = t[np.mod(t / 365.25, 1.0) >= 0.3]
+        y = 1.0 + rng.normal(0.0, 2e-3, t.size)
+        p = float(np.exp(rng.uniform(np.log(2.0), np.log(20.0))))
+        y[_box_dip(t, p, float(rng.uniform(0.0, p)), 0.2)] -= 1.5e-3
+        curves.append((t, y, np.full(t.size, 2e-3)))
+        grids.append(LinspaceGrid(1.0 / 1000.0, 2.0, n_freq))
+        periods.append(p)
+    return curves, grids, periods
+
+
+def learned_batch(n_curves: int = 16384, n_freq: int = 200_000) -> Workload:
+    curves, grids, truth = gapped_curves(n_curves, SEED + 4, n_freq)
+    return Workload("learned_batch", curves, grids, n_durations=6, truth=truth)
 
 
 def long_baseline_batch(n_curves: int = 256, n_freq: int = 4_000_000, n_points: int = 100_000) -> Workload:
@@ -206,5 +230,5 @@ def published_curve():
 
 
 __all__ = ["SEED", "Workload", "gapped_curve", "long_baseline_batch", "multiband_survey",
-           "published_curve",
-           "proposer_batch", "small_curve", "survey_batch", "survey_curves", "tls_survey"]
+           "published_curve", "gapped_curves", "learned_batch",
+           "small_curve", "survey_batch", "survey_curves", "tls_survey"]

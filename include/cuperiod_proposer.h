@@ -43,6 +43,17 @@ int cpp_features(void* stream, const double* power, const int64_t* out_base,
                  const int32_t* n_points, const double* baseline, int32_t n_curves,
                  float clip, float* x);
 
+/* Stacked features of `n_curves` curves into x[B][L][2] (FP32). Curve c's time chunks
+ * are chunks chunk_start[c] .. chunk_start[c+1]-1, each searched on the curve's coarse
+ * grid with its outputs at out_base[k] (ascending period, so cell x reads
+ * power[out_base[k] + L-1-x]). Channel 0: S[x] = sum over the chunks of power^2 (a power
+ * that is not finite and positive counts 0), standardised over the curve's L cells and
+ * clipped to [-clip, clip]; channel 1: the cadence histogram, as cpp_features. */
+int cpp_stack_features(void* stream, const double* power, const int32_t* chunk_start,
+                       const int64_t* out_base, const double* tau, const int64_t* point_base,
+                       const int32_t* n_points, const double* baseline, int32_t n_curves,
+                       float clip, float* x);
+
 /* h1 = GELU(conv7(x, W1) + b1) (and z1 = the pre-activation, when z1 is not NULL). */
 int cpp_layer1_forward(void* stream, const float* x, const float* w1, const float* b1,
                        int32_t n_curves, uint16_t* z1, uint16_t* h1);
@@ -81,12 +92,14 @@ int cpp_head_train(void* stream, const uint16_t* h4, const uint16_t* z4, const f
  * windows by greedy selection. Cell x has frequency f_lo[c] * exp(x * dlog[c]); a window
  * centred on the most probable unclaimed cell covers the cells within half_width[c]
  * (frequency units) of it, and at least `radius` cells each side. Writes centre[c][w]
- * (-1 when no cell is left) and mass[c][w], the probability of the cells it covers that
- * no earlier window claimed. */
+ * (-1 when no cell is left), mass[c][w], the probability of the cells it covers that
+ * no earlier window claimed, and (when span is not NULL) span[c][w][0..1], the first and
+ * last cell it covers: a search that is to earn that probability must cover every
+ * frequency within half_width of every cell in the span. */
 int cpp_head_windows(void* stream, const uint16_t* h4, const float* wh, const float* bh,
                      const double* f_lo, const double* dlog, const double* half_width,
                      int32_t n_curves, int32_t n_windows, int32_t radius, int32_t* centre,
-                     float* mass);
+                     float* mass, int32_t* span);
 
 /* Adam step on n FP32 parameters p with gradients g (then zeroes g); also writes the
  * BF16 copy pb (when not NULL). t is the 1-based step number. */

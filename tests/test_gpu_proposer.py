@@ -209,8 +209,9 @@ def test_head_train_and_windows():
     geo = [_dev(torch.from_numpy(a)) for a in (f_lo, dlog, half)]
     centre = torch.empty((n, n_win), dtype=torch.int32, device="cuda")
     mass = torch.empty((n, n_win), dtype=torch.float32, device="cuda")
+    span = torch.empty((n, n_win, 2), dtype=torch.int32, device="cuda")
     assert lib.cpp_head_windows(_s(), _p(h4d), _p(whd), _p(bhd), *(_p(g) for g in geo), n,
-                                n_win, radius, _p(centre), _p(mass)) == 0
+                                n_win, radius, _p(centre), _p(mass), _p(span)) == 0
     torch.cuda.synchronize()
     for b in range(n):
         q = p[b].copy()
@@ -227,8 +228,47 @@ def test_head_train_and_windows():
             hi = max(hi, int(np.floor(np.log((fx + half[b]) / f_lo[b]) / dlog[b])))
             lo, hi = max(lo, 0), min(hi, L - 1)
             assert abs(mass[b, w].item() - q[lo:hi + 1].sum()) < 1e-5
+            assert span[b, w, 0].item() == lo and span[b, w, 1].item() == hi
             q[lo:hi + 1] = 0.0
             avail[lo:hi + 1] = False
+
+
+def test_stacked_features():
+    # chunks' coarse powers summed as squares per cell, standardised per curve; the
+    # cadence histogram of the parent curve
+    torch = _torch()
+    lib = _lib()
+    rng = _rng(9)
+    n_chunks = 5
+    power = rng.normal(3.0, 1.5, n_chunks * L + 7)
+    power[11] = -np.inf
+    power[L + 40] = np.nan
+    power[2 * L + 9] = -2.0
+    base = np.array([0, L, 2 * L + 3, 3 * L + 5, 4 * L + 7], dtype=np.int64)
+    chunk_start = np.array([0, 3, 5], dtype=np.int32)
+    tau = np.concatenate([np.sort(rng.uniform(0, 80.0, 400)), np.sort(rng.uniform(0, 9.0, 50))])
+    pbase = np.array([0, 400], dtype=np.int64)
+    npts = np.array([400, 50], dtype=np.int32)
+    baseline = np.array([80.0, 9.0])
+    x = torch.empty((2, L, 2), dtype=torch.float32, device="cuda")
+    args = [_dev(torch.from_numpy(a)) for a in (power, chunk_start, base, tau, pbase, npts,
+                                                  baseline)]
+    assert lib.cpp_stack_features(_s(), *(_p(a) for a in args), 2, 30.0, _p(x)) == 0
+    torch.cuda.synchronize()
+    got = x.cpu().numpy()
+    for c in range(2):
+        stack = np.zeros(L)
+        for k in range(chunk_start[c], chunk_start[c + 1]):
+            row = power[base[k]:base[k] + L][::-1]
+            ok = np.isfinite(row) & (row > 0)
+            stack += np.where(ok, row, 0.0) ** 2
+        stack32 = stack.astype(np.float32).astype(np.float64)
+        sd = np.sqrt(np.mean((stack32 - stack.mean()) ** 2))
+        want = np.clip((stack32 - stack.mean()) / sd, -30, 30)
+        np.testing.assert_allclose(got[c, :, 0], want, rtol=1e-5, atol=1e-5)
+        t = tau[pbase[c]:pbase[c] + npts[c]]
+        hist = np.bincount(np.clip((t * (512 / baseline[c])).astype(int), 0, 511), minlength=512)
+        np.testing.assert_allclose(got[c, :, 1], np.repeat(hist, 8) * 512 / npts[c], rtol=1e-6)
 
 
 def test_features_and_adam():
@@ -309,7 +349,7 @@ def test_training_reduces_loss():
     assert losses[-1] < 0.5 * losses[0], losses[::10]
     geo = [_dev(torch.from_numpy(a)) for a in (np.full(n, 0.01), np.full(n, 0.001),
                                                 np.zeros(n))]
-    centre, mass = net.windows(xd, 4, 1, *geo)
+    centre, mass, _ = net.windows(xd, 4, 1, *geo)
     torch.cuda.synchronize()
     hit = np.mean([labels[b] in set(centre[b].cpu().numpy().tolist()) for b in range(n)])
     assert hit > 0.5

@@ -37,19 +37,102 @@ def test_coarse_plan_uses_the_plain_segments():
     assert [p.size for p, _ in ref] == [sg.n_periods for sg in plan.segments]
 
 
-def test_window_ranges_merge_and_clip():
-    grid = engine.LinspaceGrid(0.01, 2.0, 200_000)
-    f = lp.coarse_frequencies(0.01, 2.0)
-    step = (grid.stop - grid.start) / (grid.num - 1)
-    r = lp.window_ranges(np.array([L - 1]), grid, 0.01, 2.0, 1000)
-    assert r == [(0, 1000)]  # highest frequency = shortest period, clipped at index 0
-    r = lp.window_ranges(np.array([0]), grid, 0.01, 2.0, 1000)
-    assert r == [(grid.num - 1 - 1000, grid.num)]
-    c = np.array([2000, 2001])  # neighbouring cells: overlapping windows merge
-    r = lp.window_ranges(c, grid, 0.01, 2.0, 1000)
-    assert len(r) == 1 and r[0][1] - r[0][0] > 2000
-    k = grid.num - 1 - int(np.rint((f[2000] - grid.start) / step))
-    assert r[0][0] <= k - 1000 and r[0][1] >= k + 1000
+def _prep_for(grids):
+    f_lo, dlog, half = lp.window_geometry(grids, 2000)
+    return lp.PreparedGrids(cs=None, grids=list(grids), full=None, coarse=None,
+                            geometry_host=(f_lo, dlog, half))
+
+
+def test_refine_ranges_hold_every_claimed_cells_target():
+    # the refined index range must hold every grid frequency within the half-width of
+    # every claimed cell (ADVICE: claimed probability must be searched)
+    grids = [engine.LinspaceGrid(0.001, 2.0, 200_000), engine.LinspaceGrid(0.05, 2.0, 50_000)]
+    prep = _prep_for(grids)
+    f_lo, dlog, half = prep.geometry_host
+    spans = [(0, 0, 0), (0, 1500, 1503), (0, L - 3, L - 1), (1, 0, 2), (1, 2000, 2001)]
+    curve = np.array([c for c, _, _ in spans])
+    lo = np.array([a for _, a, _ in spans])
+    hi = np.array([b for _, _, b in spans])
+    k0, k1 = lp.refine_ranges(prep, lo, hi, curve)
+    for j, (c, a, b) in enumerate(spans):
+        g = grids[c]
+        freq = g.frequencies()
+        cells = f_lo[c] * np.exp(np.arange(a, b + 1) * dlog[c])
+        need = np.zeros(freq.size, bool)
+        for fc in cells:
+            need |= np.abs(freq - fc) <= half[c]
+        i = np.flatnonzero(need)
+        k = g.num - 1 - i  # ascending-period index
+        assert k.size == 0 or (k0[j] <= k.min() and k1[j] > k.max()), (j, k0[j], k1[j])
+        assert k1[j] - k0[j] <= k.size + 8
+
+
+def test_merge_ranges():
+    curve = np.array([1, 0, 1, 0, 1, 2])
+    k0 = np.array([50, 10, 40, 30, 90, 5])
+    k1 = np.array([60, 20, 55, 35, 95, 6])
+    c, a, b = lp._merge_ranges(curve, k0, k1)
+    assert list(zip(c.tolist(), a.tolist(), b.tolist())) == [
+        (0, 10, 20), (0, 30, 35), (1, 40, 60), (1, 90, 95), (2, 5, 6)]
+    c, a, b = lp._merge_ranges(np.array([3, 3]), np.array([0, 10]), np.array([10, 12]))
+    assert list(zip(c.tolist(), a.tolist(), b.tolist())) == [(3, 0, 12)]
+
+
+def _no_device(monkeypatch):
+    monkeypatch.setattr(engine, "_device", lambda: None)
+    monkeypatch.setattr(engine, "_upload",
+                        lambda a, dtype, device: np.ascontiguousarray(a, dtype=dtype))
+
+
+def test_window_batch_cuts_the_full_descriptors(monkeypatch):
+    # windows cut from the packed descriptor table search the same periods, with the
+    # same binning and reference segment, as window_plan + pack_plans
+    _no_device(monkeypatch)
+    from cuperiod_amd.workloads import survey_curves
+
+    s = BLSSettings(n_durations=6)
+    curves, grids, _ = survey_curves(3, seed=4, n_freq=60_000)
+    grids[1] = grids[1].as_gridspec()  # one explicit grid
+    cs = engine.pack_curves(curves)
+    plans = engine.plan_grids(cs, grids, s)
+    full = engine.pack_plans(cs, plans, s)
+    wins = {0: [(0, 700), (12_000, 13_500)], 1: [(100, 200), (59_000, 60_000)],
+            2: [(30_000, 31_000)]}
+    curve = np.array([c for c in wins for _ in wins[c]])
+    k0 = np.array([a for c in wins for a, _ in wins[c]])
+    k1 = np.array([b for c in wins for _, b in wins[c]])
+    wb = engine.window_batch(full, curve, k0, k1)
+    got = engine.periods_of_columns(wb, np.arange(wb.n_out))
+    want = np.concatenate([plans[c].periods()[a:b] for c in wins for a, b in wins[c]])
+    np.testing.assert_array_equal(got, want)
+    # every new column maps back to the same period of the full batch
+    tab = wb.segment_table[:wb.n_segments]
+    r = np.searchsorted(tab["out_offset"], np.arange(wb.n_out), side="right") - 1
+    src = wb.source_col[r] + (np.arange(wb.n_out) - tab["out_offset"][r])
+    np.testing.assert_array_equal(engine.periods_of_columns(full, src), got)
+    # binning, box widths and reference segment follow the period's full-batch piece
+    ftab = full.segment_table[:full.n_segments]
+    fr = np.searchsorted(ftab["out_offset"], src, side="right") - 1
+    for f in ("bin_duration", "dur_offset", "n_dur", "curve", "group", "mode", "width"):
+        np.testing.assert_array_equal(tab[f][r], ftab[f][fr], err_msg=f)
+    assert wb.curve_count.tolist() == [2200, 1100, 1000]
+
+
+def test_time_chunks(monkeypatch):
+    _no_device(monkeypatch)
+    rng = np.random.default_rng(1)
+    t = np.sort(rng.uniform(0, 300, 3000))
+    t = t[(t < 100) | (t > 160)]  # a gap
+    lcs = [(t, np.ones(t.size), np.ones(t.size)), (np.sort(rng.uniform(0, 5, 40)),
+                                                    np.ones(40), np.ones(40))]
+    cs = engine.pack_curves(lcs)
+    parent, start, count = lp.time_chunks(cs, np.array([40.0, 40.0]))
+    for p_, a, n in zip(parent, start, count):
+        tt = lcs[p_][0][a:a + n]
+        assert tt[-1] - tt[0] <= 40.0 and n >= lp.MIN_CHUNK_POINTS
+    first = parent == 0
+    assert count[first].sum() == t.size and np.all(np.diff(start[first]) > 0)
+    assert parent.tolist().count(1) == 1
 
 
 def test_window_plan_keeps_segment_binning():
@@ -73,11 +156,13 @@ def test_window_plan_keeps_segment_binning():
 def test_select_windows_by_mass():
     prop = lp.LearnedProposer.__new__(lp.LearnedProposer)
     prop.window_mass = 0.9
-    centre = np.array([[5, 9, 100, -1], [7, 8, 9, 10]])
-    mass = np.array([[0.5, 0.45, 0.01, 0.0], [0.2, 0.2, 0.2, 0.2]], dtype=np.float32)
-    kept, conf = prop.select(centre, mass)
-    assert kept[0].tolist() == [5, 9] and abs(conf[0] - 0.95) < 1e-6
-    assert kept[1].tolist() == [7, 8, 9, 10] and abs(conf[1] - 0.8) < 1e-6
+    centre = np.array([[5, 9, 100, -1], [7, 8, 9, 10], [3, 4, -1, -1]])
+    mass = np.array([[0.45, 0.5, 0.01, 0.0], [0.2, 0.2, 0.2, 0.2], [0.95, 0.04, 0, 0]],
+                    dtype=np.float32)
+    keep, conf = prop.select(centre, mass)
+    assert keep[0].tolist() == [True, True, False, False] and abs(conf[0] - 0.95) < 1e-6
+    assert keep[1].tolist() == [True] * 4 and abs(conf[1] - 0.8) < 1e-6
+    assert keep[2].tolist() == [True, False, False, False] and abs(conf[2] - 0.95) < 1e-6
 
 
 def test_parameters_round_trip(tmp_path):
