@@ -77,26 +77,21 @@ def to_input(
     )
 
 
-def _warn_flux_tuple(data: Any, lc: LightCurve, methods: Sequence[PeriodogramMethod]) -> None:
-    """Warn when a box search is about to read a normalised flux as magnitudes."""
-    if not _looks_like_arrays(data):
+def _flux_read_as_magnitudes(lc: LightCurve, methods: Sequence[PeriodogramMethod]) -> None:
+    """A ``(t, y[, dy])`` tuple without ``domain`` is read as magnitudes. When a method
+    that searches in flux is asked to, and the values sit around 1 the way a normalised
+    flux does, say so: a dip in flux is a rise in magnitude, and the search would look
+    for the wrong sign."""
+    in_flux = [m.name for m in methods if m.natural_domain is Domain.FLUX]
+    finite = lc.value[np.isfinite(lc.value)]
+    if not in_flux or finite.size == 0:
         return
-    flux_methods = [m.name for m in methods if m.natural_domain is Domain.FLUX]
-    if not flux_methods or lc.value.size == 0:
-        return
-    with np.errstate(all="ignore"), warnings.catch_warnings():
-        warnings.simplefilter("ignore", RuntimeWarning)
-        median = float(np.nanmedian(lc.value))
-    lo, hi = _NORMALISED_FLUX_RANGE
-    if lo < median < hi:
+    centre = float(np.median(finite))
+    if _NORMALISED_FLUX_RANGE[0] < centre < _NORMALISED_FLUX_RANGE[1]:
         warnings.warn(
-            f"{', '.join(flux_methods)}: the (t, y, dy) tuple is read as magnitudes "
-            f"(median {median:.3g}), but its values look like a normalised flux; pass "
-            "domain='flux' if they are fluxes, or a LightCurve with an explicit domain "
-            "to silence this.",
-            UserWarning,
-            stacklevel=3,
-        )
+            f"the values of the (t, y, dy) tuple have median {centre:.3g}, like a normalised "
+            f"flux, yet are read as magnitudes for {'/'.join(in_flux)}; say which with "
+            "domain='flux' or domain='magnitude'", UserWarning, stacklevel=3)
 
 
 def _multiband_grid(method: PeriodogramMethod, mblc: MultiBandLightCurve, settings: Any
@@ -138,20 +133,17 @@ def periodogram(
     settings: Any = None,
     domain: Domain | str | None = None,
 ) -> Periodogram | MultiResult:
-    """Periodogram(s) of one light curve. ``backend="auto"`` defers to the
-    settings' ``backend`` and resolves to ``"hip"`` on an AMD GPU."""
+    """Periodogram(s) of one light curve: one method name gives a :class:`Periodogram`,
+    a list of names a :class:`MultiResult` keyed by method. ``backend="auto"`` defers to
+    the settings' ``backend`` and resolves to ``"hip"`` on an AMD GPU."""
     lc = to_input(data, domain=domain)
-    names = [method] if isinstance(method, str) else list(method)
-    methods = [get_method(name) for name in names]
-    chosen = select_settings(methods, settings)
-    if domain is None:
-        _warn_flux_tuple(data, lc, methods)
-    results: dict[str, Periodogram] = {}
-    for m in methods:
-        results[m.name] = _run_one(m, lc, chosen[m.name], backend, grid)
-    if isinstance(method, str):
-        return results[methods[0].name]
-    return MultiResult(results)
+    single = isinstance(method, str)
+    methods = [get_method(name) for name in ([method] if single else method)]
+    per_method = select_settings(methods, settings)
+    if domain is None and _looks_like_arrays(data):
+        _flux_read_as_magnitudes(lc, methods)
+    found = {m.name: _run_one(m, lc, per_method[m.name], backend, grid) for m in methods}
+    return found[methods[0].name] if single else MultiResult(found)
 
 
 def best_periods(result: Periodogram | MultiResult, n: int | None = None, **kwargs: Any):

@@ -5,8 +5,8 @@ The row format and sink behaviour of cuPeriod's batch driver
 method with ``key, method, source, backend, n_samples, baseline, best_period`` and, for
 ranks ``k = 1..n_best``, ``period_k, power_k`` and the method's declared per-peak extras
 ``{extra}_k`` (NaN past the last peak); BLS peaks are chosen alias-diverse. Parquet and
-CSV files are written to an exclusively created temporary file and renamed into place;
-CSV text cells that a spreadsheet would evaluate are quote-prefixed.
+CSV files are written to a new hidden file beside the target and renamed over it; CSV
+text cells that a spreadsheet would evaluate are quote-prefixed.
 """
 
 from __future__ import annotations
@@ -31,7 +31,6 @@ ALIAS_DIVERSE_METHODS: frozenset[str] = frozenset({"BLS", "TLS"})
 PEAK_EXTRA_COLUMNS: Mapping[str, tuple[str, ...]] = {
     "BLS": ("depth", "depth_snr", "duration", "t0", "sde"),
     "TLS": ("sr", "depth", "duration", "t0"),
-    "GLS": ("fap",),
 }
 
 #: Extras only multi-band results carry (src/cuperiod/batch/io.py:73-75).
@@ -230,53 +229,49 @@ def _union_columns(rows: Sequence[Mapping[str, Any]]) -> list[str]:
     return list(seen)
 
 
-def _temp_path(path: Path) -> Path:
-    return path.with_name(f".{path.name}.{os.getpid()}.{secrets.token_hex(4)}.tmp")
-
-
-def _move_into_place(tmp: Path, path: Path) -> None:
+def _publish(path: Path, mode: str, fill) -> None:
+    """Write a file through ``fill(handle)`` into a new hidden sibling of ``path``, then
+    rename it over ``path``: a reader sees the old file or the complete new one. A failed
+    write leaves nothing behind; a failed rename leaves the complete file where the
+    error message says."""
+    staging = path.parent / f".{path.name}.{os.getpid()}-{secrets.token_hex(4)}.part"
+    kwargs = {} if "b" in mode else {"newline": "", "encoding": "utf-8"}
     try:
-        os.replace(tmp, path)
+        with open(staging, mode, **kwargs) as handle:
+            fill(handle)
+    except BaseException:
+        staging.unlink(missing_ok=True)
+        raise
+    try:
+        os.replace(staging, path)
     except OSError as exc:
-        raise type(exc)(
-            f"could not replace {path} ({exc}); the complete result is kept in {tmp}"
-        ) from exc
+        raise OSError(exc.errno, f"result file {path} not replaced ({exc.strerror}); "
+                      f"the finished rows are in {staging}") from exc
 
 
 def _write_parquet(rows: Sequence[Mapping[str, Any]], path: Path) -> None:
     import pyarrow as pa
     import pyarrow.parquet as pq
 
-    columns = _union_columns(rows)
-    table = pa.table({col: [row.get(col) for row in rows] for col in columns})
-    tmp = _temp_path(path)
-    try:
-        with tmp.open("xb") as fh:
-            pq.write_table(table, fh)
-    except BaseException:
-        tmp.unlink(missing_ok=True)
-        raise
-    _move_into_place(tmp, path)
+    names = _union_columns(rows)
+    table = pa.Table.from_pylist([{c: r.get(c) for c in names} for r in rows])
+    _publish(path, "xb", lambda fh: pq.write_table(table, fh))
 
 
 def _write_csv(rows: Sequence[Mapping[str, Any]], path: Path) -> None:
-    columns = _union_columns(rows)
-    lists = [c for c in columns if any(isinstance(r.get(c), list) for r in rows)]
-    if lists:
-        raise ValueError(
-            f"CSV sink cannot store array-valued columns {lists} (e.g. from store_raw); "
-            "use a .parquet sink."
-        )
-    tmp = _temp_path(path)
-    try:
-        with tmp.open("x", newline="", encoding="utf-8") as fh:
-            writer = csv.DictWriter(fh, fieldnames=columns, extrasaction="ignore")
-            writer.writeheader()
-            writer.writerows({c: csv_safe(v) for c, v in row.items()} for row in rows)
-    except BaseException:
-        tmp.unlink(missing_ok=True)
-        raise
-    _move_into_place(tmp, path)
+    names = _union_columns(rows)
+    arrays = sorted({c for r in rows for c, v in r.items() if isinstance(v, list)})
+    if arrays:
+        raise ValueError(f"a CSV sink holds scalars only, and the rows carry arrays in "
+                         f"{arrays} (store_raw): write to a .parquet sink")
+
+    def fill(fh) -> None:
+        out = csv.writer(fh)
+        out.writerow(names)
+        for r in rows:
+            out.writerow([csv_safe(r.get(c, "")) for c in names])
+
+    _publish(path, "x", fill)
 
 
 def write_rows(rows: Sequence[Mapping[str, Any]], path: Path) -> None:

@@ -1,19 +1,20 @@
-"""Light curves: the input of every BLS search.
+"""Light curves: the inputs of the searches.
 
-The subset of cuPeriod's ``LightCurve`` the BLS path reads
-(src/cuperiod/core/lightcurve.py:58-303, 625-646): construction from arrays, the
-``finite`` filter (non-finite samples and non-positive errors dropped), the
-magnitude -> flux conversion BLS applies before searching (a transit is a dip in flux),
-``baseline`` and the unusable-error diagnosis. Times are kept verbatim; the search
-subtracts its own time origin in FP64.
+:class:`LightCurve` carries the fields and the handful of operations the BLS path reads
+from cuPeriod's light curve (src/cuperiod/core/lightcurve.py:58-303): ``time`` (days),
+``value`` (magnitude or flux), an optional 1-sigma ``error``, the brightness ``domain``
+and ``meta``; :meth:`LightCurve.finite` keeps the points a search can use (finite time and
+value, and a finite positive error when there is an error column); the domain
+conversions are the photometric ones (``flux = 10**(-0.4 (mag - zp))``). Table and file
+readers are not part of this package.
 
-:class:`MultiBandLightCurve` holds several bands of one star for the multi-band search
-(src/cuperiod/core/lightcurve.py:329-575): every band a :class:`LightCurve`, all in one
-brightness domain, in insertion order.
+:class:`MultiBandLightCurve` groups several bands of one star (one brightness domain) for
+the multi-band search (src/cuperiod/core/lightcurve.py:329-575).
 """
 
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 from typing import Any, Mapping
 
@@ -22,22 +23,26 @@ import numpy as np
 from cuperiod_amd.core.columns import Domain
 from cuperiod_amd.core.errors import InsufficientDataError
 
+_LN10 = math.log(10.0)
 
-def _as_f64(a: object, name: str) -> np.ndarray:
-    if isinstance(a, np.ma.MaskedArray):
-        a = a.filled(np.nan)
-    out = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
-    if out.ndim == 0:
-        out = out.reshape(1)
-    if out.ndim != 1:
-        raise ValueError(f"{name} must be 1-D, got shape {out.shape}")
-    return out
+
+def _column(data: object, label: str, length: int | None = None) -> np.ndarray:
+    """``data`` as a contiguous 1-D float64 array (masked entries as NaN, a scalar as one
+    value), of ``length`` values when given."""
+    if isinstance(data, np.ma.MaskedArray):
+        data = data.astype(np.float64).filled(np.nan)
+    arr = np.atleast_1d(np.asarray(data, dtype=np.float64))
+    if arr.ndim != 1:
+        raise ValueError(f"light curve {label}: expected one dimension, got shape {arr.shape}")
+    if length is not None and arr.size != length:
+        raise ValueError(f"light curve {label}: {arr.size} values for {length} times")
+    return np.ascontiguousarray(arr)
 
 
 @dataclass(frozen=True, eq=False)
 class LightCurve:
-    """One band's ``time`` (days), ``value`` (magnitude or flux) and optional 1-sigma
-    ``error``, as contiguous float64 arrays."""
+    """One band: ``time`` (days), ``value`` and optional ``error`` (float64 arrays of one
+    length), its brightness ``domain`` and free-form ``meta``."""
 
     time: np.ndarray
     value: np.ndarray
@@ -46,21 +51,21 @@ class LightCurve:
     meta: Mapping[str, Any] = field(default_factory=dict)
 
     def __post_init__(self) -> None:
-        time = _as_f64(self.time, "time")
-        value = _as_f64(self.value, "value")
-        if time.size != value.size:
-            raise ValueError(
-                f"time and value length mismatch: {time.size} != {value.size}"
-            )
-        error = None if self.error is None else _as_f64(self.error, "error")
-        if error is not None and error.size != time.size:
-            raise ValueError(
-                f"error and time length mismatch: {error.size} != {time.size}"
-            )
-        object.__setattr__(self, "time", time)
-        object.__setattr__(self, "value", value)
-        object.__setattr__(self, "error", error)
-        object.__setattr__(self, "domain", Domain(self.domain))
+        time = _column(self.time, "time")
+        set_ = object.__setattr__
+        set_(self, "time", time)
+        set_(self, "value", _column(self.value, "value", time.size))
+        if self.error is not None:
+            set_(self, "error", _column(self.error, "error", time.size))
+        set_(self, "domain", Domain(self.domain))
+
+    @classmethod
+    def from_arrays(cls, time: object, value: object, error: object | None = None, *,
+                    domain: Domain | str = Domain.MAGNITUDE,
+                    meta: Mapping[str, Any] | None = None) -> "LightCurve":
+        """A light curve from array-likes (masked entries become NaN, dropped by
+        :meth:`finite`)."""
+        return cls(time, value, error, Domain(domain), dict(meta or {}))
 
     @property
     def n(self) -> int:
@@ -69,188 +74,131 @@ class LightCurve:
 
     @property
     def baseline(self) -> float:
-        """``max(time) - min(time)`` in days (0.0 if empty)."""
-        if self.time.size == 0:
-            return 0.0
-        return float(self.time.max() - self.time.min())
+        """Span of the times in days (0.0 without points)."""
+        return float(np.ptp(self.time)) if self.time.size else 0.0
 
-    @classmethod
-    def from_arrays(
-        cls,
-        time: object,
-        value: object,
-        error: object | None = None,
-        *,
-        domain: Domain | str = Domain.MAGNITUDE,
-        meta: Mapping[str, Any] | None = None,
-    ) -> "LightCurve":
-        """Build from array-likes (masked entries become NaN)."""
-        return cls(
-            time=_as_f64(time, "time"),
-            value=_as_f64(value, "value"),
-            error=None if error is None else _as_f64(error, "error"),
-            domain=Domain(domain),
-            meta=dict(meta or {}),
-        )
-
-    def _usable(self) -> np.ndarray:
-        mask = np.isfinite(self.time) & np.isfinite(self.value)
+    def _kept(self) -> np.ndarray:
+        """The points a search keeps."""
+        keep = np.isfinite(self.time) & np.isfinite(self.value)
         if self.error is not None:
-            mask &= np.isfinite(self.error) & (self.error > 0.0)
-        return mask
+            keep &= np.isfinite(self.error) & (self.error > 0.0)
+        return keep
+
+    def _subset(self, keep: np.ndarray) -> "LightCurve":
+        return LightCurve(self.time[keep], self.value[keep],
+                          None if self.error is None else self.error[keep], self.domain,
+                          self.meta)
 
     def finite(self) -> "LightCurve":
-        """A copy without non-finite samples and without non-positive errors."""
-        mask = self._usable()
-        if mask.all():
-            return self
-        return LightCurve(
-            time=self.time[mask],
-            value=self.value[mask],
-            error=None if self.error is None else self.error[mask],
-            domain=self.domain,
-            meta=self.meta,
-        )
-
-    def as_flux(self, *, zeropoint: float = 0.0) -> "LightCurve":
-        """``flux = 10**(-0.4 (mag - zp))``, ``flux_err = 0.4 ln10 * flux * mag_err``."""
-        if self.domain is Domain.FLUX:
-            return self
-        flux = np.power(10.0, -0.4 * (self.value - zeropoint))
-        error = None if self.error is None else (0.4 * np.log(10.0)) * flux * self.error
-        return LightCurve(self.time, flux, error, Domain.FLUX, self.meta)
-
-    def as_magnitude(self, *, zeropoint: float = 0.0) -> "LightCurve":
-        """``mag = -2.5 log10(flux) + zp``; non-positive flux becomes non-finite."""
-        if self.domain is Domain.MAGNITUDE:
-            return self
-        with np.errstate(invalid="ignore", divide="ignore"):
-            mag = -2.5 * np.log10(self.value) + zeropoint
-            error = (
-                None
-                if self.error is None
-                else (2.5 / np.log(10.0)) * self.error / self.value
-            )
-        return LightCurve(self.time, mag, error, Domain.MAGNITUDE, self.meta)
+        """The usable points only (this light curve when all are usable)."""
+        keep = self._kept()
+        return self if keep.all() else self._subset(keep)
 
     def in_domain(self, domain: Domain | str, *, zeropoint: float = 0.0) -> "LightCurve":
-        """This light curve converted to ``domain``."""
-        if Domain(domain) is Domain.FLUX:
-            return self.as_flux(zeropoint=zeropoint)
-        return self.as_magnitude(zeropoint=zeropoint)
+        """The light curve in ``domain``: ``flux = 10**(-0.4 (mag - zp))`` with
+        ``sigma_flux = 0.4 ln10 flux sigma_mag``, and the inverse (non-positive flux has
+        no magnitude: NaN)."""
+        target = Domain(domain)
+        if target is self.domain:
+            return self
+        if target is Domain.FLUX:
+            value = 10.0 ** (-0.4 * (self.value - zeropoint))
+            error = None if self.error is None else (0.4 * _LN10) * value * self.error
+        else:
+            with np.errstate(invalid="ignore", divide="ignore"):
+                value = zeropoint - 2.5 * np.log10(self.value)
+                error = None if self.error is None else (2.5 / _LN10) * self.error / self.value
+        return LightCurve(self.time, value, error, target, self.meta)
+
+    def as_flux(self, *, zeropoint: float = 0.0) -> "LightCurve":
+        return self.in_domain(Domain.FLUX, zeropoint=zeropoint)
+
+    def as_magnitude(self, *, zeropoint: float = 0.0) -> "LightCurve":
+        return self.in_domain(Domain.MAGNITUDE, zeropoint=zeropoint)
 
     def unusable_reason(self) -> str | None:
-        """Why the errors leave no usable point, or ``None`` when some point survives."""
+        """Why the error column leaves no point to search, or None."""
         if self.error is None:
             return None
-        data = np.isfinite(self.time) & np.isfinite(self.value)
-        n_data = int(np.count_nonzero(data))
-        if n_data == 0 or self._usable().any():
+        with_data = int(np.count_nonzero(np.isfinite(self.time) & np.isfinite(self.value)))
+        if with_data == 0 or self._kept().any():
             return None
-        return (
-            f"none of the {n_data} points with a finite time and value has a usable "
-            "error (all are zero, negative or non-finite), so every point would be "
-            "dropped; drop the error column (error=None) to run unweighted"
-        )
+        return (f"every one of the {with_data} measured points has an error that is zero, "
+                "negative or not finite; pass error=None to search without weights")
 
 
 @dataclass(frozen=True, eq=False)
 class MultiBandLightCurve:
-    """Several bands (filters) of one star: ``bands`` maps band name -> light curve, in
-    insertion order; ``meta`` is carried through results."""
+    """Bands (filters) of one star: ``bands`` maps a band name to its light curve, in the
+    order given; every band is in one brightness domain."""
 
     bands: Mapping[str, LightCurve]
     meta: Mapping[str, Any] = field(default_factory=dict)
 
     def __post_init__(self) -> None:
-        if not self.bands:
-            raise ValueError("MultiBandLightCurve requires at least one band")
-        for name, lc in self.bands.items():
-            if not isinstance(lc, LightCurve):
-                raise TypeError(
-                    f"band {name!r} is a {type(lc).__name__}, not a LightCurve; "
-                    "build each band with LightCurve.from_arrays(...)"
-                )
-        domains = {name: lc.domain for name, lc in self.bands.items()}
-        if len(set(domains.values())) > 1:
-            listing = ", ".join(f"{k}: {v.value}" for k, v in domains.items())
-            raise ValueError(
-                f"bands are in different brightness domains ({listing}); convert them "
-                "with LightCurve.in_domain(...), or pass domain='magnitude' or "
-                "domain='flux' to periodogram()"
-            )
+        if len(self.bands) == 0:
+            raise ValueError("a multi-band light curve needs at least one band")
+        wrong = [k for k, v in self.bands.items() if not isinstance(v, LightCurve)]
+        if wrong:
+            raise TypeError(f"bands {wrong} are not LightCurve objects "
+                            "(make them with LightCurve.from_arrays)")
+        seen = {lc.domain for lc in self.bands.values()}
+        if len(seen) > 1:
+            raise ValueError("the bands mix brightness domains "
+                             f"({sorted(d.value for d in seen)}); convert them to one")
+
+    @classmethod
+    def from_light_curves(cls, bands: Mapping[str, LightCurve], *,
+                          meta: Mapping[str, Any] | None = None) -> "MultiBandLightCurve":
+        return cls(dict(bands), dict(meta or {}))
 
     @property
     def band_names(self) -> tuple[str, ...]:
-        """The band labels, in order."""
-        return tuple(self.bands)
+        return tuple(self.bands.keys())
 
     @property
     def n_bands(self) -> int:
-        """Number of bands."""
         return len(self.bands)
 
-    @classmethod
-    def from_light_curves(
-        cls, bands: Mapping[str, LightCurve], *, meta: Mapping[str, Any] | None = None
-    ) -> "MultiBandLightCurve":
-        """Build from an explicit ``{band: LightCurve}`` mapping."""
-        return cls(bands=dict(bands), meta=dict(meta or {}))
+    def _each(self, op) -> "MultiBandLightCurve":
+        return MultiBandLightCurve({k: op(v) for k, v in self.bands.items()}, self.meta)
 
     def finite(self) -> "MultiBandLightCurve":
-        """Every band without its non-finite samples and non-positive errors."""
-        return MultiBandLightCurve(
-            bands={k: lc.finite() for k, lc in self.bands.items()}, meta=self.meta
-        )
+        """Every band's usable points."""
+        return self._each(LightCurve.finite)
 
     def in_domain(self, domain: Domain | str) -> "MultiBandLightCurve":
-        """Every band converted to ``domain``."""
-        return MultiBandLightCurve(
-            bands={k: lc.in_domain(domain) for k, lc in self.bands.items()}, meta=self.meta
-        )
+        return self._each(lambda lc: lc.in_domain(domain))
 
     def unusable_reason(self) -> str | None:
-        """Why the errors leave no usable point in any band, or ``None``: a band wiped
-        out by its errors does not condemn the others."""
-        reasons: dict[str, str] = {}
-        for name, lc in self.bands.items():
-            reason = lc.unusable_reason()
-            if reason is not None:
-                reasons[name] = reason
-            elif lc._usable().any():
-                return None
-        if not reasons:
+        """Why no band has a point to search, or None (one usable band suffices)."""
+        if any(lc._kept().any() for lc in self.bands.values()):
             return None
-        return "; ".join(f"band {name!r}: {why}" for name, why in reasons.items())
+        parts = [f"band {k!r}: {why}" for k, lc in self.bands.items()
+                 if (why := lc.unusable_reason()) is not None]
+        return "; ".join(parts) or None
 
     def stacked(self) -> tuple[np.ndarray, np.ndarray, np.ndarray | None, np.ndarray]:
-        """All bands as time-sorted ``(time, value, error, band)`` arrays (stable sort;
-        ``error`` is ``None`` when a band has none)."""
-        times, values, errors, labels = [], [], [], []
-        missing = False
-        for name, lc in self.bands.items():
-            times.append(lc.time)
-            values.append(lc.value)
-            if lc.error is None:
-                missing = True
-            else:
-                errors.append(lc.error)
-            labels.append(np.full(lc.n, name, dtype=object))
-        time = np.concatenate(times)
-        value = np.concatenate(values)
-        band = np.concatenate(labels)
-        error = None if missing else np.concatenate(errors)
+        """``(time, value, error, band)`` of all bands merged in time order (stable on
+        equal times; ``error`` None unless every band has one)."""
+        lcs = list(self.bands.values())
+        time = np.concatenate([lc.time for lc in lcs])
         order = np.argsort(time, kind="stable")
-        return (time[order], value[order], None if error is None else error[order],
-                band[order])
+        value = np.concatenate([lc.value for lc in lcs])[order]
+        band = np.concatenate([np.full(lc.n, k, dtype=object)
+                               for k, lc in self.bands.items()])[order]
+        error = None
+        if all(lc.error is not None for lc in lcs):
+            error = np.concatenate([lc.error for lc in lcs])[order]
+        return time[order], value, error, band
 
 
-def raise_if_unusable(data: "LightCurve | MultiBandLightCurve", label: str) -> None:
-    """Raise :class:`InsufficientDataError` when an unusable error column would empty
-    ``data`` (src/cuperiod/core/lightcurve.py:625-646)."""
-    reason = data.unusable_reason()
-    if reason is not None:
-        raise InsufficientDataError(f"{label}: {reason}")
+def raise_if_unusable(data: LightCurve | MultiBandLightCurve, label: str) -> None:
+    """:class:`InsufficientDataError` when the error column leaves nothing to search
+    (the check of src/cuperiod/core/lightcurve.py:625-646)."""
+    why = data.unusable_reason()
+    if why is not None:
+        raise InsufficientDataError(f"{label}: {why}")
 
 
 __all__ = ["LightCurve", "MultiBandLightCurve", "raise_if_unusable"]

@@ -4,8 +4,7 @@ The selection rules of cuPeriod's peak picker (src/cuperiod/core/peaks.py:57-231
 interior local maxima (strictly above the left neighbour, at least the right one, so a
 flat top reports its left edge), greedy selection by descending score with a minimum
 frequency separation, and the alias-diverse variant that also skips harmonics,
-sub-harmonics and the sidereal/solar/yearly alias lattice of every chosen peak. The
-peak-preserving downsample keeps each bin's first maximum (``:234-288``).
+sub-harmonics and the sidereal/solar/yearly alias lattice of every chosen peak.
 """
 
 from __future__ import annotations
@@ -20,9 +19,6 @@ SIDEREAL_FREQ_CPD = 1.0027379
 SOLAR_FREQ_CPD = 1.0
 #: Yearly sidelobe spacing (cycles/day).
 YEARLY_FREQ_CPD = 1.0 / 365.25
-
-_DOWNSAMPLE_VECTOR_MAX_RATIO = 64
-
 
 def _sum_is_finite(values: np.ndarray) -> bool:
     with np.errstate(invalid="ignore", over="ignore"):
@@ -156,29 +152,27 @@ def peak_preserving_downsample(
     *,
     sense: Literal["max", "min"] = "max",
 ) -> tuple[np.ndarray, np.ndarray]:
-    """Keep the most significant sample (first on ties) of each of ``n_points``
-    index bins; float32 output."""
+    """``(frequency, power)`` thinned to ``n_points`` samples, as float32: the samples are
+    cut into ``n_points`` runs of consecutive indices (run ``i`` starts at
+    ``floor(i*n/n_points)``) and each run keeps its most significant sample, the first of
+    equals (NaN counts as least significant). A spectrum of at most ``n_points`` samples
+    is returned whole."""
     if sense not in ("max", "min"):
-        raise ValueError(f"sense must be 'max' or 'min', got {sense!r}")
-    n = int(frequency.size)
-    if n_points <= 0:
+        raise ValueError(f"sense must be 'max' or 'min', not {sense!r}")
+    n, m = int(frequency.size), int(n_points)
+    if m <= 0:
         return np.empty(0, dtype=np.float32), np.empty(0, dtype=np.float32)
-    if n <= n_points:
+    if n <= m:
         return frequency.astype(np.float32), power.astype(np.float32)
-    score = power if sense == "max" else -power
-    if not _sum_is_finite(score):
-        score = np.where(np.isnan(score), -np.inf, score)
-    edges = np.linspace(0, n, n_points + 1).astype(np.int64)
-    if n <= _DOWNSAMPLE_VECTOR_MAX_RATIO * n_points:
-        starts = edges[:-1]
-        bin_max = np.maximum.reduceat(score, starts)
-        hits = np.flatnonzero(score == np.repeat(bin_max, np.diff(edges)))
-        picks = hits[np.searchsorted(hits, starts)]
-    else:
-        picks = np.empty(n_points, dtype=np.int64)
-        for i in range(n_points):
-            lo, hi = int(edges[i]), int(edges[i + 1])
-            picks[i] = lo + int(np.argmax(score[lo:hi]))
+    score = np.asarray(power if sense == "max" else -power, dtype=np.float64)
+    score = np.where(np.isnan(score), -np.inf, score)
+    first = (np.arange(m + 1, dtype=np.int64) * n) // m
+    width = int(np.diff(first).max())
+    # one row per run, padded on the right with -inf so a pad is never the first maximum
+    cols = first[:-1, None] + np.arange(width, dtype=np.int64)[None, :]
+    inside = cols < first[1:, None]
+    table = np.where(inside, score[np.minimum(cols, n - 1)], -np.inf)
+    picks = first[:-1] + np.argmax(table, axis=1)
     return frequency[picks].astype(np.float32), power[picks].astype(np.float32)
 
 

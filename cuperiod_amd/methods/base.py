@@ -195,36 +195,30 @@ def list_methods() -> list[MethodInfo]:
 def select_settings(
     methods: Sequence[PeriodogramMethod], settings: Any
 ) -> dict[str, Any]:
-    """Each method's settings from a ``settings=`` argument: ``None`` (defaults), one
-    settings object (for the methods of its class; must match one), or a
-    ``{method: settings}`` mapping (src/cuperiod/methods/base.py:459-535)."""
-    if settings is None:
-        return {m.name: m.coerce_settings(None) for m in methods}
-    if not isinstance(settings, Mapping):
-        if not any(isinstance(settings, m.settings_cls) for m in methods):
-            expected = " or ".join(dict.fromkeys(m.settings_cls.__name__ for m in methods))
-            names = ", ".join(dict.fromkeys(m.name for m in methods))
-            raise TypeError(
-                f"settings: {names} expects {expected}, got {type(settings).__name__}"
-                " (pass a {method: settings} mapping to tune several methods)"
-            )
-        return {
-            m.name: settings if isinstance(settings, m.settings_cls) else m.coerce_settings(None)
-            for m in methods
-        }
-    by_method: dict[str, tuple[str, Any]] = {}
-    for key, value in settings.items():
-        method = get_method(str(key))
-        if method.name in by_method:
-            raise ValueError(
-                f"settings: keys {by_method[method.name][0]!r} and {key!r} both name "
-                f"method {method.name}"
-            )
-        by_method[method.name] = (str(key), value)
-    return {
-        m.name: m.coerce_settings(by_method[m.name][1] if m.name in by_method else None)
-        for m in methods
-    }
+    """Each method's settings from a ``settings=`` argument, by method name: ``None``
+    gives every method its defaults; one settings object goes to the methods of its
+    class (the others get defaults) and must fit at least one of them; a mapping names
+    methods (in any spelling :func:`get_method` accepts) and gives each its settings
+    (src/cuperiod/methods/base.py:459-535)."""
+    given: dict[str, Any] = {}
+    if isinstance(settings, Mapping):
+        spelled: dict[str, Any] = {}
+        for key, value in settings.items():
+            name = get_method(str(key)).name
+            if name in spelled:
+                raise ValueError(f"settings: {spelled[name]!r} and {key!r} are the same "
+                                 f"method ({name}); give it once")
+            spelled[name] = key
+            given[name] = value
+    elif settings is not None:
+        fits = [m.name for m in methods if isinstance(settings, m.settings_cls)]
+        if not fits:
+            wanted = sorted({m.settings_cls.__name__ for m in methods})
+            raise TypeError(f"settings: a {type(settings).__name__} fits none of "
+                            f"{[m.name for m in methods]} (they take {' or '.join(wanted)}); "
+                            "to tune several methods pass {method: settings}")
+        given = dict.fromkeys(fits, settings)
+    return {m.name: m.coerce_settings(given.get(m.name)) for m in methods}
 
 
 __all__ = [

@@ -1,31 +1,30 @@
 """Multi-band Box Least Squares on the MI355X engine.
 
-Bands of one star share one trial *period*. Two statistics
-(``BLSSettings.mb_ephemeris`` or the ``ephemeris=`` argument of
-:func:`bls_multiband_power`; cuPeriod src/cuperiod/multiband/bls_mb.py:1-35):
+The bands of one star share a trial *period*; the statistic at each period is
+``sum_k max(SNR_k, 0)^2`` over the bands' depth signal-to-noise ratios. Two ways to place
+the boxes (``BLSSettings.mb_ephemeris`` or the ``ephemeris=`` argument; the modes of
+cuPeriod's src/cuperiod/multiband/bls_mb.py:1-35):
 
 ``"independent"`` (default)
-    Each band is box-searched on its own over the common grid (its own best duration
-    and mid-transit time per period); the per-period statistic is the quadrature sum of
-    the bands' best depth-SNRs, ``sum_k max(SNR_k, 0)^2``. The box geometry at each
-    period is that of the band detecting it most strongly.
+    every band runs its own single-band search on the common grid, and each period's
+    statistic stacks the bands' best boxes; duration, mid-transit time and depth are
+    those of the band with the largest SNR there (the first such band on equal SNRs).
 
 ``"shared"``
-    One box -- one duration and one mid-transit time on a phase grid common to every
-    band -- is placed for all bands at once, each band fitting its own depth inside it;
-    the summed squared depth-SNRs of the bands that dip there are maximised over the
-    boxes (:mod:`cuperiod_amd.methods._bls_joint`). Duration and mid-transit time are
-    the shared box's.
+    one box (a duration and a mid-transit time on a phase grid common to all bands) is
+    placed for every band at once, each band fitting its own depth in it, and the box
+    with the largest stacked statistic wins (:mod:`cuperiod_amd.methods._bls_joint`,
+    which drives the joint kernel).
 
-Either way the power is ``sum_k max(SNR_k, 0)^2``, ``extras["band_index"]`` indexes
-``meta["bands"]`` (the contributing bands), the depth is that band's in its own flux
-units, and ``meta["mb_ephemeris"]`` names the statistic. A band's errors are used only
-when every contributing band has them; otherwise every band is searched with a
-constant error equal to its own robust scatter (1.4826 x the median absolute deviation).
+``extras["band_index"]`` indexes ``meta["bands"]``, the bands that took part (those with
+at least three usable points); ``meta["mb_ephemeris"]`` names the mode. Every band is
+searched in flux. A band's errors are used when every taking-part band has errors;
+otherwise each band gets one constant error, its robust scatter.
 """
 
 from __future__ import annotations
 
+from dataclasses import dataclass
 from typing import Any, Final
 
 import numpy as np
@@ -37,50 +36,120 @@ from cuperiod_amd.core.grid import GridSpec
 from cuperiod_amd.core.lightcurve import LightCurve, MultiBandLightCurve
 from cuperiod_amd.core.result import Periodogram
 
-#: A band needs at least this many finite points to contribute a box search.
-_MIN_BAND_POINTS = 3
-
-#: Scale factor turning a median absolute deviation into a Gaussian sigma.
-_MAD_TO_SIGMA = 1.4826
-
 #: The multi-band box statistics.
 EPHEMERIS_MODES: Final = ("independent", "shared")
 
+#: Usable points a band needs to take part.
+_BAND_MIN_POINTS = 3
 
-def _robust_scatter(flux: np.ndarray) -> float:
-    """``1.4826 * MAD`` of a band's flux (bls_mb.py:63-76); the standard deviation when
-    the MAD is 0, and 1 for a constant band."""
-    sigma = _MAD_TO_SIGMA * float(np.median(np.abs(flux - np.median(flux))))
-    if not (np.isfinite(sigma) and sigma > 0.0):
-        sigma = float(np.std(flux))
-    if not (np.isfinite(sigma) and sigma > 0.0):
-        sigma = 1.0
-    return sigma
+#: sigma / MAD of a normal distribution.
+_SIGMA_PER_MAD = 1.4826
+
+#: Per-period outputs of a band search, by the name the periodogram gives them.
+_BOX_FIELDS = (("depth", "depth"), ("duration", "duration"), ("t0", "transit_time"))
 
 
-def _band_errors(bands: list[LightCurve]) -> list[np.ndarray]:
-    """Real errors when every band has them, else each band's constant robust scatter
-    (bls_mb.py:79-92)."""
-    if all(lc.error is not None for lc in bands):
-        return [np.asarray(lc.error, dtype=np.float64) for lc in bands]
-    return [np.full(lc.n, _robust_scatter(lc.value)) for lc in bands]
+def _noise_scale(flux: np.ndarray) -> float:
+    """A band's constant error: ``1.4826 x MAD``; the standard deviation where the MAD
+    is zero (more than half the points equal), and 1.0 for a constant band (the rule of
+    src/cuperiod/multiband/bls_mb.py:63-76)."""
+    mad = np.median(np.abs(flux - np.median(flux)))
+    for scale in (_SIGMA_PER_MAD * float(mad), float(np.std(flux))):
+        if np.isfinite(scale) and scale > 0.0:
+            return scale
+    return 1.0
 
 
 def resolve_ephemeris(settings: Any, ephemeris: str | None = None) -> str:
-    """The statistic to run: the argument, else ``settings.mb_ephemeris``, else
-    ``"independent"`` (bls_mb.py:95-117). Raises ``ValueError`` for another name."""
-    if ephemeris is not None:
-        mode = ephemeris
-    elif isinstance(settings, BLSSettings):
-        mode = settings.mb_ephemeris
-    else:
-        mode = "independent"
-    mode = str(mode).strip().lower()
+    """The mode to run: ``ephemeris`` if given, else ``settings.mb_ephemeris`` for BLS
+    settings, else ``"independent"``; names are trimmed and case-folded, and a name not in
+    :data:`EPHEMERIS_MODES` raises ``ValueError``."""
+    if ephemeris is None:
+        ephemeris = settings.mb_ephemeris if isinstance(settings, BLSSettings) else None
+    mode = "independent" if ephemeris is None else str(ephemeris).strip().lower()
     if mode not in EPHEMERIS_MODES:
-        raise ValueError(
-            f"BLS multiband: unknown ephemeris {mode!r}; choose from {EPHEMERIS_MODES}"
-        )
+        raise ValueError(f"BLS multiband: ephemeris {ephemeris!r} is none of {EPHEMERIS_MODES}")
     return mode
+
+
+@dataclass(frozen=True)
+class _BandSet:
+    """The bands that take part: names, flux light curves and the error array each is
+    searched with."""
+
+    names: tuple[str, ...]
+    curves: tuple[LightCurve, ...]
+    errors: tuple[np.ndarray, ...]
+
+    @classmethod
+    def select(cls, mblc: MultiBandLightCurve, min_detections: int) -> "_BandSet":
+        usable = [(name, lc.in_domain(Domain.FLUX)) for name, lc in mblc.finite().bands.items()]
+        usable = [(name, lc) for name, lc in usable if lc.n >= _BAND_MIN_POINTS]
+        if not usable:
+            raise InsufficientDataError(
+                f"BLS multiband: not one band has enough finite points (at least "
+                f"{_BAND_MIN_POINTS} are needed)")
+        names, curves = (tuple(col) for col in zip(*usable))
+        total = sum(lc.n for lc in curves)
+        if total < min_detections:
+            raise InsufficientDataError(
+                f"BLS multiband: the usable bands hold {total} finite points, fewer than "
+                f"min_detections = {min_detections}")
+        if all(lc.error is not None for lc in curves):
+            errors = tuple(np.asarray(lc.error, dtype=np.float64) for lc in curves)
+        else:
+            errors = tuple(np.full(lc.n, _noise_scale(lc.value)) for lc in curves)
+        return cls(names, curves, errors)
+
+    @property
+    def n_points(self) -> int:
+        return sum(lc.n for lc in self.curves)
+
+    @property
+    def baseline(self) -> float:
+        lo = min(float(lc.time.min()) for lc in self.curves)
+        hi = max(float(lc.time.max()) for lc in self.curves)
+        return hi - lo
+
+
+def _stack_independent(per_band: list[list[dict]]) -> tuple[np.ndarray, dict]:
+    """``(power, extras)`` over every period of the grid from each band's per-segment
+    outputs (``per_band[band][segment]``)."""
+    def matrix(key: str) -> np.ndarray:
+        return np.stack([np.concatenate([seg[key] for seg in segs]) for segs in per_band])
+
+    snr = np.clip(matrix("depth_snr"), 0.0, None)
+    power = np.square(snr).sum(axis=0)
+    lead = np.argmax(snr, axis=0)
+    at_lead = lambda m: np.take_along_axis(m, lead[None, :], axis=0)[0]  # noqa: E731
+    extras = {name: at_lead(matrix(key)) for name, key in _BOX_FIELDS}
+    extras["depth_snr"] = np.sqrt(power)
+    extras["band_index"] = lead.astype(np.float64)
+    return power, extras
+
+
+def _independent(bands: _BandSet, segments, settings: BLSSettings, backend: str):
+    from cuperiod_amd.methods import bls
+
+    per_band = [bls._segments_power(backend, lc.time, lc.value, err, segments, settings)
+                for lc, err in zip(bands.curves, bands.errors)]
+    return _stack_independent(per_band)
+
+
+def _shared(bands: _BandSet, segments, settings: BLSSettings, backend: str):
+    from cuperiod_amd.methods import _bls_joint
+
+    joint = _bls_joint.joint_bands([lc.time for lc in bands.curves],
+                                   [lc.value for lc in bands.curves], list(bands.errors))
+    found = _bls_joint.joint_search(joint, segments, settings.bins_per_duration,
+                                    objective=settings.objective, backend=backend,
+                                    batch=settings.batch_periods,
+                                    precision=settings.precision)
+    column = lambda key: np.concatenate([f[key] for f in found])  # noqa: E731
+    extras = {name: column(key) for name, key in _BOX_FIELDS}
+    extras["depth_snr"] = column("depth_snr")
+    extras["band_index"] = column("band_index")
+    return column("power"), extras
 
 
 def bls_multiband_power(
@@ -92,129 +161,32 @@ def bls_multiband_power(
     *,
     ephemeris: str | None = None,
 ) -> Periodogram:
-    """The multi-band BLS statistic on a shared period grid (bls_mb.py:120-237).
-
-    ``grid`` is searched exactly as given, segmented as the single-band search segments
-    it; a band contributes when it has at least 3 finite points. Raises
-    :class:`InsufficientDataError` when no band contributes, the contributing bands hold
-    fewer than ``min_detections`` points, or the grid or the baseline is empty, and
-    ``ValueError`` for an unknown ``ephemeris``.
-    """
-    from cuperiod_amd.methods.bls import _grid_segments, _segments_power, sde_of
+    """The multi-band statistic over ``grid``, segmented as the single-band search
+    segments it (src/cuperiod/multiband/bls_mb.py:120-237). Raises
+    :class:`InsufficientDataError` when no band has three usable points, the bands that
+    do hold fewer than ``settings.min_detections``, the times span no baseline or the grid
+    is empty, and ``ValueError`` for an unknown ``ephemeris``."""
+    from cuperiod_amd.methods.bls import _grid_segments, sde_of
 
     mode = resolve_ephemeris(settings, ephemeris)
-    finite = mblc.finite()
-    used = [
-        (name, lc.in_domain(Domain.FLUX))
-        for name, lc in finite.bands.items()
-        if lc.n >= _MIN_BAND_POINTS
-    ]
-    if not used:
-        raise InsufficientDataError("BLS multiband: no band has enough finite points")
-    bands = [lc for _, lc in used]
-    n_total = sum(lc.n for lc in bands)
-    if n_total < settings.min_detections:
-        raise InsufficientDataError(
-            f"BLS multiband: {n_total} finite points across usable bands "
-            f"< min_detections {settings.min_detections}"
-        )
-    stacked_time = np.concatenate([lc.time for lc in bands])
-    baseline = float(stacked_time.max() - stacked_time.min())
-    if baseline <= 0.0:
-        raise InsufficientDataError("BLS multiband: no usable time baseline")
+    bands = _BandSet.select(mblc, settings.min_detections)
+    if not bands.baseline > 0.0:
+        raise InsufficientDataError("BLS multiband: every usable point has the same time")
     segments = _grid_segments(grid, settings)
     if not segments:
         raise InsufficientDataError("BLS multiband: empty trial grid")
-    errors = _band_errors(bands)
-    meta: dict[str, Any] = {
-        **dict(finite.meta),
-        "bands": tuple(name for name, _ in used),
-        "mb_ephemeris": mode,
-    }
-    if mode == "shared":
-        period, power, extras = _shared(bands, errors, segments, settings, backend)
-    else:
-        period, power, extras = _independent(bands, errors, segments, settings, backend,
-                                             _segments_power)
+    run = _shared if mode == "shared" else _independent
+    power, found = run(bands, segments, settings, backend)
     power = np.nan_to_num(power, nan=0.0, posinf=0.0)
+    periods = np.concatenate([np.asarray(p, dtype=np.float64) for p, _ in segments])
+    extras = {name: found[name] for name in ("depth", "depth_snr", "duration", "t0")}
+    extras["sde"] = sde_of(power)
+    extras["band_index"] = found["band_index"]
     return Periodogram.from_spectrum(
-        method="BLS",
-        backend=backend,
-        frequency=1.0 / period,
-        power=power,
-        objective_sense="max",
-        n_samples=n_total,
-        baseline=baseline,
-        extras={
-            "depth": extras["depth"],
-            "depth_snr": extras["depth_snr"],
-            "duration": extras["duration"],
-            "t0": extras["t0"],
-            "sde": sde_of(power),
-            "band_index": extras["band_index"],
-        },
-        meta=meta,
-    )
-
-
-def _independent(bands, errors, segments, settings, backend, segments_power):
-    """The independent per-band stack (bls_mb.py:240-275): ``(period, power, extras)``."""
-    per_band = [
-        segments_power(backend, lc.time, lc.value, err, segments, settings)
-        for lc, err in zip(bands, errors, strict=True)
-    ]
-    parts: dict[str, list[np.ndarray]] = {
-        name: [] for name in ("period", "power", "depth", "depth_snr", "duration", "t0",
-                              "band_index")
-    }
-    for s, (periods, _) in enumerate(segments):
-        seg = [band_segments[s] for band_segments in per_band]
-        dsnr = np.vstack([np.clip(b["depth_snr"], 0.0, None) for b in seg])
-        depth = np.vstack([b["depth"] for b in seg])
-        dur = np.vstack([b["duration"] for b in seg])
-        t0 = np.vstack([b["transit_time"] for b in seg])
-        combined = np.sum(dsnr**2, axis=0)
-        best = np.argmax(dsnr, axis=0)
-        cols = np.arange(dsnr.shape[1])
-        parts["period"].append(np.asarray(periods, dtype=np.float64))
-        parts["power"].append(combined)
-        parts["depth_snr"].append(np.sqrt(combined))
-        parts["depth"].append(depth[best, cols])
-        parts["duration"].append(dur[best, cols])
-        parts["t0"].append(t0[best, cols])
-        parts["band_index"].append(best.astype(np.float64))
-    joined = {name: np.concatenate(values) for name, values in parts.items()}
-    period, power = joined.pop("period"), joined.pop("power")
-    return period, power, joined
-
-
-def _shared(bands, errors, segments, settings, backend):
-    """The joint-ephemeris search (bls_mb.py:278-312): ``(period, power, extras)``."""
-    from cuperiod_amd.methods._bls_joint import joint_bands, joint_search
-
-    prepared = joint_bands([lc.time for lc in bands], [lc.value for lc in bands], errors)
-    found = joint_search(
-        prepared,
-        segments,
-        settings.bins_per_duration,
-        objective=settings.objective,
-        backend=backend,
-        batch=settings.batch_periods,
-        precision=settings.precision,
-    )
-    period = np.concatenate([np.asarray(p, dtype=np.float64) for p, _ in segments])
-
-    def joined(name: str) -> np.ndarray:
-        return np.concatenate([f[name] for f in found])
-
-    extras = {
-        "depth": joined("depth"),
-        "depth_snr": joined("depth_snr"),
-        "duration": joined("duration"),
-        "t0": joined("transit_time"),
-        "band_index": joined("band_index"),
-    }
-    return period, joined("power"), extras
+        method="BLS", backend=backend, frequency=1.0 / periods, power=power,
+        objective_sense="max", n_samples=bands.n_points, baseline=bands.baseline,
+        extras=extras,
+        meta={**dict(mblc.meta), "bands": bands.names, "mb_ephemeris": mode})
 
 
 __all__ = ["EPHEMERIS_MODES", "bls_multiband_power", "resolve_ephemeris"]

@@ -177,16 +177,21 @@ def test_resolve_ephemeris_and_scatter():
     with pytest.raises(ValueError, match="ephemeris"):
         bls_mb.resolve_ephemeris(SETTINGS, "joint")
     flux = np.array([1.0, 2.0, 3.0, 4.0, 100.0])
-    assert bls_mb._robust_scatter(flux) == pytest.approx(1.4826 * 1.0)
-    assert bls_mb._robust_scatter(np.array([1.0, 1.0, 1.0, 2.0])) == pytest.approx(
+    assert bls_mb._noise_scale(flux) == pytest.approx(1.4826 * 1.0)
+    assert bls_mb._noise_scale(np.array([1.0, 1.0, 1.0, 2.0])) == pytest.approx(
         np.std([1.0, 1.0, 1.0, 2.0]))
-    assert bls_mb._robust_scatter(np.ones(4)) == 1.0
+    assert bls_mb._noise_scale(np.ones(4)) == 1.0
     a = cup.LightCurve.from_arrays([0, 1, 2, 3, 4], flux, [1, 1, 1, 1, 1], domain="flux")
     b = cup.LightCurve.from_arrays([0, 1, 2, 3, 4], flux * 2, None, domain="flux")
-    errs = bls_mb._band_errors([a, b])
-    np.testing.assert_allclose(errs[0], 1.4826)
-    np.testing.assert_allclose(errs[1], 2 * 1.4826)
-    np.testing.assert_array_equal(bls_mb._band_errors([a, a])[0], np.ones(5))
+    mixed = bls_mb._BandSet.select(_mblc_of({"a": a, "b": b}), 1)
+    np.testing.assert_allclose(mixed.errors[0], 1.4826)
+    np.testing.assert_allclose(mixed.errors[1], 2 * 1.4826)
+    both = bls_mb._BandSet.select(_mblc_of({"a": a, "c": a}), 1)
+    np.testing.assert_array_equal(both.errors[0], np.ones(5))
+
+
+def _mblc_of(bands):
+    return cup.MultiBandLightCurve.from_light_curves(bands)
 
 
 def test_insufficient_data_is_refused():
