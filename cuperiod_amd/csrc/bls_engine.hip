@@ -82,7 +82,8 @@ __device__ __forceinline__ int64_t block_sum_i64(int64_t v, int64_t* red) {
 }
 
 __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
-    const double* __restrict__ yw, const double* __restrict__ wv,
+    const double* __restrict__ tau, const double* __restrict__ yw,
+    const double* __restrict__ wv,
     const cpb_curve* __restrict__ curves, int32_t obj_flag,
     CurveParams* __restrict__ params, int64_t* __restrict__ yq,
     int64_t* __restrict__ wq) {
@@ -128,6 +129,10 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
         if (b > 0) cmax = fmax(cmax, fabs((double)yq[off + i] / (double)b - ybar));
     }
     cmax = block_max_f64(cmax, red_d);
+    double xmax = 0.0;
+    if (tau != nullptr)
+        for (int64_t i = tid; i < n; i += kPrepThreads) xmax = fmax(xmax, tau[off + i]);
+    xmax = tau != nullptr ? block_max_f64(xmax, red_d) : INFINITY;
 
     if (tid == 0) {
         CurveParams p;
@@ -145,26 +150,37 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
         // s_hi >= 2^26 - 1 (the weights map to [2^58, 2^59) units), so e_unc >= 511.
         const int64_t e = s_hi >> 17 > 1 ? s_hi >> 17 : 1;
         p.e_unc = (int32_t)e;
-        // Bound on |t_screen - t| in high-word units, t = Y - ybar*W: the truncated high
-        // words (1 unit of Y, 1 of W times |ybar|) and the FP32 screen's roundings --
-        // Y and W converted to float (|Y| <= |t| + |ybar| W, W < s: 2^-24 relative
-        // each), ybar rounded to float (2^-24 relative, times W) and the fma itself
-        // (|t| 2^-24, left to the margin): 3 |ybar| s 2^-24 more.
+        // Bound on |t_screen - t| in high-word units, t = Y - ybar*W. The screen reads
+        // t as the difference of two encoded prefixes U = Yh - mean_units(Wh) (Yh, Wh
+        // the high words): the truncated high words (1 unit of Y, 1 of W times |ybar|)
+        // and the two roundings of mean_units (1/2 + 2^-23 each) make 2 + |ybar| +
+        // 2^-22; the conversion of the integer difference to float (|t| 2^-24) is left
+        // to the margin.
         const double s = (double)s_hi, ed = (double)e;
-        const double terr = 1.0 + fabs(ybar) * (1.0 + 3.0 * s * 0x1p-24);
+        const double terr = 2.0 + fabs(ybar) + 0x1p-22;
+        // A box the snr screen leaves uncertified (W*Wout <= dmin) has a side of at most
+        // e + 1 high-word units, so at most e + 3 units of weight: the bounds below
+        // cover e + 3.
         double vdeg, vbnd;
         if (obj_flag == 0) {
             vdeg = 65536.0 * terr * terr / (ed * (s - ed));
-            vbnd = 2.0 * cmax * cmax * (ed + 2.0) / (s - ed - 2.0);
+            vbnd = 2.0 * cmax * cmax * (ed + 3.0) / (s - ed - 3.0);
         } else {
             vdeg = 65536.0 * terr * terr / (ed * ed * (s - ed));
-            vbnd = 2.0 * cmax * cmax * (ed + 2.0) / ((s - ed - 2.0) * (s - ed - 2.0));
+            vbnd = 2.0 * cmax * cmax * (ed + 3.0) / ((s - ed - 3.0) * (s - ed - 3.0));
         }
         p.vdeg = (float)fmin(vdeg, 3.0e38);
         p.vbnd = (float)fmin(vbnd, 3.0e38);
+        // W*Wout > e*s*(1 + 2^-20) (in float, each factor within 2^-24 of its integer)
+        // leaves no side at or below e.
+        p.dmin = (float)(ed * s * (1.0 + 0x1p-20));
+        p.ybar_d = ybar;
+        p.x_max = xmax;
         int flags = 0;
         if (!finite) flags |= 2;
         if (e < 511) flags |= 1;  // too little weight for the screen's bounds: exact sweep
+        // |ybar| * 2^29 < 1/2: mean_units is 0 for every high word
+        if (fabs(ybar) < 0x1p-31) flags |= 4;
         p.flags = flags;
         params[blockIdx.x] = p;
     }
@@ -227,7 +243,13 @@ __device__ __forceinline__ bool exact_key(const BinView<MODE>& bv, int n, int d,
                                           const CurveParams& cv, BoxKey& k) {
     const int2 h0 = bv.hi(n), h1 = bv.hi(n + d);
     const uint2 l0 = bv.lo(n), l1 = bv.lo(n + d);
-    const int64_t yin_q = join64(h1.x, l1.x) - join64(h0.x, l0.x);
+    // the y high words from their encoded form (see the prefix stores)
+    int32_t yh0 = h0.x, yh1 = h1.x;
+    if (!(cv.flags & 4)) {
+        yh0 += mean_units(h0.y, cv.ybar_d);
+        yh1 += mean_units(h1.y, cv.ybar_d);
+    }
+    const int64_t yin_q = join64(yh1, l1.x) - join64(yh0, l0.x);
     const int64_t win_q = join64(h1.y, l1.y) - join64(h0.y, l0.y);
     const int64_t wout_q = cv.w_tot - win_q;
     const double Wi = (double)win_q, Wo = (double)wout_q;
@@ -278,25 +300,32 @@ __device__ __forceinline__ void wave_best_ballot(BoxKey& k, uint32_t& key) {
 }
 
 // FP32 screen value of box (n, d): -t|t| / den with t = Y - ybar*W (a dip has t <= 0),
-// den = W*Wout (snr) or W*Wout^2 (likelihood), from the prefix high words. The high
-// words carry W to within 1 unit and Wout to within 2, so den is within 2/e_unc of its
-// value when both sides hold more than `e_unc` units (snr), or the box more than e_unc
-// and the rest more than 4*e_unc (likelihood: Wout enters squared); `cert` says so.
-// A likelihood box with little weight outside can still score high: `unc` marks those
+// den = W*Wout (snr) or W*Wout^2 (likelihood), from the prefixes' high words: their first
+// word is encoded as U = Yh - mean_units(Wh), so the difference of two is t within
+// 2 + |ybar| units, and the second is Wh. W is within 1 unit and Wout within 2, so den
+// is within a few parts in e_unc of its value when both sides hold more than `e_unc`
+// units; for snr W*Wout > dmin says so, for the likelihood the box must hold more than
+// e_unc and the rest more than 4*e_unc (Wout enters squared). `cert` reports it. A
+// likelihood box with little weight outside can still score high: `unc` marks those
 // (the rest at most 4*e_unc units) for exact evaluation.
 template <int OBJ>
 __device__ __forceinline__ float screen_value(int2 h0, int2 h1, int s_hi, int e_unc,
-                                              float ybar, bool& cert, bool& unc) {
+                                              float dmin, bool& cert, bool& unc) {
     const int dw = h1.y - h0.y;
     const int dwo = s_hi - dw;
-    const float Y = (float)(h1.x - h0.x);
+    const float t = (float)(h1.x - h0.x);
     const float W = (float)dw;
     const float Wo = (float)dwo;
-    const float t = fmaf(-ybar, W, Y);
-    const float den = OBJ == 0 ? W * Wo : W * Wo * Wo;
-    const int e_out = OBJ == 0 ? e_unc : 4 * e_unc;
-    cert = (dw > e_unc) & (dwo > e_out);
-    unc = OBJ == 1 && (dw > e_unc) & (dwo <= e_out) & (dwo > -2);
+    float den;
+    if constexpr (OBJ == 0) {
+        den = W * Wo;
+        cert = den > dmin;
+        unc = false;
+    } else {
+        den = W * Wo * Wo;
+        cert = (dw > e_unc) & (dwo > 4 * e_unc);
+        unc = (dw > e_unc) & (dwo <= 4 * e_unc) & (dwo > -2);
+    }
     return (-t) * fabsf(t) * __builtin_amdgcn_rcpf(den);
 }
 
@@ -382,6 +411,7 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
             }
 
             // (a) zero the slots of bins [0, n_bins], then fold every point.
+            const int span = n_bins + 1;
             {
                 longlong2* S2 = reinterpret_cast<longlong2*>(S);
                 const int last = slot_of(n_bins);
@@ -389,34 +419,43 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
             }
             if constexpr (GLOBAL) __threadfence();
             __syncthreads();
-            // two points per thread per pass, both loaded before either is folded
-            // the lane's addresses are formed per period (the empty asm keeps them from
-            // being hoisted and held in registers across the work item)
-            int jstart = tid;
-            asm volatile("" : "+v"(jstart));
-            for (int j0 = jstart; j0 < npts; j0 += 2 * T) {
-                const int j1 = j0 + T;
-                const double x0 = tau[poff + j0];
-                const int64_t a0 = yq[poff + j0];
-                const int64_t b0 = wq[poff + j0];
-                double x1 = 0.0;
-                int64_t a1 = 0, b1 = 0;
-                if (j1 < npts) {
-                    x1 = tau[poff + j1];
-                    a1 = yq[poff + j1];
-                    b1 = wq[poff + j1];
-                }
-                const int s0 = slot_of(fold_bin(x0, P, invP, bd, inv_bd, width));
-                atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0),
-                          (unsigned long long)a0);
-                atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0 + 1),
-                          (unsigned long long)b0);
-                if (j1 < npts) {
-                    const int s1 = slot_of(fold_bin(x1, P, invP, bd, inv_bd, width));
+            {
+                // this period's near-integer tolerances (fold_bin_tol): x/P <= x_max/P
+                // and w/bd <= width, up to roundings well inside the factor 2 of slack
+                const double tol_q = cv.x_max * invP * 0x1p-48;
+                const double tol_r = (double)width * 0x1p-48;
+                const double* __restrict__ tc = tau + poff;
+                const int64_t* __restrict__ yc = yq + poff;
+                const int64_t* __restrict__ wc = wq + poff;
+                // the lane's first point is formed per period (the empty asm keeps it
+                // from being hoisted and held in a register across the work item)
+                int j0 = tid;
+                asm volatile("" : "+v"(j0));
+                // two points per thread per pass, both loaded before either is folded
+                for (; j0 + T < npts; j0 += 2 * T) {
+                    const int j1 = j0 + T;
+                    const double x0 = tc[j0], x1 = tc[j1];
+                    const int64_t a0 = yc[j0], b0 = wc[j0];
+                    const int64_t a1 = yc[j1], b1 = wc[j1];
+                    const int s0 = slot_of(fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r));
+                    const int s1 = slot_of(fold_bin_tol(x1, P, invP, bd, inv_bd, width, tol_q, tol_r));
+                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0),
+                              (unsigned long long)a0);
+                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0 + 1),
+                              (unsigned long long)b0);
                     atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s1),
                               (unsigned long long)a1);
                     atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s1 + 1),
                               (unsigned long long)b1);
+                }
+                if (j0 < npts) {
+                    const double x0 = tc[j0];
+                    const int64_t a0 = yc[j0], b0 = wc[j0];
+                    const int s0 = slot_of(fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r));
+                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0),
+                              (unsigned long long)a0);
+                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0 + 1),
+                              (unsigned long long)b0);
                 }
             }
             if constexpr (GLOBAL) __threadfence();
@@ -440,21 +479,27 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
             }
 
             // (b1) inclusive prefix over [0, n_bins]: every thread sums a contiguous
-            // chunk, one DPP scan per wave and one exchange between waves give each
-            // chunk its offset.
+            // chunk of c bins, one DPP scan per wave and one exchange between waves give
+            // each chunk its offset. MODE 0 makes c odd: the lanes' 8-byte stores, c
+            // entries apart, then fall on distinct LDS banks. The y high word is stored
+            // as U = Yh - mean_units(Wh, ybar): the difference of two such words is the
+            // box's Y - ybar*W to within 2 + |ybar| units.
             {
-                const int span = n_bins + 1;
-                const int c = (span + T - 1) / T;
-                const int base = tid * c;
                 const longlong2* S2 = reinterpret_cast<const longlong2*>(S);
+                const bool plain = (cv.flags & 4) != 0;
                 int64_t sy = 0, sw = 0;
-                int64_t ry[CHUNK], rw[CHUNK];
-                if (MODE == 0) {
+                int64_t ry[CHUNK + 1], rw[CHUNK + 1];
+                const int c = MODE == 0 ? ((span + T - 1) / T) | 1 : (span + T - 1) / T;
+                const int base = tid * c;
+                if constexpr (MODE == 0) {
+                    // the overwritten bins of a fused wrap copy read their source bins
+                    // (only the one or two lanes whose chunk holds them)
+                    const bool wrap_here = fused_wrap && base + c > wrap_lo && base < n_bins;
 #pragma unroll
-                    for (int kk = 0; kk < CHUNK; ++kk) {
+                    for (int kk = 0; kk <= CHUNK; ++kk) {
                         const int i = base + kk;
                         if (kk < c && i < span) {
-                            const int src = (fused_wrap && i >= wrap_lo && i < n_bins)
+                            const int src = (wrap_here && i >= wrap_lo && i < n_bins)
                                                 ? 1 + (i - wrap_lo)
                                                 : i;
                             const longlong2 v = S2[slot_of(src)];
@@ -487,16 +532,19 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                     oy += s_red_i[2 * w];
                     ow += s_red_i[2 * w + 1];
                 }
-                if (MODE == 0) {
+                if constexpr (MODE == 0) {
                     int2* Hw = reinterpret_cast<int2*>(S);
                     uint2* Lw = reinterpret_cast<uint2*>(reinterpret_cast<char*>(S) + 8 * (size_t)wl);
 #pragma unroll
-                    for (int kk = 0; kk < CHUNK; ++kk) {
+                    for (int kk = 0; kk <= CHUNK; ++kk) {
                         const int i = base + kk;
                         if (kk < c && i < span) {
                             const uint64_t vy = (uint64_t)(ry[kk] + oy);
                             const uint64_t vw = (uint64_t)(rw[kk] + ow);
-                            Hw[i] = make_int2((int32_t)(vy >> 32), (int32_t)(vw >> 32));
+                            const int32_t wh = (int32_t)(vw >> 32);
+                            int32_t u = (int32_t)(vy >> 32);
+                            if (!plain) u -= mean_units(wh, cv.ybar_d);
+                            Hw[i] = make_int2(u, wh);
                             Lw[i] = make_uint2((uint32_t)vy, (uint32_t)vw);
                         }
                     }
@@ -509,9 +557,10 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                             const longlong2 v = reinterpret_cast<const longlong2*>(S4)[s];
                             oy += v.x;
                             ow += v.y;
-                            S4[s] = make_int4((int32_t)((uint64_t)oy >> 32),
-                                              (int32_t)((uint64_t)ow >> 32),
-                                              (int32_t)(uint32_t)oy, (int32_t)(uint32_t)ow);
+                            const int32_t wh = (int32_t)((uint64_t)ow >> 32);
+                            int32_t u = (int32_t)((uint64_t)oy >> 32);
+                            if (!plain) u -= mean_units(wh, cv.ybar_d);
+                            S4[s] = make_int4(u, wh, (int32_t)(uint32_t)oy, (int32_t)(uint32_t)ow);
                         }
                     }
                 }
@@ -521,14 +570,14 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
 
             // (b2) FP32 screen: best value (as non-negative float bits) per (thread,
             // duration); boxes a screen value cannot rank are flagged for (b3).
-            const float ybar = cv.ybar;
             const int s_hi = cv.s_hi;
             const int e_unc = cv.e_unc;
+            const float dmin = cv.dmin;
             int cm[DMAX];
             int dd[DMAX];
             uint32_t unc_bits = 0;
             uint32_t dok = 0;  // durations with 1 <= d < n_bins (uniform)
-            int dmax_ok = 0;  // the longest duration in use (uniform)
+            int dmax_ok = 0;   // the longest duration in use (uniform)
 #pragma unroll
             for (int di = 0; di < DMAX; ++di) {
                 cm[di] = 0;
@@ -538,33 +587,51 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                     dmax_ok = max(dmax_ok, dd[di]);
                 }
             }
-            auto screen_box = [&](int n, int2 h0, int di, int d) {
+            auto screen_box = [&](int2 h0, int2 h1, int di) {
                 bool cert, unc;
-                const float v = screen_value<OBJ>(h0, bv.hi(n + d), s_hi, e_unc, ybar, cert, unc);
+                const float v = screen_value<OBJ>(h0, h1, s_hi, e_unc, dmin, cert, unc);
                 cm[di] = max(cm[di], cert ? __float_as_int(v) : 0);
                 if (OBJ == 1 && unc) unc_bits |= 1u << di;
             };
-            // Start-outer, duration-inner: one read of H[n] per start, and the
+            // Start-outer, duration-inner: one read of the prefix at n per start, and the
             // durations' boxes are independent work for the scheduler. Rounds of starts
             // whose every box ends within the bins skip the per-box bound check.
-            for (int n0 = 0; n0 < n_bins; n0 += T) {
+            int n_first = 0;  // first start of the checked rounds
+            if (MODE == 0 && n_dur == DMAX && dok == (DMAX >= 32 ? 0xFFFFFFFFu : (1u << (DMAX & 31)) - 1u)) {
+                // every duration of the template in use: the full rounds are unrolled
+                // (span <= T*CHUNK), reading all durations' prefixes before the arithmetic
+#pragma unroll
+                for (int r = 0; r < CHUNK; ++r) {
+                    if (r * T + T - 1 + dmax_ok > n_bins) break;
+                    const int n = r * T + tid;
+                    const int2 h0 = bv.hi(n);
+                    int2 h1[DMAX];
+#pragma unroll
+                    for (int di = 0; di < DMAX; ++di) h1[di] = bv.hi(n + dd[di]);
+#pragma unroll
+                    for (int di = 0; di < DMAX; ++di) screen_box(h0, h1[di], di);
+                    n_first = r * T + T;
+                }
+            }
+            for (int n0 = n_first; n0 < n_bins; n0 += T) {
                 const int n = n0 + tid;
                 if (n0 + T - 1 + dmax_ok <= n_bins) {
                     const int2 h0 = bv.hi(n);
 #pragma unroll
                     for (int di = 0; di < DMAX; ++di)
-                        if ((dok >> di) & 1u) screen_box(n, h0, di, dd[di]);
+                        if ((dok >> di) & 1u) screen_box(h0, bv.hi(n + dd[di]), di);
                 } else if (n < n_bins) {
                     const int2 h0 = bv.hi(n);
 #pragma unroll
                     for (int di = 0; di < DMAX; ++di)
-                        if (((dok >> di) & 1u) && n + dd[di] <= n_bins) screen_box(n, h0, di, dd[di]);
+                        if (((dok >> di) & 1u) && n + dd[di] <= n_bins)
+                            screen_box(h0, bv.hi(n + dd[di]), di);
                 }
             }
-            int vmax = 0;
+            int lane_max = 0;
 #pragma unroll
-            for (int di = 0; di < DMAX; ++di) vmax = max(vmax, cm[di]);
-            vmax = wave_max_nonneg_dpp(vmax);
+            for (int di = 0; di < DMAX; ++di) lane_max = max(lane_max, cm[di]);
+            const int vmax = wave_max_nonneg_dpp(lane_max);
             if (lane == 0) s_red_v[wave] = vmax;
             __syncthreads();
             int Vb = s_red_v[0];
@@ -576,35 +643,38 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
             const int thr_bits = __float_as_int(thr);
 
             // (b3) exact re-evaluation of what the screen kept, first wins: the wanted
-            // durations of this thread, in ascending order.
+            // durations of this thread, in ascending order. Waves none of whose lanes
+            // reached the threshold (or holds an uncertain box) have nothing to do.
             BoxKey best{-1.0, 1.0, 0, 0};
             uint32_t bkey = 0xFFFFFFFFu;
-            uint32_t want_bits = 0;
+            if (__ballot(force || lane_max >= thr_bits || unc_bits != 0) != 0) {
+                uint32_t want_bits = 0;
 #pragma unroll
-            for (int di = 0; di < DMAX; ++di)
-                if (force || (cm[di] >= thr_bits) || ((unc_bits >> di) & 1u))
-                    want_bits |= 1u << di;
-            want_bits &= dok;
-            while (want_bits) {
-                const int di = __builtin_ctz(want_bits);
-                want_bits &= want_bits - 1;
-                const int d = s_dur[di];
-                const int m = n_bins - d + 1;
-                for (int n = tid; n < m; n += T) {
-                    bool cert, unc;
-                    const float v = screen_value<OBJ>(bv.hi(n), bv.hi(n + d), s_hi, e_unc,
-                                                      ybar, cert, unc);
-                    if (force || (cert && v >= thr) || unc) {
-                        BoxKey kb;
-                        if (exact_key<OBJ, MODE>(bv, n, d, cv, kb) &&
-                            kb.num2 * best.den > best.num2 * kb.den) {
-                            best = kb;
-                            bkey = ((uint32_t)di << 24) | (uint32_t)n;
+                for (int di = 0; di < DMAX; ++di)
+                    if (force || (cm[di] >= thr_bits) || ((unc_bits >> di) & 1u))
+                        want_bits |= 1u << di;
+                want_bits &= dok;
+                while (want_bits) {
+                    const int di = __builtin_ctz(want_bits);
+                    want_bits &= want_bits - 1;
+                    const int d = s_dur[di];
+                    const int m = n_bins - d + 1;
+                    for (int n = tid; n < m; n += T) {
+                        bool cert, unc;
+                        const float v = screen_value<OBJ>(bv.hi(n), bv.hi(n + d), s_hi, e_unc,
+                                                          dmin, cert, unc);
+                        if (force || (cert && v >= thr) || unc) {
+                            BoxKey kb;
+                            if (exact_key<OBJ, MODE>(bv, n, d, cv, kb) &&
+                                kb.num2 * best.den > best.num2 * kb.den) {
+                                best = kb;
+                                bkey = ((uint32_t)di << 24) | (uint32_t)n;
+                            }
                         }
                     }
                 }
+                wave_best_ballot(best, bkey);
             }
-            wave_best_ballot(best, bkey);
             if (lane == 0) {
                 s_red_b[wave] = best;
                 s_red_k[wave] = bkey;
@@ -1374,11 +1444,12 @@ __global__ __launch_bounds__(kWave) void peak_rounding_kernel(
 
 namespace cpb_engine {
 
-hipError_t launch_prep_curves(hipStream_t st, const double* yw, const double* wv,
-                              const cpb_curve* curves, int32_t n_curves, int32_t obj_flag,
-                              CurveParams* params, int64_t* yq, int64_t* wq) {
-    hipLaunchKernelGGL(prep_curves_kernel, dim3(n_curves), dim3(kPrepThreads), 0, st, yw, wv,
-                       curves, obj_flag, params, yq, wq);
+hipError_t launch_prep_curves(hipStream_t st, const double* tau, const double* yw,
+                              const double* wv, const cpb_curve* curves, int32_t n_curves,
+                              int32_t obj_flag, CurveParams* params, int64_t* yq,
+                              int64_t* wq) {
+    hipLaunchKernelGGL(prep_curves_kernel, dim3(n_curves), dim3(kPrepThreads), 0, st, tau, yw,
+                       wv, curves, obj_flag, params, yq, wq);
     return hipGetLastError();
 }
 
@@ -1685,8 +1756,8 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     const auto* csp = reinterpret_cast<const int32_t*>(db + off_cs);
     const auto* cip = reinterpret_cast<const int32_t*>(db + off_ci);
 
-    hipLaunchKernelGGL(prep_curves_kernel, dim3(n_curves), dim3(kPrepThreads), 0, st, yw,
-                       wv, curve_in, obj_flag,
+    hipLaunchKernelGGL(prep_curves_kernel, dim3(n_curves), dim3(kPrepThreads), 0, st, tau,
+                       yw, wv, curve_in, obj_flag,
                        cparams, yq_d, wq_d);
     CPB_HIP(ctx, hipGetLastError());
 

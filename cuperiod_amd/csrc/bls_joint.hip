@@ -848,7 +848,7 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
     auto* yq = static_cast<int64_t*>(ctx->yq.ptr);
     auto* wq = static_cast<int64_t*>(ctx->wq.ptr);
 
-    hipError_t e = launch_prep_curves(st, yw, wv, curve_in, n_bands, obj_flag, params, yq, wq);
+    hipError_t e = launch_prep_curves(st, nullptr, yw, wv, curve_in, n_bands, obj_flag, params, yq, wq);
     if (e != hipSuccess) return hip_fail(ctx, e, "prep_curves_kernel launch");
     if (n_items == 0) return CPB_OK;
 

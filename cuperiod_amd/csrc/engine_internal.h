@@ -47,7 +47,11 @@ struct CurveParams {
     int32_t n_points;
     int32_t s_hi;     // w_tot >> 32
     int32_t e_unc;    // high-word weight below which a box side is "uncertain"
-    int32_t flags;    // bit 0: every box exactly; bit 1: non-finite input (no box)
+    int32_t flags;    // bit 0: every box exactly; bit 1: non-finite input (no box);
+                      // bit 2: mean_units(wh, ybar_d) is 0 for every high word
+    double ybar_d;    // y_tot / w_tot in FP64 (the screen's prefix encoding)
+    double x_max;     // largest time offset of the curve (infinity when not known)
+    float dmin;       // snr screen: W*Wout above this means both sides hold > e_unc
 };
 
 struct WorkItem {
@@ -107,6 +111,31 @@ __device__ __forceinline__ int fold_bin(double x, double P, double invP, double 
     ind = ind < 0 ? 0 : ind;
     ind = ind > width - 1 ? width - 1 : ind;
     return ind;
+}
+
+// fold_bin with the two near-integer tests against fixed tolerances of one period:
+// `tol_q` >= 2^-49 max(x/P) and `tol_r` >= 2^-49 max(w/bd). A reciprocal quotient is
+// within 2^-51 |q| of the FP64 division's, so floor (trunc) of the two can differ only
+// when an integer lies within that distance; the test sends those points, and a few
+// more, to the exact division. The result is fold_bin's.
+__device__ __forceinline__ int fold_bin_tol(double x, double P, double invP, double bd,
+                                            double inv_bd, int width, double tol_q,
+                                            double tol_r) {
+    const double q = x * invP;
+    double k = floor(q);
+    if (fabs(q - rint(q)) <= tol_q) k = floor(x / P);
+    const double w = x - P * k;
+    const double r = w * inv_bd;
+    int j = (int)r;
+    if (fabs(r - rint(r)) <= tol_r) j = (int)trunc(w / bd);
+    return min(max(j + 1, 0), width - 1);
+}
+
+// The part of a prefix's high word that the curve's mean accounts for: rint(ybar * wh).
+// |ybar| < 2 and |wh| < 2^29, so the product is within 2^-23 of exact and the result
+// fits an int32; the same expression encodes and decodes a prefix, so both agree.
+__device__ __forceinline__ int32_t mean_units(int32_t wh, double ybar) {
+    return (int32_t)rint(ybar * (double)wh);
 }
 
 __device__ __forceinline__ double segment_period(const cpb_segment& sg,
@@ -289,8 +318,10 @@ inline int device_buffer(cpb_context* ctx, DevBuf& b, size_t bytes, void** out) 
 
 // Fixed-point conversion of every curve's points and its parameters (one workgroup per
 // curve); defined with the kernel in bls_engine.hip.
-hipError_t launch_prep_curves(hipStream_t st, const double* yw, const double* wv,
-                              const cpb_curve* curves, int32_t n_curves, int32_t obj_flag,
-                              CurveParams* params, int64_t* yq, int64_t* wq);
+// `tau` (the time offsets) may be NULL: x_max is then infinity.
+hipError_t launch_prep_curves(hipStream_t st, const double* tau, const double* yw,
+                              const double* wv, const cpb_curve* curves, int32_t n_curves,
+                              int32_t obj_flag, CurveParams* params, int64_t* yq,
+                              int64_t* wq);
 
 }  // namespace cpb_engine

@@ -375,7 +375,7 @@ extern "C" int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau,
     auto* params = static_cast<CurveParams*>(ctx->curves.ptr);
     auto* yq = static_cast<int64_t*>(ctx->yq.ptr);
     auto* wq = static_cast<int64_t*>(ctx->wq.ptr);
-    hipError_t e = launch_prep_curves(st, yw, wv, reinterpret_cast<const cpb_curve*>(db),
+    hipError_t e = launch_prep_curves(st, nullptr, yw, wv, reinterpret_cast<const cpb_curve*>(db),
                                       n_curves, 0, params, yq, wq);
     if (e != hipSuccess) return hip_fail(ctx, e, "prep_curves_kernel launch");
     if (n_items == 0) return CPB_OK;
