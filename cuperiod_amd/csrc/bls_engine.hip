@@ -189,15 +189,16 @@ __global__ __launch_bounds__(kPrepThreads) void prep_curves_kernel(
 // ---------------------------------------------------------------------------------
 // Main kernel.
 //
-// LDS (or, in MODE 2, global scratch) of one workgroup, in 16-byte slots of int64
-// pairs: during the fold, slot_of(i) = i + i/8 holds the fixed-point sums (y, w) of
-// phase bin i; the pad slot after every 8 bins puts the 8 bins a thread scans on
-// distinct banks. After the prefix:
-//   MODE 0 (span <= T*kChunk, LDS): dense arrays H[i] = (cy_hi, cw_hi) from byte 0 and
-//          L[i] = (cy_lo, cw_lo) from byte 8*wl, written over the slots;
-//   MODE 1 (LDS) and MODE 2 (global scratch): in place, slot_of(i) = (cy_hi, cw_hi,
-//          cy_lo, cw_lo).
-// cy/cw are the inclusive prefix sums; hi/lo their 32-bit words.
+// LDS (or, in MODE 2, global scratch) of one workgroup. During the fold, the
+// fixed-point sums (y, w) of phase bin i:
+//   MODE 0 (span <= T*kChunk, LDS, 16*wl bytes): y in Y[i] from byte 0, w in W[i] from
+//          byte 8*wl. After the prefix, H[i] = (U, cw_hi) over Y and L[i] = (cy_lo,
+//          cw_lo) over W;
+//   MODE 1 (LDS) and MODE 2 (global scratch): 16-byte slots, slot_of(i) = i + i/8 holds
+//          (y, w) (the pad slot after every 8 bins puts the 8 bins a thread scans on
+//          distinct banks), and after the prefix (U, cw_hi, cy_lo, cw_lo) in place.
+// cy/cw are the inclusive prefix sums, hi/lo their 32-bit words, and U = cy_hi -
+// mean_units(cw_hi, ybar) the encoded y high word.
 
 constexpr int kChunk = 8;
 // Periods per work item (the item's period table lives in LDS).
@@ -410,9 +411,17 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                 continue;
             }
 
-            // (a) zero the slots of bins [0, n_bins], then fold every point.
+            // (a) zero the bins [0, n_bins], then fold every point.
             const int span = n_bins + 1;
-            {
+            if constexpr (MODE == 0) {
+                // pairs of bins; 2*ceil(span/2) <= wl, as wl is a multiple of 64
+                longlong2* Y2 = reinterpret_cast<longlong2*>(S);
+                longlong2* W2 = reinterpret_cast<longlong2*>(S + wl);
+                for (int i = tid; i < (span + 1) / 2; i += T) {
+                    Y2[i] = make_longlong2(0, 0);
+                    W2[i] = make_longlong2(0, 0);
+                }
+            } else {
                 longlong2* S2 = reinterpret_cast<longlong2*>(S);
                 const int last = slot_of(n_bins);
                 for (int i = tid; i <= last; i += T) S2[i] = make_longlong2(0, 0);
@@ -424,38 +433,48 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                 // and w/bd <= width, up to roundings well inside the factor 2 of slack
                 const double tol_q = cv.x_max * invP * 0x1p-48;
                 const double tol_r = (double)width * 0x1p-48;
-                const double* __restrict__ tc = tau + poff;
-                const int64_t* __restrict__ yc = yq + poff;
-                const int64_t* __restrict__ wc = wq + poff;
+                // point loads at 32-bit byte offsets from the curve's (uniform) bases
+                const char* __restrict__ tc = reinterpret_cast<const char*>(tau + poff);
+                const char* __restrict__ yc = reinterpret_cast<const char*>(yq + poff);
+                const char* __restrict__ wc = reinterpret_cast<const char*>(wq + poff);
+                auto point = [&](int j, double& x, int64_t& a, int64_t& b) {
+                    const uint32_t off = (uint32_t)j * 8u;
+                    x = *reinterpret_cast<const double*>(tc + off);
+                    a = *reinterpret_cast<const int64_t*>(yc + off);
+                    b = *reinterpret_cast<const int64_t*>(wc + off);
+                };
+                // bin i's y and w accumulators
+                unsigned long long* const acc = reinterpret_cast<unsigned long long*>(S);
+                auto add_point = [&](int bin, int64_t a, int64_t b) {
+                    if constexpr (MODE == 0) {
+                        atomicAdd(acc + bin, (unsigned long long)a);
+                        atomicAdd(acc + wl + bin, (unsigned long long)b);
+                    } else {
+                        const int sl = slot_of(bin);
+                        atomicAdd(acc + 2 * sl, (unsigned long long)a);
+                        atomicAdd(acc + 2 * sl + 1, (unsigned long long)b);
+                    }
+                };
                 // the lane's first point is formed per period (the empty asm keeps it
                 // from being hoisted and held in a register across the work item)
                 int j0 = tid;
                 asm volatile("" : "+v"(j0));
                 // two points per thread per pass, both loaded before either is folded
                 for (; j0 + T < npts; j0 += 2 * T) {
-                    const int j1 = j0 + T;
-                    const double x0 = tc[j0], x1 = tc[j1];
-                    const int64_t a0 = yc[j0], b0 = wc[j0];
-                    const int64_t a1 = yc[j1], b1 = wc[j1];
-                    const int s0 = slot_of(fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r));
-                    const int s1 = slot_of(fold_bin_tol(x1, P, invP, bd, inv_bd, width, tol_q, tol_r));
-                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0),
-                              (unsigned long long)a0);
-                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0 + 1),
-                              (unsigned long long)b0);
-                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s1),
-                              (unsigned long long)a1);
-                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s1 + 1),
-                              (unsigned long long)b1);
+                    double x0, x1;
+                    int64_t a0, b0, a1, b1;
+                    point(j0, x0, a0, b0);
+                    point(j0 + T, x1, a1, b1);
+                    const int i0 = fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r);
+                    const int i1 = fold_bin_tol(x1, P, invP, bd, inv_bd, width, tol_q, tol_r);
+                    add_point(i0, a0, b0);
+                    add_point(i1, a1, b1);
                 }
                 if (j0 < npts) {
-                    const double x0 = tc[j0];
-                    const int64_t a0 = yc[j0], b0 = wc[j0];
-                    const int s0 = slot_of(fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r));
-                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0),
-                              (unsigned long long)a0);
-                    atomicAdd(reinterpret_cast<unsigned long long*>(S + 2 * s0 + 1),
-                              (unsigned long long)b0);
+                    double x0;
+                    int64_t a0, b0;
+                    point(j0, x0, a0, b0);
+                    add_point(fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r), a0, b0);
                 }
             }
             if constexpr (GLOBAL) __threadfence();
@@ -471,7 +490,14 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                     longlong2* S2 = reinterpret_cast<longlong2*>(S);
                     for (int kk = 0; kk < oversample; ++kk) {
                         const int dst = n_bins - oversample + kk;
-                        if (dst >= 0 && dst < width) S2[slot_of(dst)] = S2[slot_of(1 + kk)];
+                        if (dst >= 0 && dst < width) {
+                            if constexpr (MODE == 0) {
+                                S[dst] = S[1 + kk];
+                                S[wl + dst] = S[wl + 1 + kk];
+                            } else {
+                                S2[slot_of(dst)] = S2[slot_of(1 + kk)];
+                            }
+                        }
                     }
                 }
                 if constexpr (GLOBAL) __threadfence();
@@ -502,9 +528,8 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                             const int src = (wrap_here && i >= wrap_lo && i < n_bins)
                                                 ? 1 + (i - wrap_lo)
                                                 : i;
-                            const longlong2 v = S2[slot_of(src)];
-                            sy += v.x;
-                            sw += v.y;
+                            sy += S[src];
+                            sw += S[wl + src];
                         }
                         ry[kk] = sy;
                         rw[kk] = sw;
@@ -1641,9 +1666,10 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     // launch per layout, so a wide segment does not push narrow ones out of LDS. (The
     // reduced-precision kernel takes every item in one launch.)
     const int threads = ctx->threads;
-    // Classes: 0 MODE 0 up to 1,024 bins (18 KiB of LDS; its own instantiation for 8
-    // waves per SIMD), 1-3 MODE 0 up to 1,408 / 1,728 / threads*8 bins (26, 31, 37 KiB:
-    // the most bins that leave room for 6, 5, 4 workgroups per CU), 4 MODE 1, 5 MODE 2.
+    // Classes: 0 MODE 0 up to 1,024 bins (16 KiB of LDS; its own instantiation for 8
+    // waves per SIMD), 1-3 MODE 0 up to 1,600 / 1,920 / threads*8 bins (16 bytes a bin:
+    // 25, 30, 32 KiB, the most bins that leave room for 6, 5, 4 workgroups in a CU's
+    // 160 KiB with up to 1.25 KiB of static LDS each), 4 MODE 1, 5 MODE 2.
     constexpr int kClasses = 6;
     auto layout_of = [&](int width) {
         if (ctx->accumulation != 0) return 0;
@@ -1651,7 +1677,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         const size_t need = (size_t)16 * (w + (w - 1) / 8 + 1);
         if (need + 2048 > (size_t)ctx->lds_limit) return 5;
         if (w > threads * kChunk) return 4;
-        return w <= 1024 ? 0 : (w <= 1408 ? 1 : (w <= 1728 ? 2 : 3));
+        return w <= 1024 ? 0 : (w <= 1600 ? 1 : (w <= 1920 ? 2 : 3));
     };
     constexpr int kModeOf[kClasses] = {3, 0, 0, 0, 1, 2};
     struct Layout {
@@ -1808,9 +1834,9 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         const Layout& lay = layouts[L];
         if (lay.count == 0) continue;
         const int wlL = (lay.wmax + 63) / 64 * 64;
-        const int n_slots = wlL + (wlL - 1) / 8 + 1;
-        const size_t lds_need = (size_t)16 * n_slots;
         const int mode = kModeOf[L];
+        const int n_slots = wlL + (wlL - 1) / 8 + 1;
+        const size_t lds_need = (size_t)16 * (mode == 0 || mode == 3 ? wlL : n_slots);
         (void)static_lds;
         int64_t* wide_ptr = nullptr;
         LaunchArgs la;
