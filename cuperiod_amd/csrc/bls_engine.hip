@@ -1245,14 +1245,19 @@ constexpr int kPeakCap = 64;       // re-searched periods per reference segment,
 constexpr int kPeakThreads = 256;
 
 // One workgroup per reference segment g (engine segments seg_idx[seg_start[g] ..
-// seg_start[g+1]), all of one curve): the largest sanitised power, then every period
-// within kPeakBand of it, as (engine segment, index in segment) pairs.
+// seg_start[g+1]), all of one curve): the largest sanitised power, then the periods
+// within kPeakBand of it, as (engine segment, index in segment) pairs. When more than
+// kPeakCap qualify, the choice is the same on every run: first the periods at the
+// maximum itself, then the others, each in (segment, index) order; slots are handed out
+// by a block-wide count of each 256-period stretch, not by atomics.
 __global__ __launch_bounds__(kPeakThreads) void peak_band_kernel(
     const double* __restrict__ out, const cpb_segment* __restrict__ segs,
     const int32_t* __restrict__ seg_start, const int32_t* __restrict__ seg_idx,
     const cpb_curve* __restrict__ curves, int2* __restrict__ cand,
     int32_t* __restrict__ count) {
-    __shared__ double red[kPeakThreads / kWave];
+    constexpr int NW = kPeakThreads / kWave;
+    __shared__ double red[NW];
+    __shared__ int32_t s_cnt[NW];
     __shared__ int32_t s_n;
     const int c = blockIdx.x;  // the group
     const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
@@ -1268,23 +1273,45 @@ __global__ __launch_bounds__(kPeakThreads) void peak_band_kernel(
     if (tid == 0) s_n = 0;
     __syncthreads();
     m = red[0];
-    for (int w = 1; w < kPeakThreads / kWave; ++w) m = fmax(m, red[w]);
-    const cpb_curve cv = curves[segs[seg_idx[s0]].curve];
-    if (m > 0.0 && cv.sum_ivar > 0.0 && isfinite(cv.sum_y) && isfinite(cv.sum_ivar)) {
-        const double thr = m - m * kPeakBand;
+    for (int w = 1; w < NW; ++w) m = fmax(m, red[w]);
+    // appends the periods with lo <= power <= hi in order until kPeakCap are held
+    auto take = [&](double lo, double hi) {
         for (int s = s0; s < s1; ++s) {
             const int si = seg_idx[s];
             const cpb_segment& sg = segs[si];
-            for (int64_t k = tid; k < sg.n_periods; k += kPeakThreads) {
-                if (sanitize(out[sg.out_offset + k]) >= thr) {
-                    const int i = atomicAdd(&s_n, 1);
-                    if (i < kPeakCap) cand[(int64_t)c * kPeakCap + i] = make_int2(si, (int)k);
+            for (int64_t k0 = 0; k0 < sg.n_periods; k0 += kPeakThreads) {
+                const int taken = s_n;  // uniform: written before the last barrier
+                if (taken >= kPeakCap) return;
+                const int64_t k = k0 + tid;
+                bool hit = false;
+                if (k < sg.n_periods) {
+                    const double p = sanitize(out[sg.out_offset + k]);
+                    hit = p >= lo && p <= hi;
                 }
+                const unsigned long long bal = __ballot(hit);
+                if (lane == 0) s_cnt[wave] = __popcll(bal);
+                __syncthreads();
+                int slot = taken + __popcll(bal & ((1ull << lane) - 1ull));
+                int total = 0;
+                for (int w = 0; w < NW; ++w) {
+                    if (w < wave) slot += s_cnt[w];
+                    total += s_cnt[w];
+                }
+                if (hit && slot < kPeakCap)
+                    cand[(int64_t)c * kPeakCap + slot] = make_int2(si, (int)k);
+                __syncthreads();  // s_cnt and s_n are read
+                if (tid == 0) s_n = taken + total;
+                __syncthreads();
             }
         }
+    };
+    const cpb_curve cv = curves[segs[seg_idx[s0]].curve];
+    if (m > 0.0 && cv.sum_ivar > 0.0 && isfinite(cv.sum_y) && isfinite(cv.sum_ivar)) {
+        take(m, m);
+        take(m - m * kPeakBand, nextafter(m, 0.0));
     }
     __syncthreads();
-    if (tid == 0) count[c] = s_n;
+    if (tid == 0) count[c] = min(s_n, kPeakCap);
 }
 
 // One wave per (reference segment, candidate): the reference's FP64 search of that

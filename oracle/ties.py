@@ -128,19 +128,25 @@ def prove_same_points(got: dict, ref: dict, inp: BoxInputs, idx: np.ndarray,
 
 
 def prove_cross_period_tie(ref: dict, inp: BoxInputs, i: int, j: int,
-                           name: str = "") -> None:
-    """Assert that periods ``i`` and ``j`` tie exactly in ``ref`` (the oracle's own
-    powers are bit-identical) and that their boxes hold the same multiset of points, so
-    their exact statistics are equal: the reference's choice between them is decided
-    by numpy's (unstable) sort order of equal scores."""
-    assert ref["power"][i] == ref["power"][j], (
-        f"{name}: periods {i} and {j}: powers {ref['power'][i]!r} vs {ref['power'][j]!r}")
+                           name: str = "") -> int:
+    """Assert that periods ``i`` and ``j`` tie in exact arithmetic: their boxes hold the
+    same multiset of points, so their exact statistics are equal, and the oracle's own
+    FP64 powers there differ by no more than the rounding of summing the same terms in
+    another order (at most 1e-12 relative; the reference's choice between such periods
+    is decided by that rounding and by numpy's unstable sort of equal scores). Returns
+    the distance between the two oracle powers in units in the last place."""
+    pi, pj = float(ref["power"][i]), float(ref["power"][j])
+    ulps = int(round(abs(pi - pj) / np.spacing(max(abs(pi), abs(pj))))) if pi != pj else 0
+    assert abs(pi - pj) <= 1e-12 * max(abs(pi), abs(pj)), (
+        f"{name}: periods {i} and {j}: powers {pi!r} vs {pj!r} ({ulps} ulp apart)")
     bi, bj = recover_boxes(ref, inp, i), recover_boxes(ref, inp, j)
     assert bi and bj, f"{name}: boxes not recoverable at {i}, {j}"
     mi = [box_members(inp, i, n, d) for n, d in bi]
     mj = [box_members(inp, j, n, d) for n, d in bj]
     assert any(np.array_equal(x, y) for x in mi for y in mj), (
-        f"{name}: periods {i} and {j} have equal powers but boxes of different points")
+        f"{name}: periods {i} and {j} have powers {ulps} ulp apart but boxes of "
+        "different points")
+    return ulps
 
 
 __all__ = ["BoxInputs", "box_members", "fold_bins", "prove_cross_period_tie",

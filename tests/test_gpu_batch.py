@@ -87,10 +87,11 @@ def test_survey_best_periods_identical_to_oracle():
     # rounding). Flat-topped peaks (adjacent frequencies that fold the best box's points
     # identically) are exact ties too; the engine searches each segment's near-maximum
     # periods again with the reference's FP64 sequence, so it reports the oracle's sample.
-    # Two separate local maxima with bit-identical oracle powers are ordered by numpy's
-    # sort of the candidate list (not stable), which the device summary (the first
-    # maximum in ascending frequency) cannot follow: there the two periods must be
-    # proven to hold the same points, an exact tie the reference itself cannot break.
+    # Separate periods whose boxes hold the same points have equal exact statistics;
+    # the oracle orders them by FP64 summation-order rounding (a few ulps) and numpy's
+    # unstable sort of equal scores, and a plateau of such periods can be longer than the
+    # engine's re-searched set: where the best periods differ, the two periods must be
+    # proven to hold the same points, an exact tie the reference breaks by rounding.
     from cuperiod_amd.core.result import Periodogram
     from cuperiod_amd.workloads import survey_batch
     from parity import prove_cross_period_tie

@@ -77,3 +77,38 @@ def test_autotuner_picks_a_candidate(packed, tmp_path, monkeypatch):
         assert again.cached and again.shape == res.shape
     finally:
         autotune.apply(autotune.DEFAULT_SHAPE)
+
+
+def test_plateau_choice_does_not_depend_on_the_launch_shape():
+    # One deep point alone in the best box at thousands of periods: more periods tie at
+    # the maximum than the reference rounding searches again, and which of them it takes
+    # must be the same for every launch shape and every run.
+    import torch
+
+    from cuperiod_amd import _lib
+
+    rng = np.random.default_rng(11)
+    t = np.sort(rng.uniform(0.0, 40.0, 60))
+    y = 1.0 + 1e-4 * rng.standard_normal(60)
+    y[17] -= 0.05
+    settings = BLSSettings(n_durations=6)
+    pb = engine.pack([(t, y, np.full(60, 1e-4))], [engine.LinspaceGrid(1 / 40, 2.0, 20_000)],
+                     settings)
+    ctx = _lib.context(0)
+    with ctx.peak_rounding(False):
+        raw, _ = engine.run_packed(pb)
+    power = raw[0].cpu().numpy()
+    assert int((power == np.nanmax(power)).sum()) > 64
+    autotune.apply(autotune.DEFAULT_SHAPE)
+    ref_out, ref_sum = engine.run_packed(pb)
+    ref_out, ref_sum = ref_out.clone(), ref_sum.clone()
+    try:
+        for shape in autotune.CANDIDATES:
+            autotune.apply(shape)
+            for _ in range(2):
+                out, summ = engine.run_packed(pb)
+                torch.cuda.synchronize()
+                assert torch.equal(out, ref_out), shape
+                assert torch.equal(summ, ref_sum), shape
+    finally:
+        autotune.apply(autotune.DEFAULT_SHAPE)

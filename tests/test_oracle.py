@@ -246,3 +246,32 @@ def test_tie_proof_refuses_a_box_with_other_points():
     moved["transit_time"][0] = math.fmod(n2 * bd + 0.5 * d * bd, P) + inp.t_min
     with pytest.raises(AssertionError, match="different points"):
         prove_same_points(ref, moved, inp, np.array([0]))
+
+
+def test_cross_period_tie_needs_the_same_points():
+    # A curve whose deepest point is alone in the best box at many periods: those periods
+    # tie in exact arithmetic (the same single point), and the oracle's FP64 powers there
+    # agree to summation-order rounding; two periods whose boxes hold other points are
+    # refused.
+    from oracle.ties import BoxInputs, box_members, prove_cross_period_tie, recover_boxes
+
+    rng = np.random.default_rng(5)
+    t = np.sort(rng.uniform(0, 40, 60))
+    y = 1 + 1e-4 * rng.standard_normal(60)
+    y[17] -= 0.05
+    dy = np.full(60, 1e-4)
+    periods = np.linspace(2.0, 3.0, 80)
+    durations = np.array([0.01])
+    ref = oracle.bls_power(t, y, dy, periods, durations, 3)
+    inp = BoxInputs.for_specs(t, [(periods, durations)], 3)
+    groups = {}
+    for i in range(periods.size):
+        boxes = recover_boxes(ref, inp, i)
+        assert boxes, i
+        groups.setdefault(tuple(box_members(inp, i, *boxes[0]).tolist()), []).append(i)
+    single = groups.get((17,), [])
+    assert len(single) >= 2
+    assert prove_cross_period_tie(ref, inp, single[0], single[-1]) <= 64
+    other = next(g[0] for k, g in groups.items() if k != (17,))
+    with pytest.raises(AssertionError, match="ulp apart|different points"):
+        prove_cross_period_tie(ref, inp, single[0], other)
