@@ -235,6 +235,54 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def fp64_error_leg(parity: dict | None, long_baseline: dict | None) -> dict:
+    """Largest relative power error against the FP64 oracle on each single-GPU workload:
+    the small curve and the gapped curve searched here, the survey batch's parity sample
+    and the long-baseline precision study taken from their legs."""
+    import oracle
+    from cuperiod_amd.methods import _bls_core as core
+    from cuperiod_amd.workloads import gapped_curve, small_curve
+
+    def rel_error(got, ref) -> float:
+        ok = np.isfinite(got) & np.isfinite(ref)
+        if not ok.any():
+            return 0.0
+        return float(np.max(np.abs(got[ok] - ref[ok]) / np.maximum(np.abs(ref[ok]), 1e-300)))
+
+    found = {}
+    wl = small_curve()
+    t, y, dy = wl.curves[0]
+    periods = wl.grids[0].periods()
+    got = core.bls_power(t, y, dy, periods, wl.durations, 10).power
+    ref = oracle.bls_power(t, y, dy, periods, wl.durations, 10)["power"]
+    best = lambda p: int(np.argmax(np.nan_to_num(p, nan=-np.inf, neginf=-np.inf)))  # noqa: E731
+    found["small curve: N=2,000 over 100 d, 50,000 frequencies, 4 durations"] = {
+        "max_rel_power_error": rel_error(got, ref), "periods_checked": int(periods.size),
+        "best_period_identical": bool(periods[best(got)] == periods[best(ref)])}
+    wl = gapped_curve()
+    t, y, dy = wl.curves[0]
+    periods = wl.grids[0].periods()
+    got = core.bls_power(t, y, dy, periods, wl.durations, 10).power
+    k = best(got)
+    rng = np.random.default_rng(77)
+    idx = np.unique(np.concatenate([rng.choice(periods.size, 3000, replace=False),
+                                    np.arange(max(0, k - 200), min(periods.size, k + 200))]))
+    ref = oracle.bls_power(t, y, dy, periods[idx], wl.durations, 10)["power"]
+    found["gapped curve: N~14,000 over 1,000 d, 1,000,000 frequencies, 8 durations"] = {
+        "max_rel_power_error": rel_error(got[idx], ref), "periods_checked": int(idx.size),
+        "best_period_identical": bool(periods[idx][best(ref)] == periods[k])}
+    if parity:
+        found["survey batch: the parity sample"] = {
+            "max_rel_power_error": parity["max_rel_power_error"],
+            "best_period_match": parity["best_period_match"]}
+    if long_baseline and long_baseline.get("precision"):
+        fx = long_baseline["precision"]["modes"]["fixed64"]
+        found["long-baseline curve: N=100,000, 8,000 frequencies of 4,000,000"] = {
+            "max_rel_power_error": fx["max_rel_power_error"],
+            "best_period_identical": fx["best_period_identical"]}
+    return found
+
+
 def end_to_end_leg(wl, settings, chunk: int) -> dict:
     """``batch_periodograms`` on the whole batch: checks, packing, uploads, search, peak
     candidates and rows, with each curve's 200,000-frequency linspace grid."""
@@ -768,6 +816,7 @@ def main() -> int:
         "end_to_end": None,
         "long_baseline": None,
         "learned": None,
+        "max_error_vs_fp64": None,
         "next_rows": None,
     }
     if not args.no_cpu and args.cpu_curves > 0:
@@ -789,6 +838,10 @@ def main() -> int:
         log("long-baseline batch ...")
         line["long_baseline"] = long_baseline_leg(args.long_curves, not args.no_cpu)
         log(f"long-baseline: {line['long_baseline']['value']} curves/s")
+    if not args.no_cpu:
+        log("FP64 error per workload ...")
+        line["max_error_vs_fp64"] = fp64_error_leg(line["parity"], line["long_baseline"])
+        log(f"FP64 errors: {json.dumps(line['max_error_vs_fp64'])}")
     if not args.no_learned:
         from cuperiod_amd.learned.proposer import DEFAULT_WEIGHTS
 
