@@ -1879,7 +1879,17 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
             // fewer than 2^32 work-items per launch (a graph kernel node refuses more);
             // the kernel loops over the items its grid does not cover
             const int64_t cap = ((int64_t)1 << 31) / threads;
-            la.grid = dim3((unsigned)std::min<int64_t>(lay.count, persistent > 0 ? persistent : cap));
+            // a persistent grid holds no more workgroups per CU than can be resident at
+            // once (the class's LDS, with up to 1.25 KiB of static LDS each, and the
+            // waves its register cap admits); more would run after the others in a tail
+            int64_t g = cap;
+            if (persistent > 0) {
+                const int waves_cap = mode == 3 ? 8 : 6;
+                const int by_waves = std::max(1, 4 * waves_cap / (threads / kWave));
+                const int by_lds = std::max<int>(1, ctx->lds_per_cu / (int)(lds_need + 1280));
+                g = (int64_t)ctx->n_cu * std::min(ctx->grid_per_cu, std::min(by_waves, by_lds));
+            }
+            la.grid = dim3((unsigned)std::min<int64_t>(lay.count, g));
             la.lds = lds_need;
         }
         la.st = st;
@@ -1983,6 +1993,10 @@ int cpb_create(int device, cpb_context** out) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) ==
             hipSuccess && cus > 0)
         ctx->n_cu = cus;
+    int lds_cu = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                              device) == hipSuccess && lds_cu > 0)
+        ctx->lds_per_cu = lds_cu;
     ctx->static_scratch = static_scratch_address(device, ctx->static_error);
     *out = ctx;
     return CPB_OK;

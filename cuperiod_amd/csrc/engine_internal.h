@@ -218,6 +218,7 @@ struct cpb_context {
     int device = 0;
     std::string error;
     int lds_limit = 65536;
+    int lds_per_cu = 163840;
     int n_cu = 1;
     // `tables` holds the curve, segment and work-item descriptors of one call; they
     // are staged in pinned host memory (`h_tables`) and sent in one copy. The event
