@@ -254,11 +254,17 @@ def refine_ranges(prep: PreparedGrids, span_lo: np.ndarray, span_hi: np.ndarray,
 
 
 class LearnedProposer:
-    """Chunked coarse search, network windows, refinement and fallback over a batch."""
+    """Chunked coarse search, network windows, refinement and fallback over a batch.
+
+    The default ``window_mass`` (0.99) and ``threshold`` (0.5) were chosen on 2,048
+    held-out gapped curves (a seed apart from the training and benchmark curves): from
+    mass 0.99 up every setting of the threshold found the plain best period on every
+    curve, and 0.5 still sends the least confident curves (0.05% there) to the full grid.
+    """
 
     def __init__(self, params: dict[str, np.ndarray] | None = None, *, max_windows: int = 32,
-                 radius: int = 1, fine_per_window: int = 2000, window_mass: float = 0.995,
-                 threshold: float = 0.9, chunk_duration: float = 0.2,
+                 radius: int = 1, fine_per_window: int = 2000, window_mass: float = 0.99,
+                 threshold: float = 0.5, chunk_duration: float = 0.2,
                  coarse_oversample: int = 3) -> None:
         if params is None:
             params = load_parameters(DEFAULT_WEIGHTS)
