@@ -20,9 +20,9 @@ For each light curve on a linspace frequency grid (the plain, full-grid search):
    or ``max_windows`` are kept;
 4. a curve whose kept mass is below ``threshold`` falls back to the full grid; every
    other curve is searched at the full grid's frequencies within ``fine_per_window / 2``
-   grid steps of every cell its kept windows claim. That half-width exceeds half a coarse
-   cell on the grids the proposer serves, so a claimed cell that is the nearest one to
-   the best frequency puts the best frequency inside the refined range.
+   grid steps of every cell its kept windows claim, and at least half a coarse cell
+   each side of them, so a claimed cell that is the nearest one to the best frequency
+   puts the best frequency inside the refined range on any grid.
 
 Refinement evaluates the plain grid's own frequencies with the plain search's segment
 binning (the full search's descriptors, cut to the windows), so a refined curve's best
@@ -240,14 +240,17 @@ def refine_ranges(prep: PreparedGrids, span_lo: np.ndarray, span_hi: np.ndarray,
                   curve: np.ndarray):
     """Ascending-period index ranges ``[k0, k1)`` of the full grid of ``curve`` that hold
     every frequency within the refinement half-width of the cells ``span_lo ..
-    span_hi`` (one window per row)."""
+    span_hi`` and every frequency whose nearest coarse cell is one of them (one window
+    per row), so the probability a window claims is always searched."""
     f_lo, dlog, half = (a[curve] for a in prep.geometry_host)
     start = np.array([g.start for g in prep.grids])[curve]
     stop = np.array([g.stop for g in prep.grids])[curve]
     num = np.array([g.num for g in prep.grids], dtype=np.int64)[curve]
     step = (stop - start) / (num - 1)
-    fa = f_lo * np.exp(span_lo * dlog) - half
-    fb = f_lo * np.exp(span_hi * dlog) + half
+    # every frequency within the half-width of a claimed cell, and every frequency whose
+    # nearest coarse cell is claimed (half a cell each side), whichever reaches further
+    fa = np.minimum(f_lo * np.exp(span_lo * dlog) - half, f_lo * np.exp((span_lo - 0.5) * dlog))
+    fb = np.maximum(f_lo * np.exp(span_hi * dlog) + half, f_lo * np.exp((span_hi + 0.5) * dlog))
     i_lo = np.clip(np.floor((fa - start) / step) - 1, 0, num - 1).astype(np.int64)
     i_hi = np.clip(np.ceil((fb - start) / step) + 1, 0, num - 1).astype(np.int64)
     return num - 1 - i_hi, num - i_lo

@@ -3,6 +3,7 @@
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 from cuperiod_amd.batch import engine
 from cuperiod_amd.core.config import BLSSettings
@@ -37,17 +38,19 @@ def test_coarse_plan_uses_the_plain_segments():
     assert [p.size for p, _ in ref] == [sg.n_periods for sg in plan.segments]
 
 
-def _prep_for(grids):
-    f_lo, dlog, half = lp.window_geometry(grids, 2000)
+def _prep_for(grids, fine_per_window=2000):
+    f_lo, dlog, half = lp.window_geometry(grids, fine_per_window)
     return lp.PreparedGrids(cs=None, grids=list(grids), full=None, coarse=None,
                             geometry_host=(f_lo, dlog, half))
 
 
-def test_refine_ranges_hold_every_claimed_cells_target():
+@pytest.mark.parametrize("fine_per_window", [2000, 20])
+def test_refine_ranges_hold_every_claimed_cells_target(fine_per_window):
     # the refined index range must hold every grid frequency within the half-width of
-    # every claimed cell (ADVICE: claimed probability must be searched)
+    # every claimed cell, and every frequency whose nearest cell is claimed; at 20 fine
+    # frequencies per window the half-width is far below half a coarse cell
     grids = [engine.LinspaceGrid(0.001, 2.0, 200_000), engine.LinspaceGrid(0.05, 2.0, 50_000)]
-    prep = _prep_for(grids)
+    prep = _prep_for(grids, fine_per_window)
     f_lo, dlog, half = prep.geometry_host
     spans = [(0, 0, 0), (0, 1500, 1503), (0, L - 3, L - 1), (1, 0, 2), (1, 2000, 2001)]
     curve = np.array([c for c, _, _ in spans])
@@ -61,6 +64,8 @@ def test_refine_ranges_hold_every_claimed_cells_target():
         need = np.zeros(freq.size, bool)
         for fc in cells:
             need |= np.abs(freq - fc) <= half[c]
+        nearest = np.rint(np.log(freq / f_lo[c]) / dlog[c])
+        need |= (nearest >= a) & (nearest <= b)
         i = np.flatnonzero(need)
         k = g.num - 1 - i  # ascending-period index
         assert k.size == 0 or (k0[j] <= k.min() and k1[j] > k.max()), (j, k0[j], k1[j])
