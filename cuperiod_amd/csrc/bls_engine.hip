@@ -517,22 +517,41 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                 int64_t ry[CHUNK + 1], rw[CHUNK + 1];
                 const int c = MODE == 0 ? ((span + T - 1) / T) | 1 : (span + T - 1) / T;
                 const int base = tid * c;
+                // bins [wave_lo, wave_hi) are this wave's (uniform)
+                const int wave_lo = __builtin_amdgcn_readfirstlane(wave) * kWave * c;
+                const int wave_hi = wave_lo + kWave * c;
                 if constexpr (MODE == 0) {
-                    // the overwritten bins of a fused wrap copy read their source bins
-                    // (only the one or two lanes whose chunk holds them)
-                    const bool wrap_here = fused_wrap && base + c > wrap_lo && base < n_bins;
+                    if (wave_hi <= (fused_wrap ? wrap_lo : span)) {
+                        // every bin of the wave exists and is read in place: loads at
+                        // fixed offsets from the lane's first bin, no per-bin tests
+                        const int64_t* Sy = S + base;
+                        const int64_t* Sw = S + wl + base;
 #pragma unroll
-                    for (int kk = 0; kk <= CHUNK; ++kk) {
-                        const int i = base + kk;
-                        if (kk < c && i < span) {
-                            const int src = (wrap_here && i >= wrap_lo && i < n_bins)
-                                                ? 1 + (i - wrap_lo)
-                                                : i;
-                            sy += S[src];
-                            sw += S[wl + src];
+                        for (int kk = 0; kk <= CHUNK; ++kk) {
+                            if (kk < c) {
+                                sy += Sy[kk];
+                                sw += Sw[kk];
+                            }
+                            ry[kk] = sy;
+                            rw[kk] = sw;
                         }
-                        ry[kk] = sy;
-                        rw[kk] = sw;
+                    } else {
+                        // the overwritten bins of a fused wrap copy read their source
+                        // bins (only the one or two lanes whose chunk holds them)
+                        const bool wrap_here = fused_wrap && base + c > wrap_lo && base < n_bins;
+#pragma unroll
+                        for (int kk = 0; kk <= CHUNK; ++kk) {
+                            const int i = base + kk;
+                            if (kk < c && i < span) {
+                                const int src = (wrap_here && i >= wrap_lo && i < n_bins)
+                                                    ? 1 + (i - wrap_lo)
+                                                    : i;
+                                sy += S[src];
+                                sw += S[wl + src];
+                            }
+                            ry[kk] = sy;
+                            rw[kk] = sw;
+                        }
                     }
                 } else {
                     for (int kk = 0; kk < c; ++kk) {
@@ -560,18 +579,29 @@ __global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 0 ? 6 : 1)) void bls_
                 if constexpr (MODE == 0) {
                     int2* Hw = reinterpret_cast<int2*>(S);
                     uint2* Lw = reinterpret_cast<uint2*>(reinterpret_cast<char*>(S) + 8 * (size_t)wl);
+                    // mean_units is 0 for every high word of a `plain` curve (|ybar| <
+                    // 2^-31, high words < 2^27), so it needs no test here
+                    const double ybar = cv.ybar_d;
+                    // the lane's chunk as 32-bit words: word pairs at fixed offsets
+                    int32_t* Hb = reinterpret_cast<int32_t*>(Hw + base);
+                    uint32_t* Lb = reinterpret_cast<uint32_t*>(Lw + base);
+                    auto store = [&](int kk) {
+                        const uint64_t vy = (uint64_t)(ry[kk] + oy);
+                        const uint64_t vw = (uint64_t)(rw[kk] + ow);
+                        const int32_t wh = (int32_t)(vw >> 32);
+                        Hb[2 * kk] = (int32_t)(vy >> 32) - mean_units(wh, ybar);
+                        Hb[2 * kk + 1] = wh;
+                        Lb[2 * kk] = (uint32_t)vy;
+                        Lb[2 * kk + 1] = (uint32_t)vw;
+                    };
+                    if (wave_hi <= span) {
 #pragma unroll
-                    for (int kk = 0; kk <= CHUNK; ++kk) {
-                        const int i = base + kk;
-                        if (kk < c && i < span) {
-                            const uint64_t vy = (uint64_t)(ry[kk] + oy);
-                            const uint64_t vw = (uint64_t)(rw[kk] + ow);
-                            const int32_t wh = (int32_t)(vw >> 32);
-                            int32_t u = (int32_t)(vy >> 32);
-                            if (!plain) u -= mean_units(wh, cv.ybar_d);
-                            Hw[i] = make_int2(u, wh);
-                            Lw[i] = make_uint2((uint32_t)vy, (uint32_t)vw);
-                        }
+                        for (int kk = 0; kk <= CHUNK; ++kk)
+                            if (kk < c) store(kk);
+                    } else {
+#pragma unroll
+                        for (int kk = 0; kk <= CHUNK; ++kk)
+                            if (kk < c && base + kk < span) store(kk);
                     }
                 } else {
                     int4* S4 = reinterpret_cast<int4*>(S);

@@ -172,15 +172,28 @@ __device__ __forceinline__ int64_t dpp_i64(int64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// One step of a 64-bit DPP scan as 32-bit halves: the low words added with a carry out,
+// the high words with the carry in (an add and an add-with-carry, each taking its
+// shifted operand straight from the DPP move).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void dpp_add_i64(uint32_t& lo, uint32_t& hi) {
+    const uint32_t dlo = dpp_u32<CTRL, ROW_MASK>(lo);
+    const uint32_t dhi = dpp_u32<CTRL, ROW_MASK>(hi);
+    const uint32_t s = lo + dlo;
+    hi = hi + dhi + (uint32_t)(s < dlo);
+    lo = s;
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave (all lanes active).
 __device__ __forceinline__ int64_t wave_incl_scan_dpp(int64_t v) {
-    v += dpp_i64<0x111, 0xF>(v);
-    v += dpp_i64<0x112, 0xF>(v);
-    v += dpp_i64<0x114, 0xF>(v);
-    v += dpp_i64<0x118, 0xF>(v);
-    v += dpp_i64<0x142, 0xA>(v);
-    v += dpp_i64<0x143, 0xC>(v);
-    return v;
+    uint32_t lo = (uint32_t)(uint64_t)v, hi = (uint32_t)((uint64_t)v >> 32);
+    dpp_add_i64<0x111, 0xF>(lo, hi);
+    dpp_add_i64<0x112, 0xF>(lo, hi);
+    dpp_add_i64<0x114, 0xF>(lo, hi);
+    dpp_add_i64<0x118, 0xF>(lo, hi);
+    dpp_add_i64<0x142, 0xA>(lo, hi);
+    dpp_add_i64<0x143, 0xC>(lo, hi);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // Maximum over the wave of non-negative int32 values, returned uniformly.
