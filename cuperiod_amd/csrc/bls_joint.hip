@@ -28,6 +28,8 @@
 // t^2 / (W_in W_out W_tot) * 2^(2 e_y - e_w) with t = Y_tot W_in - Y_in W_tot, which is
 // the reference's depth^2/(1/W_in + 1/W_out) without its divisions and cancellations.
 
+#include <climits>
+
 #include "engine_internal.h"
 
 using namespace cpb_engine;
@@ -417,6 +419,66 @@ __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, i
         }
         __syncthreads();
     }
+    // Chunked prefix: `wpb` waves per band, each lane sums a contiguous chunk of c bins,
+    // one DPP scan per wave and one exchange between the band's waves give every chunk its
+    // offset, and a second pass over the chunk writes the FP64 prefixes in place. The
+    // fused wrap copy is made by the first pass: the lanes owning an overwritten bin read
+    // its source (bins 1..os, which nobody writes in that pass) and store it in place, so
+    // the second pass reads every bin as copied.
+    constexpr int CMAX = 16;
+    const int wpb = gk <= NW ? NW / gk : 1;
+    const int c = (span + wpb * kWave - 1) / (wpb * kWave);
+    if (gk <= NW && c <= CMAX) {
+        __shared__ int64_t s_part[2 * NW];
+        const int w_u = __builtin_amdgcn_readfirstlane(wave);
+        const int kk = w_u / wpb, part = w_u - kk * wpb;
+        const bool active = kk < gk;
+        const int base = (part * kWave + lane) * c;
+        longlong2* B = S2 + (int64_t)kk * wl;
+        int64_t sy = 0, sw = 0;
+        if (active) {
+            for (int k = 0; k < c; ++k) {
+                const int i = base + k;
+                if (i <= nb) {
+                    longlong2 v;
+                    if (fused_wrap && i >= wrap_lo && i < nb) {
+                        v = B[1 + (i - wrap_lo)];
+                        B[i] = v;
+                    } else {
+                        v = B[i];
+                    }
+                    sy += v.x;
+                    sw += v.y;
+                }
+            }
+        }
+        const int64_t iy = wave_incl_scan_dpp(sy);
+        const int64_t iw = wave_incl_scan_dpp(sw);
+        if (lane == kWave - 1) {
+            s_part[2 * w_u] = iy;
+            s_part[2 * w_u + 1] = iw;
+        }
+        __syncthreads();
+        if (active) {
+            int64_t oy = iy - sy, ow = iw - sw;
+            for (int p = 0; p < part; ++p) {
+                oy += s_part[2 * (kk * wpb + p)];
+                ow += s_part[2 * (kk * wpb + p) + 1];
+            }
+            double2* D = reinterpret_cast<double2*>(B);
+            for (int k = 0; k < c; ++k) {
+                const int i = base + k;
+                if (i <= nb) {
+                    const longlong2 v = B[i];
+                    oy += v.x;
+                    ow += v.y;
+                    D[i] = make_double2((double)oy, (double)ow);
+                }
+            }
+        }
+        __syncthreads();
+        return;
+    }
     for (int kk = wave; kk < gk; kk += NW) {
         longlong2* B = S2 + (int64_t)kk * wl;
         double2* D = reinterpret_cast<double2*>(B);
@@ -449,7 +511,7 @@ __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, i
 }
 
 template <int T, int DMAX, int RMAX, int OBJ>
-__global__ __launch_bounds__(T) void bls_joint_group_kernel(
+__global__ __launch_bounds__(T, T == 256 ? 3 : 1) void bls_joint_group_kernel(
     const double* __restrict__ tau, const int64_t* __restrict__ yq,
     const int64_t* __restrict__ wq, const CurveParams* __restrict__ bp,
     const cpb_star* __restrict__ stars, const cpb_segment* __restrict__ segs,
@@ -532,6 +594,7 @@ __global__ __launch_bounds__(T) void bls_joint_group_kernel(
                     const double2* B = D2 + (int64_t)kk * wl;
 #pragma unroll
                     for (int r = 0; r < RMAX; ++r) {
+                        if (r * T >= nb) break;  // uniform: no start of this round is in use
                         const int n = tid + r * T;
                         if (n < nb) {
                             const double2 p0 = B[n];
@@ -604,7 +667,60 @@ __global__ __launch_bounds__(T) void bls_joint_group_kernel(
             const int bn = found ? (int)(bkey & 0xFFFFFFu) : 0;
             const int bdur = found ? s_dur[bkey >> 24] : 0;
 
-            if (found) {
+            // With more than one band group and the wrap copy's sources apart from the bins
+            // it overwrites, the winner's prefix sums come from one pass over the star's
+            // points instead of re-folding every other group: with W = [nb - os, nb) the
+            // overwritten bins, a point in bin b adds to the prefix at X once for
+            // b <= X, b outside W, and once more for 1 <= b <= os with its copy's bin
+            // nb - os + b - 1 <= X (exact int64 sums, rounded to FP64 as the folded
+            // prefixes are, so the outputs are the same).
+            const bool direct = found && K > group && nb - oversample > oversample;
+            if (direct) {
+                __shared__ unsigned long long s_acc[4 * kJointMaxBands];
+                for (int i = tid; i < 4 * K; i += T) s_acc[i] = 0ull;
+                __syncthreads();
+                const int wlo = nb - oversample, x0 = bn, x1 = bn + bdur;
+                int k = 0;
+                for (int64_t j = s_cum[0] + tid; j < s_cum[K]; j += T) {
+                    while (j >= s_cum[k + 1]) ++k;
+                    const int64_t p = s_off[k] + (j - s_cum[k]);
+                    const int b = fold_bin(tau[p], P, invP, bd, inv_bd, width);
+                    const bool kept = b < wlo || b >= nb;
+                    const int mirror = (b >= 1 && b <= oversample) ? wlo + b - 1 : INT_MAX;
+                    const int m0 = (kept && b <= x0) + (mirror <= x0);
+                    const int m1 = (kept && b <= x1) + (mirror <= x1);
+                    if (m1 != 0) {
+                        const uint64_t y = (uint64_t)yq[p], w = (uint64_t)wq[p];
+                        atomicAdd(&s_acc[4 * k + 2], (unsigned long long)(y * (uint64_t)m1));
+                        atomicAdd(&s_acc[4 * k + 3], (unsigned long long)(w * (uint64_t)m1));
+                        if (m0 != 0) {
+                            atomicAdd(&s_acc[4 * k], (unsigned long long)(y * (uint64_t)m0));
+                            atomicAdd(&s_acc[4 * k + 1], (unsigned long long)(w * (uint64_t)m0));
+                        }
+                    }
+                }
+                __syncthreads();
+                for (int kk = tid; kk < K; kk += T) {
+                    const CurveParams c = bp[star.band_offset + kk];
+                    const double yi = (double)(int64_t)s_acc[4 * kk + 2] - (double)(int64_t)s_acc[4 * kk];
+                    const double wi = (double)(int64_t)s_acc[4 * kk + 3] - (double)(int64_t)s_acc[4 * kk + 1];
+                    const double Y = ldexp(yi, c.y_exp);
+                    const double W = ldexp(wi, c.w_exp);
+                    const double Yo = ldexp((double)c.y_tot - yi, c.y_exp);
+                    const double Wo = ldexp((double)c.w_tot - wi, c.w_exp);
+                    double snr = 0.0, depth = 0.0;
+                    if (W >= c.ivar_floor && Wo >= c.ivar_floor) {
+                        const double yin = Y / W;
+                        const double yout = Yo / Wo;
+                        if (yout >= yin) {
+                            depth = yout - yin;
+                            snr = depth / sqrt(1.0 / W + 1.0 / Wo);
+                        }
+                    }
+                    s_snr[kk] = snr;
+                    s_depth[kk] = depth;
+                }
+            } else if (found) {
                 // the resident group first, then re-folds of the others
                 for (int pass = 0; pass * group < K; ++pass) {
                     const int g0 = pass == 0 ? last_g0 : (pass - 1) * group;
@@ -684,12 +800,19 @@ hipError_t launch_group(dim3 grid, size_t lds, hipStream_t st, const double* tau
 __host__ __forceinline__ int group_rmax(int dmax) { return dmax <= 8 ? 4 : (dmax <= 16 ? 2 : 1); }
 
 template <int T, int OBJ>
-hipError_t dispatch_group(int dmax, dim3 grid, size_t lds, hipStream_t st, const double* tau,
-                          const int64_t* yq, const int64_t* wq, const CurveParams* bp,
-                          const cpb_star* stars, const cpb_segment* segs,
+hipError_t dispatch_group(int dmax, int span, dim3 grid, size_t lds, hipStream_t st,
+                          const double* tau, const int64_t* yq, const int64_t* wq,
+                          const CurveParams* bp, const cpb_star* stars, const cpb_segment* segs,
                           const WorkItem* items, int64_t n_items, const double* periods,
                           const int32_t* dur, int32_t os, int32_t wl, int32_t group,
                           double* out, int64_t n_out) {
+    // 1,024-thread workgroups over spans of up to 2,048 bins need two starts per thread:
+    // the smaller tile of running totals leaves registers for the sweep's chains
+    if constexpr (T >= 1024)
+        if (span <= 2 * T && dmax <= 8)
+            return launch_group<T, 8, 2, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs,
+                                              items, n_items, periods, dur, os, wl, group, out,
+                                              n_out);
     if (dmax <= 8)
         return launch_group<T, 8, 4, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
                                           n_items, periods, dur, os, wl, group, out, n_out);
@@ -893,18 +1016,18 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
             if (T == kGroupThreadsNarrow)
                 e = obj_flag == 0
                         ? dispatch_group<kGroupThreadsNarrow, 0>(
-                              cls[c].dmax, grid, lds, st, tau, yq, wq, params, sp, gp, items,
+                              cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp, items,
                               cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out)
                         : dispatch_group<kGroupThreadsNarrow, 1>(
-                              cls[c].dmax, grid, lds, st, tau, yq, wq, params, sp, gp, items,
+                              cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp, items,
                               cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out);
             else
                 e = obj_flag == 0
                         ? dispatch_group<kGroupThreads, 0>(
-                              cls[c].dmax, grid, lds, st, tau, yq, wq, params, sp, gp, items,
+                              cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp, items,
                               cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out)
                         : dispatch_group<kGroupThreads, 1>(
-                              cls[c].dmax, grid, lds, st, tau, yq, wq, params, sp, gp, items,
+                              cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp, items,
                               cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out);
             if (e != hipSuccess) return hip_fail(ctx, e, "bls_joint_group_kernel launch");
         }
