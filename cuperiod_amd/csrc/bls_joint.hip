@@ -53,6 +53,37 @@ __device__ __forceinline__ bool first_wins(double a, uint32_t ka, double b, uint
     return a > b || (a == b && ka < kb);
 }
 
+// One DPP step of the wave's first-wins maximum: each lane compares its (value, key)
+// with the pair CTRL brings in (rows outside ROW_MASK keep their own pair).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void dpp_first_wins(double& v, uint32_t& k) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const int lo = (int)(uint32_t)b, hi = (int)(uint32_t)(b >> 32);
+    const uint32_t olo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+    const uint32_t ohi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+    const uint32_t ok = (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, CTRL, ROW_MASK, 0xF, false);
+    const double ov = __longlong_as_double((long long)(((uint64_t)ohi << 32) | olo));
+    if (first_wins(ov, ok, v, k)) {
+        v = ov;
+        k = ok;
+    }
+}
+
+// First-wins maximum (largest value, smallest key among equals) over the wave, returned
+// uniformly: pairs within quads, half rows and rows by DPP permutations, then across
+// rows by row broadcasts; lane 63 ends with the whole wave's pair. The rule is
+// commutative and associative, so the order of the combinations does not matter.
+__device__ __forceinline__ void wave_first_wins(double& v, uint32_t& k) {
+    dpp_first_wins<0xB1, 0xF>(v, k);   // quad_perm [1,0,3,2]
+    dpp_first_wins<0x4E, 0xF>(v, k);   // quad_perm [2,3,0,1]
+    dpp_first_wins<0x141, 0xF>(v, k);  // row_half_mirror
+    dpp_first_wins<0x140, 0xF>(v, k);  // row_mirror
+    dpp_first_wins<0x142, 0xA>(v, k);  // row_bcast:15
+    dpp_first_wins<0x143, 0xC>(v, k);  // row_bcast:31
+    v = readlane_f64(v, kWave - 1);
+    k = (uint32_t)__builtin_amdgcn_readlane((int)k, kWave - 1);
+}
+
 template <int T, int DMAX, int OBJ, bool GLOBAL>
 __global__ __launch_bounds__(T) void bls_joint_kernel(
     const double* __restrict__ tau, const int64_t* __restrict__ yq,
@@ -224,15 +255,7 @@ __global__ __launch_bounds__(T) void bls_joint_kernel(
                 }
             }
             // first-wins maximum over the workgroup
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(best, off, kWave);
-                const uint32_t ok = (uint32_t)__shfl_xor((int)bkey, off, kWave);
-                if (first_wins(ov, ok, best, bkey)) {
-                    best = ov;
-                    bkey = ok;
-                }
-            }
+            wave_first_wins(best, bkey);
             if (lane == 0) {
                 s_red_v[wave] = best;
                 s_red_k[wave] = bkey;
@@ -642,15 +665,7 @@ __global__ __launch_bounds__(T, T == 256 ? 3 : 1) void bls_joint_group_kernel(
                     }
                 }
             }
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(best, off, kWave);
-                const uint32_t ok = (uint32_t)__shfl_xor((int)bkey, off, kWave);
-                if (first_wins(ov, ok, best, bkey)) {
-                    best = ov;
-                    bkey = ok;
-                }
-            }
+            wave_first_wins(best, bkey);
             if (lane == 0) {
                 s_red_v[wave] = best;
                 s_red_k[wave] = bkey;
