@@ -55,15 +55,11 @@ def alias_diverse_for(method: str) -> bool:
 def peak_extra_columns(methods: Sequence[str], *, multiband: bool = False) -> tuple[str, ...]:
     """Ordered union of the per-peak extras ``methods`` declare; ``multiband`` adds the
     extras only multi-band results carry."""
-    seen: dict[str, None] = {}
-    for name in methods:
-        key = _canonical_method(name)
-        for extra in PEAK_EXTRA_COLUMNS.get(key, ()):
-            seen.setdefault(extra, None)
-        if multiband:
-            for extra in MULTIBAND_PEAK_EXTRA_COLUMNS.get(key, ()):
-                seen.setdefault(extra, None)
-    return tuple(seen)
+    tables = (PEAK_EXTRA_COLUMNS, MULTIBAND_PEAK_EXTRA_COLUMNS) if multiband \
+        else (PEAK_EXTRA_COLUMNS,)
+    names = (extra for method in map(_canonical_method, methods)
+             for table in tables for extra in table.get(method, ()))
+    return tuple(dict.fromkeys(names))
 
 
 def source_label(source: Any) -> str:
@@ -185,40 +181,39 @@ def peaks_to_row(
 ) -> dict[str, Any]:
     """The row of :func:`periodogram_to_row` from already selected peaks (most
     significant first) and, for ``store_raw``, the downsampled ``(frequency, power)``."""
+    nan = float("nan")
+    declared = () if extra_keys is None else tuple(extra_keys)
     row: dict[str, Any] = {"key": key, "method": method}
-    if source is not None:
-        row["source"] = source
-    row["backend"] = backend
-    row["n_samples"] = n_samples
-    row["baseline"] = baseline
-    row["best_period"] = peaks[0].period if peaks else float("nan")
-    declared = tuple(extra_keys) if extra_keys is not None else ()
+    row.update({} if source is None else {"source": source})
+    row.update(backend=backend, n_samples=n_samples, baseline=baseline,
+               best_period=peaks[0].period if peaks else nan)
     for rank in range(1, n_best + 1):
         peak = peaks[rank - 1] if rank <= len(peaks) else None
-        row[f"period_{rank}"] = peak.period if peak else float("nan")
-        row[f"power_{rank}"] = peak.power if peak else float("nan")
-        extra = peak.extra if peak is not None else {}
-        for name in declared:
-            value = extra.get(name)
-            row[f"{name}_{rank}"] = float("nan") if value is None else float(value)
-        for name, value in extra.items():
-            if name not in declared:
-                row[f"{name}_{rank}"] = value
+        extra = {} if peak is None else peak.extra
+        row[f"period_{rank}"] = nan if peak is None else peak.period
+        row[f"power_{rank}"] = nan if peak is None else peak.power
+        # declared extras always get a (float) column; undeclared ones as they come
+        row.update({f"{name}_{rank}": nan if extra.get(name) is None else float(extra[name])
+                    for name in declared})
+        row.update({f"{name}_{rank}": value for name, value in extra.items()
+                    if name not in declared})
     if raw is not None:
-        freq, power = raw
-        row["pgram_frequency"] = freq.tolist()
-        row["pgram_power"] = power.tolist()
+        row["pgram_frequency"], row["pgram_power"] = (a.tolist() for a in raw)
     return row
+
+
+def _numeric_text(text: str) -> bool:
+    try:
+        float(text)
+    except ValueError:
+        return False
+    return True
 
 
 def csv_safe(value: Any) -> Any:
     """Quote-prefix a text cell starting with ``= + - @ TAB CR`` unless it is a number."""
-    if isinstance(value, str) and value and value[0] in _CSV_FORMULA_PREFIXES:
-        try:
-            float(value)
-        except ValueError:
-            return "'" + value
-    return value
+    risky = isinstance(value, str) and value[:1] in _CSV_FORMULA_PREFIXES and value != ""
+    return "'" + value if risky and not _numeric_text(value) else value
 
 
 def _union_columns(rows: Sequence[Mapping[str, Any]]) -> list[str]:

@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import dataclasses
 import hashlib
+import itertools
 import json
 import re
 import time
@@ -61,6 +62,7 @@ from cuperiod_amd.methods.base import (
     get_method,
     normalize_backend,
     requested_backend,
+    searchable_points,
     select_settings,
 )
 
@@ -137,27 +139,29 @@ def _worker_backend(device: str, backend: str | None, settings: Any) -> str:
 # --- sinks -----------------------------------------------------------------------------
 
 
+_FILE_SINK_SUFFIXES = frozenset({".parquet", ".pq", ".csv"})
+
+
 def _classify_sink(sink: str | Path | None) -> tuple[str, Path, Path]:
     """``("memory"|"file"|"dir", directory, path)``; a file sink's directory is made."""
     if sink is None:
         return "memory", Path(), Path()
     path = Path(sink)
     suffix = path.suffix.lower()
-    if suffix in {".parquet", ".pq", ".csv"}:
-        if path.is_dir():
+    is_dir = path.is_dir()
+    if suffix not in _FILE_SINK_SUFFIXES:
+        if suffix and not is_dir:
             raise ValueError(
-                f"sink {str(sink)!r} is an existing directory, but its {suffix} suffix "
-                "asks for a single-file sink; pass a file path, or a directory path "
-                "without an extension for a resumable multi-part sink."
-            )
-        path.parent.mkdir(parents=True, exist_ok=True)
-        return "file", path.parent, path
-    if suffix and not path.is_dir():
+                f"unsupported sink {str(sink)!r}: a file sink must end in .parquet or .csv; "
+                "give a directory (no extension) for a resumable multi-part sink.")
+        return "dir", path, path
+    if is_dir:
         raise ValueError(
-            f"unsupported sink {str(sink)!r}: a file sink must end in .parquet or .csv; "
-            "pass a directory (no extension) for a resumable multi-part sink."
-        )
-    return "dir", path, path
+            f"sink {str(sink)!r} is an existing directory, yet its {suffix} suffix asks "
+            "for a single-file sink; give a file path, or a directory path without an "
+            "extension for a resumable multi-part sink.")
+    path.parent.mkdir(parents=True, exist_ok=True)
+    return "file", path.parent, path
 
 
 def _part_path(sink_dir: Path, idx: int) -> Path:
@@ -180,14 +184,12 @@ def _corrupt_errors() -> tuple[type[BaseException], ...]:
 
 
 def _quarantine(path: Path, *, hidden: bool) -> Path:
-    """Move an unreadable sink file aside (never delete it)."""
-    stamp = time.strftime("%Y%m%dT%H%M%S")
-    prefix = "." if hidden else ""
-    target = path.with_name(f"{prefix}{path.name}.corrupt-{stamp}")
-    n = 1
-    while target.exists():
-        target = path.with_name(f"{prefix}{path.name}.corrupt-{stamp}-{n}")
-        n += 1
+    """Move an unreadable sink file aside (never delete it): to
+    ``[.]<name>.corrupt-<timestamp>[-<n>]``, the first such name that is free."""
+    stem = f"{'.' if hidden else ''}{path.name}.corrupt-{time.strftime('%Y%m%dT%H%M%S')}"
+    candidates = (stem if n == 0 else f"{stem}-{n}" for n in itertools.count())
+    target = next(path.with_name(name) for name in candidates
+                  if not path.with_name(name).exists())
     path.replace(target)
     return target
 
@@ -286,39 +288,43 @@ def _run_fingerprint(methods, settings_map, *, n_best, store_raw, columns, domai
     return json.loads(json.dumps(raw))
 
 
+def _settings_changes(stored: Any, current: Mapping[str, Any]) -> list[str]:
+    """Per method whose settings both requests record: the fields that differ."""
+    if not isinstance(stored, Mapping):
+        return []
+    out = []
+    for method, now in current.items():
+        before = stored.get(method)
+        if isinstance(before, Mapping):
+            fields = sorted(f for f in set(before) & set(now) if before[f] != now[f])
+            if fields:
+                out.append(f"{method} settings ({', '.join(fields)})")
+    return out
+
+
 def _merge_config(stored: Mapping[str, Any], current: Mapping[str, Any],
                   sink_dir: Path) -> dict[str, Any]:
     """Refuse a resumed request whose rows would differ from the directory's; a new
     method is recorded."""
-    changed = [f for f in _FINGERPRINT_FIELDS if f in stored and stored[f] != current.get(f)]
-    extra = list(current.get("extra_keys", []))
-    old_extra = stored.get("extra_keys")
-    if isinstance(old_extra, list):
-        added = [k for k in extra if k not in old_extra]
-        if added:
-            changed.append(
-                f"new per-peak columns {added} that the existing part files lack; write "
-                "that method to a directory of its own"
-            )
-        extra = list(old_extra)
-    old_settings = stored.get("settings")
-    old_settings = old_settings if isinstance(old_settings, Mapping) else {}
-    for name, new in current.get("settings", {}).items():
-        old = old_settings.get(name)
-        if not isinstance(old, Mapping):
-            continue
-        diff = sorted(k for k in set(old) & set(new) if old[k] != new[k])
-        if diff:
-            changed.append(f"{name} settings ({', '.join(diff)})")
-    if changed:
+    problems = [name for name in _FINGERPRINT_FIELDS
+                if name in stored and stored[name] != current.get(name)]
+    columns = list(current.get("extra_keys", []))
+    if isinstance(stored.get("extra_keys"), list):
+        missing = [k for k in columns if k not in stored["extra_keys"]]
+        if missing:
+            problems.append(f"new per-peak columns {missing} that the existing part files "
+                            "lack; write that method to a directory of its own")
+        columns = list(stored["extra_keys"])
+    problems += _settings_changes(stored.get("settings"), current.get("settings", {}))
+    if problems:
         raise ValueError(
             f"directory sink {sink_dir} was written with a different request "
-            f"({'; '.join(changed)}); resuming would mix incompatible rows. Repeat the "
-            "original request, write to a fresh directory, or pass resume=False to "
-            "replace the directory's part files."
-        )
-    return {**current, "extra_keys": extra,
-            "settings": {**old_settings, **current.get("settings", {})}}
+            f"({'; '.join(problems)}): resuming would mix rows that do not belong "
+            "together. Repeat the original request, use a fresh directory, or pass "
+            "resume=False to replace the directory's part files.")
+    before = stored.get("settings") if isinstance(stored.get("settings"), Mapping) else {}
+    return {**current, "extra_keys": columns,
+            "settings": {**before, **current.get("settings", {})}}
 
 
 def _dir_manifest_guard(sink_dir: Path, resume: bool, config: Mapping[str, Any]) -> dict:
@@ -397,14 +403,7 @@ def _prepare(position: int, key: str, lc: LightCurve, settings: Any,
     from cuperiod_amd.methods.bls import _grid_segments
 
     raise_if_unusable(lc, "BLS")
-    single = lc.in_domain(Domain.FLUX)
-    finite = single.finite()
-    if finite.n < settings.min_detections:
-        raise InsufficientDataError(
-            f"BLS: {finite.n} finite points < min_detections {settings.min_detections}"
-        )
-    if finite.baseline <= 0.0:
-        raise InsufficientDataError("BLS: no usable time baseline")
+    finite = searchable_points(lc.in_domain(Domain.FLUX), settings.min_detections, "BLS")
     own = None
     if callable(grid):
         # a grid rule: this light curve's grid (a GridSpec or a parametric LinspaceGrid)
@@ -433,13 +432,7 @@ def _prepare_tls(position: int, key: str, lc: LightCurve, settings: Any,
     from cuperiod_amd.methods.tls import TLSMethod
 
     raise_if_unusable(lc, "TLS")
-    finite = lc.in_domain(Domain.FLUX).finite()
-    if finite.n < settings.min_detections:
-        raise InsufficientDataError(
-            f"TLS: {finite.n} finite points < min_detections {settings.min_detections}"
-        )
-    if finite.baseline <= 0.0:
-        raise InsufficientDataError("TLS: no usable time baseline")
+    finite = searchable_points(lc.in_domain(Domain.FLUX), settings.min_detections, "TLS")
     g = grid if grid is not None else TLSMethod().default_grid(finite, settings)
     return _Prepared(position, key, source_label(lc), finite,
                      np.asarray(g.period, dtype=np.float64))
@@ -814,14 +807,11 @@ def batch_periodograms(
             "hold; use a .parquet sink or a directory sink."
         )
     items = resolve_inputs(inputs)
-    summary_sink = str(sink) if sink is not None else None
+    summary_sink = None if sink is None else str(sink)
     if not items:
-        if sink is not None:
-            warnings.warn(
-                "batch_periodograms: no inputs to process; the sink was left untouched",
-                UserWarning,
-                stacklevel=2,
-            )
+        if summary_sink is not None:
+            warnings.warn("batch_periodograms: nothing to process, so the sink was left "
+                          "untouched (no inputs)", UserWarning, stacklevel=2)
         return BatchSummary(0, 0, 0, 0, methods, device, summary_sink,
                             rows=[] if sink_kind == "memory" else None, n_workers=0)
 
@@ -884,22 +874,16 @@ def batch_periodograms(
         runner.submit(i, chunks[i], chunk_cfgs[i])
     runner.drain()
 
-    all_rows = [row for i in sorted(rows_by_chunk) for row in rows_by_chunk[i]]
-    errors = [err for i in sorted(errors_by_chunk) for err in errors_by_chunk[i]]
+    def in_order(parts: Mapping[int, list]) -> list:
+        return [entry for i in sorted(parts) for entry in parts[i]]
+
+    kept_rows, errors = in_order(rows_by_chunk), in_order(errors_by_chunk)
     if sink_kind == "file":
-        _finalize_file(all_rows, sink_file, resume)
-    return BatchSummary(
-        n_inputs=len(items),
-        n_done=n_done,
-        n_failed=len(errors),
-        n_skipped=n_skipped,
-        methods=methods,
-        device=device,
-        sink=summary_sink,
-        errors=errors,
-        rows=None if sink_kind != "memory" else all_rows,
-        n_workers=1 if chunks else 0,
-    )
+        _finalize_file(kept_rows, sink_file, resume)
+    return BatchSummary(len(items), n_done, len(errors), n_skipped, methods, device,
+                        summary_sink, errors=errors,
+                        rows=kept_rows if sink_kind == "memory" else None,
+                        n_workers=int(bool(chunks)))
 
 
 __all__ = ["DEFAULT_CHUNK_SIZE", "BatchSummary", "batch_periodograms"]

@@ -77,6 +77,28 @@ def select_top_peaks(
     return np.asarray(chosen, dtype=np.int64)
 
 
+def _alias_lattice(harmonic_max: int) -> tuple[np.ndarray, np.ndarray]:
+    """Harmonic orders ``2..harmonic_max`` and the alias offsets (sidereal and solar day
+    at k = 1, 2, then the yearly sidelobe), as float64 columns."""
+    orders = np.arange(2, harmonic_max + 1, dtype=np.float64)[:, None]
+    day = np.array([SIDEREAL_FREQ_CPD, SOLAR_FREQ_CPD], dtype=np.float64)
+    offsets = np.concatenate([(np.array([1.0, 2.0])[None, :] * day[:, None]).ravel(),
+                              [YEARLY_FREQ_CPD]])[:, None]
+    return orders, offsets
+
+
+def _related(f_new, f_chosen: np.ndarray, tol: float, separation: float,
+             orders: np.ndarray, offsets: np.ndarray) -> np.ndarray:
+    """Element-wise over ``f_chosen``: does ``f_new`` duplicate, harmonise or alias it
+    (the test of :func:`is_alias_related`, one chosen frequency per column)."""
+    near = np.abs(f_new - f_chosen) < separation
+    harmonic = np.abs(f_new - orders * f_chosen) < orders * tol
+    subharmonic = np.abs(f_new - f_chosen / orders) < tol
+    below = np.abs(f_new - np.abs(f_chosen - offsets)) < tol
+    above = np.abs(f_new - (f_chosen + offsets)) < tol
+    return near | (harmonic | subharmonic).any(0) | (below | above).any(0)
+
+
 def is_alias_related(
     f_new: float,
     f_chosen: float,
@@ -85,24 +107,14 @@ def is_alias_related(
     *,
     min_separation: float | None = None,
 ) -> bool:
-    """Whether ``f_new`` duplicates ``f_chosen`` (closer than ``min_separation``), is
-    one of its harmonics / sub-harmonics up to ``harmonic_max`` (tolerance ``m*tol``
-    around the m-th harmonic, ``tol`` around a sub-harmonic), or sits on its
-    sidereal/solar (k = 1, 2) or yearly alias."""
-    separation = tol if min_separation is None else min_separation
-    if abs(f_new - f_chosen) < separation:
-        return True
-    for m in range(2, harmonic_max + 1):
-        if abs(f_new - m * f_chosen) < m * tol:
-            return True
-        if abs(f_new - f_chosen / m) < tol:
-            return True
-    offsets = [k * day for day in (SIDEREAL_FREQ_CPD, SOLAR_FREQ_CPD) for k in (1, 2)]
-    offsets.append(YEARLY_FREQ_CPD)
-    for off in offsets:
-        if abs(f_new - abs(f_chosen - off)) < tol or abs(f_new - (f_chosen + off)) < tol:
-            return True
-    return False
+    """Whether ``f_new`` duplicates ``f_chosen`` (closer than ``min_separation``, default
+    ``tol``), is one of its harmonics / sub-harmonics up to ``harmonic_max`` (within
+    ``m*tol`` of the m-th harmonic, ``tol`` of a sub-harmonic), or sits within ``tol`` of
+    a sidereal/solar (k = 1, 2) or yearly alias on either side of it."""
+    orders, offsets = _alias_lattice(harmonic_max)
+    sep = tol if min_separation is None else min_separation
+    return bool(_related(float(f_new), np.array([float(f_chosen)]), tol, sep, orders,
+                         offsets)[0])
 
 
 def select_diverse_peaks(
@@ -121,28 +133,26 @@ def select_diverse_peaks(
     candidates visited."""
     if n_peaks <= 0:
         return np.empty(0, dtype=np.int64)
-    chosen: list[int] = []
-    seen = 0
+    orders, offsets = _alias_lattice(harmonic_max)
+    sep = tol if min_separation is None else min_separation
+    picked = np.empty(0, dtype=np.int64)
+    picked_f = np.empty(0, dtype=np.float64)
+    visited = 0
     for idx in _descending(score, candidates):
-        seen += 1
-        f_new = float(frequency[idx])
-        if all(
-            not is_alias_related(
-                f_new, float(frequency[j]), tol, harmonic_max,
-                min_separation=min_separation,
-            )
-            and not is_alias_related(
-                float(frequency[j]), f_new, tol, harmonic_max,
-                min_separation=min_separation,
-            )
-            for j in chosen
+        visited += 1
+        f = float(frequency[idx])
+        if picked.size and bool(
+            (_related(f, picked_f, tol, sep, orders, offsets)
+             | _related(picked_f, f, tol, sep, orders, offsets)).any()
         ):
-            chosen.append(int(idx))
-            if len(chosen) >= n_peaks:
-                break
+            continue
+        picked = np.append(picked, np.int64(idx))
+        picked_f = np.append(picked_f, f)
+        if picked.size >= n_peaks:
+            break
     if examined is not None:
-        examined.append(seen)
-    return np.asarray(chosen, dtype=np.int64)
+        examined.append(visited)
+    return picked
 
 
 def peak_preserving_downsample(

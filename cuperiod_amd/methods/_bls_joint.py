@@ -51,8 +51,9 @@ class JointBands:
     """Every contributing band of one star, ready for the joint sweep (host float64).
 
     ``tau`` holds each band's times minus the common origin ``t_min``; ``yw``/``ivar``
-    the median-centred ``y*ivar`` and ``1/dy**2``; ``starts`` delimits the bands in the
-    concatenated arrays; ``sum_y``/``sum_ivar``/``floor`` are per band.
+    the median-centred ``y*ivar`` and ``1/dy**2``; band ``k`` occupies
+    ``starts[k]:starts[k + 1]`` of the concatenated arrays; ``sum_y``/``sum_ivar``/
+    ``floor`` are per band.
     """
 
     tau: np.ndarray
@@ -67,41 +68,37 @@ class JointBands:
     @property
     def n_bands(self) -> int:
         """Number of bands."""
-        return int(self.starts.size - 1)
+        return len(self.starts) - 1
 
     def band(self, k: int) -> slice:
-        """The slice of band ``k`` in the concatenated arrays."""
-        return slice(int(self.starts[k]), int(self.starts[k + 1]))
+        """Where band ``k`` lies in the concatenated arrays."""
+        lo, hi = self.starts[k:k + 2]
+        return slice(int(lo), int(hi))
+
+
+def _centred_band(t, y, dy, t_min: float) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """One band's ``(t - t_min, (y - median(y)) / dy^2, 1 / dy^2)``."""
+    y = np.asarray(y, dtype=np.float64)
+    weight = 1.0 / np.asarray(dy, dtype=np.float64) ** 2
+    return np.asarray(t, dtype=np.float64) - t_min, (y - float(np.median(y))) * weight, weight
 
 
 def joint_bands(times: list[np.ndarray], values: list[np.ndarray],
                 errors: list[np.ndarray]) -> JointBands:
     """Prepare the bands of one star: each centred on its own median, ``ivar =
     1/dy^2``, times from the earliest time of any band."""
-    t_min = float(min(float(np.min(t)) for t in times))
-    taus, yws, ivars, sum_y, sum_ivar, floors = [], [], [], [], [], []
-    for t, y, dy in zip(times, values, errors, strict=True):
-        y_c = np.asarray(y, dtype=np.float64)
-        y_c = y_c - float(np.median(y_c))
-        ivar = 1.0 / (np.asarray(dy, dtype=np.float64) ** 2)
-        yw = y_c * ivar
-        taus.append(np.asarray(t, dtype=np.float64) - t_min)
-        yws.append(yw)
-        ivars.append(ivar)
-        sum_y.append(float(yw.sum()))
-        sum_ivar.append(float(ivar.sum()))
-        floors.append(_ivar_floor(int(yw.size), sum_ivar[-1]))
-    starts = np.concatenate([[0], np.cumsum([t.size for t in taus])]).astype(np.int64)
-    return JointBands(
-        tau=np.ascontiguousarray(np.concatenate(taus)),
-        yw=np.ascontiguousarray(np.concatenate(yws)),
-        ivar=np.ascontiguousarray(np.concatenate(ivars)),
-        starts=starts,
-        sum_y=np.asarray(sum_y, dtype=np.float64),
-        sum_ivar=np.asarray(sum_ivar, dtype=np.float64),
-        floor=np.asarray(floors, dtype=np.float64),
-        t_min=t_min,
-    )
+    t_min = min(float(np.min(t)) for t in times)
+    prepared = [_centred_band(t, y, dy, t_min)
+                for t, y, dy in zip(times, values, errors, strict=True)]
+    tau, yw, ivar = (np.concatenate(column) for column in zip(*prepared))
+    sizes = np.array([band[0].size for band in prepared], dtype=np.int64)
+    sum_y = np.array([float(band[1].sum()) for band in prepared], dtype=np.float64)
+    sum_ivar = np.array([float(band[2].sum()) for band in prepared], dtype=np.float64)
+    floor = np.array([_ivar_floor(int(n), float(w)) for n, w in zip(sizes, sum_ivar)],
+                     dtype=np.float64)
+    return JointBands(tau=tau, yw=yw, ivar=ivar,
+                      starts=np.concatenate([np.zeros(1, dtype=np.int64), np.cumsum(sizes)]),
+                      sum_y=sum_y, sum_ivar=sum_ivar, floor=floor, t_min=t_min)
 
 
 def _empty(n: int) -> dict[str, np.ndarray]:

@@ -16,7 +16,11 @@ from collections.abc import Mapping, Sequence
 from dataclasses import dataclass
 from typing import Any, ClassVar
 
-from cuperiod_amd.core.errors import BackendUnavailableError, UnknownMethodError
+from cuperiod_amd.core.errors import (
+    BackendUnavailableError,
+    InsufficientDataError,
+    UnknownMethodError,
+)
 from cuperiod_amd.core.grid import GridSpec
 from cuperiod_amd.core.lightcurve import LightCurve
 from cuperiod_amd.core.result import Periodogram
@@ -45,6 +49,19 @@ def hip_ready() -> bool:
     return _lib.device_count() > 0
 
 
+def searchable_points(lc: LightCurve, min_detections: int, label: str) -> LightCurve:
+    """The finite points of ``lc``, refused (``InsufficientDataError``, messages prefixed
+    with the method's ``label``) when fewer than ``min_detections`` remain or they span
+    no time."""
+    finite = lc.finite()
+    if finite.n < min_detections:
+        raise InsufficientDataError(
+            f"{label}: {finite.n} finite points < min_detections {min_detections}")
+    if not finite.baseline > 0.0:
+        raise InsufficientDataError(f"{label}: no usable time baseline")
+    return finite
+
+
 class PeriodogramMethod(ABC):
     """Abstract periodogram method (same class attributes as cuPeriod's)."""
 
@@ -61,14 +78,11 @@ class PeriodogramMethod(ABC):
 
     def coerce_settings(self, settings: Any | None) -> Any:
         """A settings instance of this method's type (defaults if ``None``)."""
-        if settings is None:
-            return self.settings_cls()
-        if not isinstance(settings, self.settings_cls):
-            raise TypeError(
-                f"{self.name} expects {self.settings_cls.__name__}, "
-                f"got {type(settings).__name__}"
-            )
-        return settings
+        wanted = self.settings_cls
+        if settings is not None and not isinstance(settings, wanted):
+            raise TypeError(f"{self.name} expects {wanted.__name__}, "
+                            f"got {type(settings).__name__}")
+        return wanted() if settings is None else settings
 
     def resolve_backend(self, requested: str) -> str:
         """``"auto"``/``"gpu"``/the GPU backend's name -> the GPU backend when the HIP
@@ -162,34 +176,28 @@ def register(method: PeriodogramMethod) -> PeriodogramMethod:
 
 def get_method(name: str) -> PeriodogramMethod:
     """A registered method by name (case and separators ignored)."""
-    try:
-        return _REGISTRY[_normalize_name(name)]
-    except KeyError:
-        raise UnknownMethodError(
-            f"unknown method {name!r}; registered: {sorted(_REGISTRY)}"
-        ) from None
+    found = _REGISTRY.get(_normalize_name(name))
+    if found is None:
+        raise UnknownMethodError(f"unknown method {name!r}; registered: {sorted(_REGISTRY)}")
+    return found
 
 
 def method_names() -> list[str]:
     return sorted(_REGISTRY)
 
 
+def _info(name: str, method: PeriodogramMethod, ready: bool) -> MethodInfo:
+    domain = method.natural_domain
+    return MethodInfo(name=name, objective_sense=method.objective_sense,
+                      supports_multiband=method.supports_multiband,
+                      natural_domain=str(domain) if domain else "any",
+                      available_backends=method.all_backends if ready else (),
+                      all_backends=method.all_backends)
+
+
 def list_methods() -> list[MethodInfo]:
-    out = []
     ready = hip_ready()
-    for name in sorted(_REGISTRY):
-        m = _REGISTRY[name]
-        out.append(
-            MethodInfo(
-                name=name,
-                objective_sense=m.objective_sense,
-                supports_multiband=m.supports_multiband,
-                natural_domain=str(m.natural_domain) if m.natural_domain else "any",
-                available_backends=m.all_backends if ready else (),
-                all_backends=m.all_backends,
-            )
-        )
-    return out
+    return [_info(name, _REGISTRY[name], ready) for name in sorted(_REGISTRY)]
 
 
 def select_settings(

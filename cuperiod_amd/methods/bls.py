@@ -25,7 +25,7 @@ from cuperiod_amd.core.grid import GridSpec, check_grid_size
 from cuperiod_amd.core.lightcurve import LightCurve
 from cuperiod_amd.core.result import Periodogram
 from cuperiod_amd.methods._bls_core import HIP_BACKEND, bls_power_segments
-from cuperiod_amd.methods.base import PeriodogramMethod, register
+from cuperiod_amd.methods.base import PeriodogramMethod, register, searchable_points
 
 #: Per-period fields every search reports.
 _SEGMENT_FIELDS = ("period", "power", "depth", "depth_snr", "duration", "transit_time")
@@ -48,15 +48,14 @@ def _max_period(baseline: float, settings: BLSSettings) -> float:
 
 
 def _segment_bounds(baseline: float, settings: BLSSettings) -> list[tuple[float, float]]:
-    """``[(p_lo, p_hi), ...]`` of the default log segments, ascending."""
-    max_period = _max_period(baseline, settings)
-    bounds: list[tuple[float, float]] = []
-    p_lo = settings.min_period_days
-    while p_lo < max_period:
-        p_hi = min(p_lo * settings.segment_factor, max_period)
-        bounds.append((p_lo, p_hi))
-        p_lo = p_hi
-    return bounds
+    """``[(p_lo, p_hi), ...]`` of the default log segments, ascending: edges from
+    ``min_period_days``, each the previous one times ``segment_factor``, the last one
+    clipped to the longest searchable period."""
+    top = _max_period(baseline, settings)
+    edges = [settings.min_period_days]
+    while edges[-1] < top:
+        edges.append(min(edges[-1] * settings.segment_factor, top))
+    return list(zip(edges[:-1], edges[1:]))
 
 
 def _segment_periods(
@@ -75,13 +74,13 @@ def _segment_periods(
     df = settings.grid_duration_frac / (settings.grid_oversample * baseline)
     total = sum(max(0, math.ceil((1.0 / p_lo - 1.0 / p_hi) / df)) for p_lo, p_hi in bounds)
     check_grid_size(total, automatic=True, time=time, baseline=baseline, context=context)
-    parts: list[tuple[float, np.ndarray]] = []
-    for p_lo, p_hi in bounds:
-        freq = np.arange(1.0 / p_hi, 1.0 / p_lo, df)
-        freq = freq[freq < 1.0 / p_lo]
-        if freq.size:
-            parts.append((p_lo, np.asarray(1.0 / freq[::-1], dtype=np.float64)))
-    return parts
+    def periods_of(p_lo: float, p_hi: float) -> np.ndarray:
+        f_top = 1.0 / p_lo
+        freq = np.arange(1.0 / p_hi, f_top, df)
+        return np.asarray(1.0 / freq[freq < f_top][::-1], dtype=np.float64)
+
+    segments = ((p_lo, periods_of(p_lo, p_hi)) for p_lo, p_hi in bounds)
+    return [(p_lo, periods) for p_lo, periods in segments if periods.size]
 
 
 def _segment_grids(
@@ -112,15 +111,9 @@ def _log_segments(periods: np.ndarray, settings: BLSSettings) -> list[Segment]:
     default segment starting at ``p0 f^k``; periods keep their order."""
     p0 = float(periods.min())
     index = _log_segment_index(periods, settings)
-    segments: list[Segment] = []
-    for k in np.unique(index).tolist():
-        segments.append(
-            (
-                np.ascontiguousarray(periods[index == k]),
-                _segment_durations(p0 * settings.segment_factor**k, settings),
-            )
-        )
-    return segments
+    return [(np.ascontiguousarray(periods[index == k]),
+             _segment_durations(p0 * settings.segment_factor**k, settings))
+            for k in np.unique(index).tolist()]
 
 
 def _grid_segments(grid: GridSpec, settings: BLSSettings) -> list[Segment]:
@@ -132,22 +125,14 @@ def _grid_segments(grid: GridSpec, settings: BLSSettings) -> list[Segment]:
         return []
     if not np.all(np.isfinite(periods)):
         raise ValueError("BLS: trial periods must be finite; drop frequency 0 from the grid")
-    starts = grid.meta.get("segment_starts")
-    sizes = grid.meta.get("segment_sizes")
-    if (
-        starts is not None
-        and sizes is not None
-        and len(starts) == len(sizes)
-        and sum(int(s) for s in sizes) == periods.size
-    ):
-        segments: list[Segment] = []
-        offset = 0
-        for p_lo, size in zip(starts, sizes):
-            stop = offset + int(size)
-            segments.append((periods[offset:stop], _segment_durations(float(p_lo), settings)))
-            offset = stop
-        return segments
-    return _log_segments(periods, settings)
+    starts, sizes = grid.meta.get("segment_starts"), grid.meta.get("segment_sizes")
+    if starts is None or sizes is None or len(starts) != len(sizes):
+        return _log_segments(periods, settings)
+    ends = np.cumsum([int(n) for n in sizes], dtype=np.int64)
+    if ends.size == 0 or int(ends[-1]) != periods.size:
+        return _log_segments(periods, settings)
+    return [(periods[int(stop) - int(n):int(stop)], _segment_durations(float(p_lo), settings))
+            for p_lo, n, stop in zip(starts, sizes, ends)]
 
 
 def _segments_power(
@@ -173,32 +158,24 @@ def sde_of(power: np.ndarray) -> np.ndarray:
     return (power - mean) / std if std > 0.0 else np.zeros_like(power)
 
 
+#: Periodogram extras and the engine output each one is read from.
+_EXTRA_SOURCES = (("depth", "depth"), ("depth_snr", "depth_snr"), ("duration", "duration"),
+                  ("t0", "transit_time"))
+
+
 def _assemble(
     chunks: dict[str, list[np.ndarray]], n: int, baseline: float, backend: str, meta: Any
 ) -> Periodogram:
     """Concatenate segment outputs into one periodogram: power with NaN/+-inf set to
     0, SDE, ``frequency = 1/period``, ascending-frequency order."""
-    arrays = {key: np.concatenate(parts) for key, parts in chunks.items()}
-    power = np.nan_to_num(arrays["power"], nan=0.0, posinf=0.0, neginf=0.0)
-    sde = sde_of(power)
-    frequency = 1.0 / arrays["period"]
-    extras = {
-        "depth": arrays["depth"],
-        "depth_snr": arrays["depth_snr"],
-        "duration": arrays["duration"],
-        "t0": arrays["transit_time"],
-        "sde": sde,
-    }
+    joined = {key: np.concatenate(parts) for key, parts in chunks.items()}
+    power = np.nan_to_num(joined["power"], nan=0.0, posinf=0.0, neginf=0.0)
+    extras = {name: joined[source] for name, source in _EXTRA_SOURCES}
+    extras["sde"] = sde_of(power)
     return Periodogram.from_spectrum(
-        method="BLS",
-        backend=backend,
-        frequency=frequency,
-        power=power,
-        objective_sense="max",
-        n_samples=n,
-        baseline=baseline,
-        extras=extras,
-        meta=meta,
+        method="BLS", backend=backend, objective_sense="max",
+        frequency=1.0 / joined["period"], power=power, extras=extras,
+        n_samples=n, baseline=baseline, meta=meta,
     )
 
 
@@ -220,22 +197,16 @@ class BLSMethod(PeriodogramMethod):
         """Log period segments, each uniform in frequency, up to ``min(max_period_days,
         baseline/min_transits)``; ``meta`` records the segmentation."""
         finite = lc.finite()
-        parts = _segment_periods(finite.baseline, settings, time=finite.time)
-        if not parts:
+        segments = _segment_periods(finite.baseline, settings, time=finite.time)
+        if not segments:
             raise InsufficientDataError(
-                "BLS: baseline too short for the configured period range"
-            )
-        periods = np.concatenate([p for _, p in parts])
-        return GridSpec(
-            kind="period",
-            values=periods,
-            uniform=False,
-            meta={
-                "segmented": True,
-                "segment_starts": tuple(float(p_lo) for p_lo, _ in parts),
-                "segment_sizes": tuple(int(p.size) for _, p in parts),
-            },
-        )
+                "BLS: baseline too short for the configured period range")
+        starts = tuple(float(p_lo) for p_lo, _ in segments)
+        sizes = tuple(int(periods.size) for _, periods in segments)
+        return GridSpec(kind="period", uniform=False,
+                        values=np.concatenate([periods for _, periods in segments]),
+                        meta={"segmented": True, "segment_starts": starts,
+                              "segment_sizes": sizes})
 
     def power(
         self,
@@ -245,25 +216,14 @@ class BLSMethod(PeriodogramMethod):
         backend: str,
         engine: object | None = None,
     ) -> Periodogram:
-        finite = lc.finite()
-        n = finite.n
-        if n < settings.min_detections:
-            raise InsufficientDataError(
-                f"BLS: {n} finite points < min_detections {settings.min_detections}"
-            )
-        if finite.baseline <= 0.0:
-            raise InsufficientDataError("BLS: no usable time baseline")
-        jd = finite.time
-        flux = finite.value
-        err = finite.error if finite.error is not None else np.ones_like(flux)
+        finite = searchable_points(lc, settings.min_detections, "BLS")
         segments = _grid_segments(grid, settings)
         if not segments:
             raise InsufficientDataError("BLS: empty trial grid")
-        chunks: dict[str, list[np.ndarray]] = {name: [] for name in _SEGMENT_FIELDS}
-        for seg in _segments_power(backend, jd, flux, err, segments, settings):
-            for name in _SEGMENT_FIELDS:
-                chunks[name].append(seg[name])
-        return _assemble(chunks, n, finite.baseline, backend, finite.meta)
+        err = finite.error if finite.error is not None else np.ones_like(finite.value)
+        found = _segments_power(backend, finite.time, finite.value, err, segments, settings)
+        chunks = {name: [seg[name] for seg in found] for name in _SEGMENT_FIELDS}
+        return _assemble(chunks, finite.n, finite.baseline, backend, finite.meta)
 
     def multiband_power(
         self,

@@ -34,7 +34,7 @@ from cuperiod_amd.core.grid import GridSpec, check_grid_size
 from cuperiod_amd.core.lightcurve import LightCurve
 from cuperiod_amd.core.result import Periodogram
 from cuperiod_amd.methods._bls_core import HIP_BACKEND, _device_of, _ptr, _stream_handle, _torch
-from cuperiod_amd.methods.base import PeriodogramMethod, register
+from cuperiod_amd.methods.base import PeriodogramMethod, register, searchable_points
 
 #: Smallest admissible template energy (an empty window is skipped).
 _W_EPS: Final = 1e-300
@@ -330,18 +330,10 @@ class TLSMethod(PeriodogramMethod):
         backend: str,
         engine: object | None = None,
     ) -> Periodogram:
-        finite = lc.finite()
-        n = finite.n
-        if n < settings.min_detections:
-            raise InsufficientDataError(
-                f"TLS: {n} finite points < min_detections {settings.min_detections}"
-            )
-        if finite.baseline <= 0.0:
-            raise InsufficientDataError("TLS: no usable time baseline")
-        periods = grid.period
-        res = tls_power(finite.time, finite.value, finite.error, periods, settings=settings,
-                        backend=backend)
-        return tls_periodogram(finite, periods, res, backend)
+        finite = searchable_points(lc, settings.min_detections, "TLS")
+        found = tls_power(finite.time, finite.value, finite.error, grid.period,
+                          settings=settings, backend=backend)
+        return tls_periodogram(finite, grid.period, found, backend)
 
     def estimate_device_bytes(self, n_points: int) -> int:
         return 64 * 1024**2 + n_points * 8 * 6
