@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "cpb_set_joint_kernel",
     "cpb_tls_search",
     "cpb_linspace_grid",
+    "cpb_select_diverse_peaks",
     "cpb_set_timing",
     "cpb_main_kernel_ms",
     "cpb_set_accumulation",
@@ -194,6 +195,9 @@ def _declare(lib: ctypes.CDLL) -> None:
     ]
     lib.cpb_linspace_grid.restype = ctypes.c_int
     lib.cpb_linspace_grid.argtypes = [f64, f64, i64, ctypes.POINTER(f64)]
+    lib.cpb_select_diverse_peaks.restype = ctypes.c_int
+    lib.cpb_select_diverse_peaks.argtypes = [vp, vp, i64, i32, f64, f64, vp, i32, vp, i32, vp,
+                                             ctypes.POINTER(i32), ctypes.POINTER(i64)]
     lib.cpb_set_timing.restype = ctypes.c_int
     lib.cpb_set_timing.argtypes = [vp, ctypes.c_int]
     lib.cpb_main_kernel_ms.restype = ctypes.c_int

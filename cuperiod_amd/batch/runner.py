@@ -499,10 +499,29 @@ def _launch(prepared: Sequence[_Prepared], settings: Any, grid: GridSpec | None)
     """Pack the curves and enqueue one batched search (returns without waiting)."""
     from cuperiod_amd.batch import engine
 
+    import torch
+
     grids = [p.grid if p.grid is not None else grid for p in prepared]
     pb = engine.pack([_curve_arrays(p.finite) for p in prepared], grids, settings)
     out, summary = engine.run_packed(pb)
-    return pb, out, summary
+    searched = torch.cuda.Event()
+    searched.record(torch.cuda.current_stream(out.device))
+    return pb, out, summary, searched
+
+
+_FETCH_STREAMS: dict[int, Any] = {}
+
+
+def _fetch_stream(device) -> Any:
+    """A stream of the device's own for reading finished chunks back (default priority),
+    so the reads wait for their chunk's search only, not for the search of the next
+    chunk queued behind it on torch's current stream."""
+    import torch
+
+    stream = _FETCH_STREAMS.get(device.index)
+    if stream is None:
+        stream = _FETCH_STREAMS[device.index] = torch.cuda.Stream(device)
+    return stream
 
 
 def _collect(handle: Any, *, candidates: bool = False) -> list:
@@ -510,40 +529,78 @@ def _collect(handle: Any, *, candidates: bool = False) -> list:
     with ``candidates``, a :class:`_Candidates` (the rows need no full spectrum)."""
     from cuperiod_amd.batch import engine
 
-    pb, out, summary = handle
+    import torch
+
+    pb, out, summary, searched = handle
     if not candidates:
         res = engine.finish(pb, out, summary, spectra=True)
         return [(pb.plans[c].periods(), res.spectra[c]) for c in range(len(pb.plans))]
-    pk = engine.peak_candidates(pb, out)
-    summ = summary.cpu().numpy()
+    side = _fetch_stream(out.device)
+    side.wait_event(searched)
+    with torch.cuda.stream(side):
+        pk = engine.peak_candidates(pb, out)
+        summ = summary.cpu().numpy()
+    reader = _ChunkReader(out, pb.curve_base, pb.curve_count, side)
     results = []
     for c, plan in enumerate(pb.plans):
         n = int(pb.curve_count[c])
         base = int(pb.curve_base[c])
 
         def fetch(c=c, plan=plan, n=n, base=base):
-            host = out[:, base:base + n].cpu().numpy()
+            with torch.cuda.stream(side):
+                host = out[:, base:base + n].cpu().numpy()
             return plan.periods(), {name: host[i].copy()
                                     for i, name in enumerate(_OUTPUT_NAMES)}
-
-        def fetch_power(n=n, base=base):
-            return out[0, base:base + n].cpu().numpy()
-
-        def fetch_extras(k, base=base):
-            import torch
-
-            cols = torch.from_numpy(np.asarray(k, dtype=np.int64) + base).to(out.device)
-            return out[[1, 3, 4, 5]][:, cols].cpu().numpy().T
 
         x = pk.index[c]
         results.append(_Candidates(
             periods=plan.periods_at(n - 1 - x) if x.size else np.zeros(0),
             values=pk.values[c], total=int(pk.total[c]), written=int(pk.written[c]),
             cap=pk.cap, mean=float(summ[c, 0]), std=float(summ[c, 1]), fetch=fetch,
-            fetch_power=fetch_power, fetch_extras=fetch_extras,
-            all_periods=plan.periods,
+            fetch_power=lambda c=c: reader.power_rows([c])[c],
+            fetch_extras=lambda k, c=c: reader.extras([(c, k)])[0],
+            all_periods=plan.periods, reader=reader, curve=c,
         ))
     return results
+
+
+#: Outputs a peak's extras are read from (depth, depth_snr, duration, transit_time).
+_EXTRA_ROWS = (1, 3, 4, 5)
+
+
+class _ChunkReader:
+    """Reads finished outputs of one chunk back on the fetch stream, many curves per
+    transfer: power rows, and the extras at given columns."""
+
+    def __init__(self, out: Any, base: np.ndarray, count: np.ndarray, stream: Any) -> None:
+        self.out, self.base, self.count, self.stream = out, base, count, stream
+
+    def power_rows(self, curves: Sequence[int]) -> dict[int, np.ndarray]:
+        """Each curve's power row (ascending period)."""
+        import torch
+
+        if not curves:
+            return {}
+        spans = [(int(self.base[c]), int(self.count[c])) for c in curves]
+        with torch.cuda.stream(self.stream):
+            host = torch.cat([self.out[0, b:b + n] for b, n in spans]).cpu().numpy()
+        cuts = np.cumsum([n for _, n in spans])[:-1]
+        return dict(zip(curves, np.split(host, cuts)))
+
+    def extras(self, requests: Sequence[tuple[int, np.ndarray]]) -> list[np.ndarray]:
+        """For each ``(curve, columns)``: the ``(len(columns), 4)`` extras there."""
+        import torch
+
+        sizes = [len(k) for _, k in requests]
+        if sum(sizes) == 0:
+            return [np.zeros((0, len(_EXTRA_ROWS))) for _ in requests]
+        cols = np.concatenate([np.asarray(k, dtype=np.int64) + int(self.base[c])
+                               for c, k in requests])
+        with torch.cuda.stream(self.stream):
+            index = torch.from_numpy(cols).to(self.out.device)
+            picked = self.out.index_select(1, index)[list(_EXTRA_ROWS)]
+            host = picked.cpu().numpy().T
+        return np.split(host, np.cumsum(sizes)[:-1])
 
 
 @dataclass
@@ -562,6 +619,8 @@ class _Candidates:
     fetch_power: Callable[[], np.ndarray] | None = None
     fetch_extras: Callable[[np.ndarray], np.ndarray] | None = None
     all_periods: Callable[[], np.ndarray] | None = None
+    reader: Any = None
+    curve: int = -1
 
 
 def _peaks_from(period, freq, power, extras, chosen, mean, std) -> list[Peak]:
@@ -576,55 +635,124 @@ def _peaks_from(period, freq, power, extras, chosen, mean, std) -> list[Peak]:
     return peaks
 
 
+def _stub(prep: _Prepared, settings: Any, freq, period, score) -> Periodogram:
+    return Periodogram(method="BLS", backend="hip", frequency=freq, period=period,
+                       power=score, objective_sense="max", n_samples=prep.finite.n,
+                       baseline=prep.finite.baseline,
+                       peak_defaults=peak_defaults_from_settings(settings))
+
+
+def _row_of(prep: _Prepared, cfg: _ChunkConfig, peaks: list[Peak]) -> dict[str, Any]:
+    return peaks_to_row(prep.key, "BLS", "hip", prep.finite.n, prep.finite.baseline, peaks,
+                        n_best=cfg.n_best, extra_keys=cfg.extra_keys, source=prep.label)
+
+
+def _row_from_candidates(prep: _Prepared, cand: _Candidates, settings: Any,
+                         cfg: _ChunkConfig) -> dict[str, Any] | None:
+    """The row when the device candidates decide the selection, else ``None``."""
+    if not (cand.written <= cand.cap and cand.periods.size):
+        return None
+    count = max(int(cfg.n_best), 1)
+    freq = 1.0 / cand.periods
+    period = 1.0 / freq  # what Periodogram.from_spectrum stores
+    score = cand.values[:, 0]
+    seen: list[int] = []
+    chosen = _stub(prep, settings, freq, period, score).select_peaks(
+        freq, score, np.arange(score.size), count, alias_diverse=alias_diverse_for("BLS"),
+        examined=seen)
+    # Equal scores among the visited candidates (and the one after them) leave their
+    # order to how numpy sorts the whole candidate list. That order cannot change the
+    # outcome when no chosen peak has such a score: every member of a tied group is then
+    # blocked by a peak of higher score, chosen before the group in any order.
+    visited = np.sort(score)[::-1][:min(seen[0] + 1, score.size)]
+    values, counts = np.unique(visited, return_counts=True)
+    tied = values[counts > 1]
+    decided = (chosen.size == count or cand.written == cand.total) and (
+        tied.size == 0 or not np.isin(score[chosen], tied).any())
+    if not decided:
+        return None
+    return _row_of(prep, cfg, _peaks_from(period, freq, score, cand.values[chosen, 1:],
+                                          chosen, cand.mean, cand.std))
+
+
+def _select_on_power(prep: _Prepared, cand: _Candidates, settings: Any, cfg: _ChunkConfig,
+                     power: np.ndarray) -> tuple:
+    """The single-curve path's selection on the full power row (ascending period):
+    ``(period, freq, score, chosen, n)`` in ascending frequency."""
+    periods_all = cand.all_periods()
+    freq = (1.0 / periods_all)[::-1]
+    period = 1.0 / freq
+    score = np.nan_to_num(power[::-1], nan=0.0, posinf=0.0, neginf=0.0)
+    pg = _stub(prep, settings, freq, period, score)
+    chosen = pg.select_peaks(freq, score, pg._peak_candidates(score), max(int(cfg.n_best), 1),
+                             alias_diverse=alias_diverse_for("BLS"))
+    return period, freq, score, chosen, periods_all.size
+
+
+def _row_from_selection(prep: _Prepared, cand: _Candidates, cfg: _ChunkConfig, selection,
+                        extras: np.ndarray) -> dict[str, Any]:
+    period, freq, score, chosen, _ = selection
+    return _row_of(prep, cfg, _peaks_from(period, freq, score, extras, chosen, cand.mean,
+                                          cand.std))
+
+
 def _candidate_row(prep: _Prepared, cand: _Candidates, settings: Any,
                    cfg: _ChunkConfig) -> dict[str, Any] | None:
     """The row :func:`periodogram_to_row` gives for this curve, from its device
     candidates. When they cannot decide the selection -- it would read past them, or
-    equal scores among the visited ones make the order depend on how numpy sorts the
+    equal scores among the visited ones could change it depending on how numpy sorts the
     whole candidate list -- the selection runs on the curve's full power row instead (one
     row, not seven), with the other outputs fetched at the chosen columns only; ``None``
     when even that is unavailable. SDE uses the device's mean and standard deviation of
     the power."""
-    count = max(int(cfg.n_best), 1)
-    diverse = alias_diverse_for("BLS")
-    defaults = peak_defaults_from_settings(settings)
-
-    def stub(freq, period, score):
-        return Periodogram(method="BLS", backend="hip", frequency=freq, period=period,
-                           power=score, objective_sense="max", n_samples=prep.finite.n,
-                           baseline=prep.finite.baseline, peak_defaults=defaults)
-
-    if cand.written <= cand.cap and cand.periods.size:
-        freq = 1.0 / cand.periods
-        period = 1.0 / freq  # what Periodogram.from_spectrum stores
-        score = cand.values[:, 0]
-        seen: list[int] = []
-        chosen = stub(freq, period, score).select_peaks(
-            freq, score, np.arange(score.size), count, alias_diverse=diverse, examined=seen)
-        visited = np.sort(score)[::-1][:min(seen[0] + 1, score.size)]
-        decided = (chosen.size == count or cand.written == cand.total) and (
-            np.unique(visited).size == visited.size)
-        if decided:
-            peaks = _peaks_from(period, freq, score, cand.values[chosen, 1:], chosen,
-                                cand.mean, cand.std)
-            return peaks_to_row(prep.key, "BLS", "hip", prep.finite.n, prep.finite.baseline,
-                                peaks, n_best=cfg.n_best, extra_keys=cfg.extra_keys,
-                                source=prep.label)
+    row = _row_from_candidates(prep, cand, settings, cfg)
+    if row is not None:
+        return row
     if cand.fetch_power is None or cand.fetch_extras is None or cand.all_periods is None:
         return None
-    # the full candidate list, in the order the single-curve path builds it
-    periods_all = cand.all_periods()
-    n = periods_all.size
-    freq = (1.0 / periods_all)[::-1]
-    period = 1.0 / freq
-    score = np.nan_to_num(cand.fetch_power()[::-1], nan=0.0, posinf=0.0, neginf=0.0)
-    pg = stub(freq, period, score)
-    chosen = pg.select_peaks(freq, score, pg._peak_candidates(score), count,
-                             alias_diverse=diverse)
+    selection = _select_on_power(prep, cand, settings, cfg, cand.fetch_power())
+    chosen, n = selection[3], selection[4]
     extras = cand.fetch_extras(n - 1 - chosen) if chosen.size else np.zeros((0, 4))
-    peaks = _peaks_from(period, freq, score, extras, chosen, cand.mean, cand.std)
-    return peaks_to_row(prep.key, "BLS", "hip", prep.finite.n, prep.finite.baseline, peaks,
-                        n_best=cfg.n_best, extra_keys=cfg.extra_keys, source=prep.label)
+    return _row_from_selection(prep, cand, cfg, selection, extras)
+
+
+def _candidate_rows(prepared: Sequence[_Prepared], cands: Sequence[_Candidates],
+                    settings: Any, cfg: _ChunkConfig) -> list:
+    """:func:`_candidate_row` for a launch's curves, with the power rows and the extras
+    the undecided curves need read back in one transfer each. Per curve: the row,
+    ``None`` (the full spectrum is needed) or the exception it raised."""
+    out: list = [None] * len(prepared)
+    undecided = []
+    for i, (prep, cand) in enumerate(zip(prepared, cands)):
+        if not isinstance(cand, _Candidates):
+            continue
+        try:
+            out[i] = _row_from_candidates(prep, cand, settings, cfg)
+        except Exception as exc:  # noqa: BLE001 - recorded per light curve
+            out[i] = exc
+            continue
+        if out[i] is None and cand.reader is not None and cand.all_periods is not None:
+            undecided.append(i)
+    if not undecided:
+        return out
+    reader = cands[undecided[0]].reader
+    powers = reader.power_rows([cands[i].curve for i in undecided])
+    selections = {}
+    for i in undecided:
+        try:
+            selections[i] = _select_on_power(prepared[i], cands[i], settings, cfg,
+                                             powers[cands[i].curve])
+        except Exception as exc:  # noqa: BLE001 - recorded per light curve
+            out[i] = exc
+    order = list(selections)
+    extras = reader.extras([(cands[i].curve, selections[i][4] - 1 - selections[i][3])
+                            for i in order])
+    for i, ex in zip(order, extras):
+        try:
+            out[i] = _row_from_selection(prepared[i], cands[i], cfg, selections[i], ex)
+        except Exception as exc:  # noqa: BLE001 - recorded per light curve
+            out[i] = exc
+    return out
 
 
 class _ChunkRunner:
@@ -690,15 +818,17 @@ class _ChunkRunner:
                     if runs(pos, mi):
                         by_pos.setdefault(pos, {})[mi] = row
                 continue
-            for prep, res in zip(prepared, _collect(handle, candidates=not cfg.store_raw)):
+            results = _collect(handle, candidates=not cfg.store_raw)
+            from_candidates = (_candidate_rows(prepared, results, settings, cfg)
+                               if not cfg.store_raw else [None] * len(prepared))
+            for prep, res, row in zip(prepared, results, from_candidates):
                 if not runs(prep.position, mi):
                     continue
                 try:
-                    row = None
-                    if isinstance(res, _Candidates):
-                        row = _candidate_row(prep, res, settings, cfg)
-                        if row is None:
-                            res = res.fetch()
+                    if isinstance(row, Exception):
+                        raise row
+                    if row is None and isinstance(res, _Candidates):
+                        res = res.fetch()
                     if row is None:
                         periods, spectrum = res
                         pg = _periodogram(prep, periods, spectrum, settings)

@@ -79,24 +79,28 @@ def select_top_peaks(
 
 def _alias_lattice(harmonic_max: int) -> tuple[np.ndarray, np.ndarray]:
     """Harmonic orders ``2..harmonic_max`` and the alias offsets (sidereal and solar day
-    at k = 1, 2, then the yearly sidelobe), as float64 columns."""
-    orders = np.arange(2, harmonic_max + 1, dtype=np.float64)[:, None]
+    at k = 1, 2, then the yearly sidelobe), as float64 vectors."""
+    orders = np.arange(2, harmonic_max + 1, dtype=np.float64)
     day = np.array([SIDEREAL_FREQ_CPD, SOLAR_FREQ_CPD], dtype=np.float64)
-    offsets = np.concatenate([(np.array([1.0, 2.0])[None, :] * day[:, None]).ravel(),
-                              [YEARLY_FREQ_CPD]])[:, None]
+    offsets = np.concatenate([(day[:, None] * np.array([1.0, 2.0])[None, :]).ravel(),
+                              [YEARLY_FREQ_CPD]])
     return orders, offsets
 
 
-def _related(f_new, f_chosen: np.ndarray, tol: float, separation: float,
-             orders: np.ndarray, offsets: np.ndarray) -> np.ndarray:
-    """Element-wise over ``f_chosen``: does ``f_new`` duplicate, harmonise or alias it
-    (the test of :func:`is_alias_related`, one chosen frequency per column)."""
-    near = np.abs(f_new - f_chosen) < separation
-    harmonic = np.abs(f_new - orders * f_chosen) < orders * tol
-    subharmonic = np.abs(f_new - f_chosen / orders) < tol
-    below = np.abs(f_new - np.abs(f_chosen - offsets)) < tol
-    above = np.abs(f_new - (f_chosen + offsets)) < tol
-    return near | (harmonic | subharmonic).any(0) | (below | above).any(0)
+def _related(f_new, f_chosen, tol: float, separation: float, orders: np.ndarray,
+             offsets: np.ndarray) -> np.ndarray:
+    """The test of :func:`is_alias_related` element-wise over ``f_new`` and ``f_chosen``
+    (arrays or scalars, broadcast against each other; the harmonic orders and alias
+    offsets run along an extra last axis): does ``f_new`` duplicate, harmonise or alias
+    ``f_chosen``."""
+    a = np.asarray(f_new, dtype=np.float64)[..., None]
+    b = np.asarray(f_chosen, dtype=np.float64)[..., None]
+    near = (np.abs(a - b) < separation)[..., 0]
+    harmonic = (np.abs(a - orders * b) < orders * tol).any(-1)
+    subharmonic = (np.abs(a - b / orders) < tol).any(-1)
+    below = (np.abs(a - np.abs(b - offsets)) < tol).any(-1)
+    above = (np.abs(a - (b + offsets)) < tol).any(-1)
+    return near | harmonic | subharmonic | below | above
 
 
 def is_alias_related(
@@ -113,8 +117,43 @@ def is_alias_related(
     a sidereal/solar (k = 1, 2) or yearly alias on either side of it."""
     orders, offsets = _alias_lattice(harmonic_max)
     sep = tol if min_separation is None else min_separation
-    return bool(_related(float(f_new), np.array([float(f_chosen)]), tol, sep, orders,
-                         offsets)[0])
+    return bool(_related(float(f_new), float(f_chosen), tol, sep, orders, offsets))
+
+
+_NATIVE: list = []
+
+
+def _native_library():
+    """The engine library when it loads (its host-side selector is used then), else
+    ``None`` and the selection runs in numpy."""
+    if not _NATIVE:
+        try:
+            from cuperiod_amd import _lib
+
+            _NATIVE.append(_lib.load_library())
+        except Exception:  # noqa: BLE001 - any load failure: the numpy rule
+            _NATIVE.append(None)
+    return _NATIVE[0]
+
+
+def _select_native(lib, frequency, order, n_peaks, tol, sep, orders, offsets, examined):
+    import ctypes
+
+    freq = np.ascontiguousarray(frequency, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int64)
+    orders = np.ascontiguousarray(orders, dtype=np.float64)
+    offsets = np.ascontiguousarray(offsets, dtype=np.float64)
+    chosen = np.empty(max(int(n_peaks), 1), dtype=np.int64)
+    count, visited = ctypes.c_int32(), ctypes.c_int64()
+    status = lib.cpb_select_diverse_peaks(
+        freq.ctypes.data, order.ctypes.data, order.size, int(n_peaks), float(tol),
+        float(sep), orders.ctypes.data, orders.size, offsets.ctypes.data, offsets.size,
+        chosen.ctypes.data, ctypes.byref(count), ctypes.byref(visited))
+    if status != 0:
+        raise ValueError(f"alias-diverse selection failed (status {status})")
+    if examined is not None:
+        examined.append(int(visited.value))
+    return chosen[:count.value].copy()
 
 
 def select_diverse_peaks(
@@ -130,29 +169,53 @@ def select_diverse_peaks(
 ) -> np.ndarray:
     """Greedy highest-score peaks skipping any candidate alias-related (either way
     round) to an already chosen one. ``examined`` (a list) receives the number of
-    candidates visited."""
+    candidates visited.
+
+    Candidates are taken in score order a block at a time: the block is tested against
+    every peak chosen so far in one vectorised pass, and each peak chosen inside it
+    blocks the rest of the block the same way; the first unblocked candidate is always
+    the greedy rule's next choice."""
     if n_peaks <= 0:
         return np.empty(0, dtype=np.int64)
     orders, offsets = _alias_lattice(harmonic_max)
     sep = tol if min_separation is None else min_separation
-    picked = np.empty(0, dtype=np.int64)
-    picked_f = np.empty(0, dtype=np.float64)
-    visited = 0
-    for idx in _descending(score, candidates):
-        visited += 1
-        f = float(frequency[idx])
-        if picked.size and bool(
-            (_related(f, picked_f, tol, sep, orders, offsets)
-             | _related(picked_f, f, tol, sep, orders, offsets)).any()
-        ):
-            continue
-        picked = np.append(picked, np.int64(idx))
-        picked_f = np.append(picked_f, f)
-        if picked.size >= n_peaks:
+    order = _descending(score, candidates)
+    lib = _native_library()
+    if lib is not None:
+        return _select_native(lib, frequency, order, n_peaks, tol, sep, orders, offsets,
+                              examined)
+
+    def related(a, b):
+        return (_related(a, b, tol, sep, orders, offsets)
+                | _related(b, a, tol, sep, orders, offsets))
+
+    f = np.asarray(frequency[order], dtype=np.float64)
+    picked: list[int] = []
+    visited = order.size
+    block = 128
+    for lo in range(0, order.size, block):
+        fb = f[lo:lo + block]
+        if picked:
+            blocked = related(fb[None, :], f[picked][:, None]).any(0)
+        else:
+            blocked = np.zeros(fb.size, dtype=bool)
+        j = 0
+        while True:
+            free = np.flatnonzero(~blocked[j:])
+            if free.size == 0:
+                break
+            j += int(free[0])
+            picked.append(lo + j)
+            if len(picked) >= n_peaks:
+                visited = lo + j + 1
+                break
+            blocked[j + 1:] |= related(fb[j + 1:], fb[j])
+            j += 1
+        if len(picked) >= n_peaks:
             break
     if examined is not None:
         examined.append(visited)
-    return picked
+    return np.asarray(order[picked], dtype=np.int64)
 
 
 def peak_preserving_downsample(

@@ -2160,6 +2160,52 @@ int cpb_linspace_grid(double start, double stop, int64_t num, double* out) {
     return CPB_OK;
 }
 
+namespace {
+
+// Does a duplicate, harmonise or alias b (one direction of the alias rule).
+bool alias_related(double a, double b, double tol, double separation, const double* orders,
+                   int32_t n_orders, const double* offsets, int32_t n_offsets) {
+    if (std::fabs(a - b) < separation) return true;
+    for (int32_t k = 0; k < n_orders; ++k) {
+        const double m = orders[k];
+        if (std::fabs(a - m * b) < m * tol || std::fabs(a - b / m) < tol) return true;
+    }
+    for (int32_t k = 0; k < n_offsets; ++k) {
+        const double o = offsets[k];
+        if (std::fabs(a - std::fabs(b - o)) < tol || std::fabs(a - (b + o)) < tol) return true;
+    }
+    return false;
+}
+
+}  // namespace
+
+int cpb_select_diverse_peaks(const double* freq, const int64_t* order, int64_t n,
+                             int32_t n_peaks, double tol, double separation,
+                             const double* orders, int32_t n_orders, const double* offsets,
+                             int32_t n_offsets, int64_t* chosen, int32_t* n_chosen,
+                             int64_t* examined) {
+    if (n < 0 || n_peaks < 0 || n_orders < 0 || n_offsets < 0 || !n_chosen || !examined ||
+        (n > 0 && (!freq || !order)) || (n_peaks > 0 && !chosen) ||
+        (n_orders > 0 && !orders) || (n_offsets > 0 && !offsets))
+        return CPB_E_INVALID;
+    int32_t count = 0;
+    int64_t visited = 0;
+    for (int64_t i = 0; i < n && count < n_peaks; ++i) {
+        ++visited;
+        const double f = freq[order[i]];
+        bool free = true;
+        for (int32_t c = 0; c < count && free; ++c) {
+            const double g = freq[chosen[c]];
+            free = !alias_related(f, g, tol, separation, orders, n_orders, offsets, n_offsets) &&
+                   !alias_related(g, f, tol, separation, orders, n_orders, offsets, n_offsets);
+        }
+        if (free) chosen[count++] = order[i];
+    }
+    *n_chosen = count;
+    *examined = visited;
+    return CPB_OK;
+}
+
 int cpb_set_timing(cpb_context* ctx, int enable) {
     if (!ctx) return CPB_E_INVALID;
     CPB_HIP(ctx, hipSetDevice(ctx->device));

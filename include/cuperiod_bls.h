@@ -269,6 +269,22 @@ int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau, const doub
  * start, last = stop), computed on the HOST: the values mode-1 segments use. */
 int cpb_linspace_grid(double start, double stop, int64_t num, double* out);
 
+/* Greedy alias-diverse peak choice on the HOST (replaces the loop of
+ * src/cuperiod/core/peaks.py:169-231, select_diverse_peaks, with its alias test
+ * is_alias_related, :121-166): candidates are visited in the given order (most
+ * significant first); one is chosen unless it is related, either way round, to a peak
+ * already chosen -- |a - b| < separation, |a - m*b| < m*tol or |a - b/m| < tol for each
+ * harmonic order m in orders[0..n_orders), |a - |b - o|| < tol or |a - (b + o)| < tol
+ * for each alias offset o in offsets[0..n_offsets) -- until n_peaks are chosen.
+ * freq[i] is candidate i's frequency; chosen[0..*n_chosen) receives their indices
+ * into freq, *examined the number of candidates visited. Every comparison is the
+ * same IEEE double expression the Python rule evaluates. */
+int cpb_select_diverse_peaks(const double* freq, const int64_t* order, int64_t n,
+                             int32_t n_peaks, double tol, double separation,
+                             const double* orders, int32_t n_orders, const double* offsets,
+                             int32_t n_offsets, int64_t* chosen, int32_t* n_chosen,
+                             int64_t* examined);
+
 /* Enable (1) or disable (0) HIP-event timing of the main search kernel of later
  * searches on this context (the events are recorded on the search's stream). */
 int cpb_set_timing(cpb_context* ctx, int enable);

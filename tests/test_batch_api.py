@@ -312,3 +312,27 @@ def test_candidates_decide_distinct_scores_without_the_power_row():
     row = runner._candidate_row(prep, cand, settings, cfg)
     assert row is not None
     _same(row, _expected_row("k", lc))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_tied_scores_decided_only_when_order_cannot_matter(seed):
+    """Spectra with many exactly tied local maxima: the row built from the candidates
+    (which skips the power row when no chosen peak's score is tied among the visited
+    candidates) equals the row of the full power row's selection."""
+    rng = np.random.default_rng(1000 + seed)
+    settings = BLSSettings()
+    prep = runner._prepare(0, "k", _curve(seed, n=300, baseline=40.0), settings, None)
+    cfg = runner._ChunkConfig(("BLS",), {"BLS": "hip"}, {"BLS": settings}, 10, False, None,
+                              bio.peak_extra_columns(["BLS"]))
+    n = int(rng.integers(2_000, 20_000))
+    periods = np.sort(1.0 / np.linspace(0.025, 2.0, n))
+    levels = int(rng.integers(3, 40))
+    spectrum = {"power": np.round(rng.gamma(2.0, 1.0, n) * levels) / levels,
+                "depth": rng.uniform(0, 1e-3, n), "depth_snr": rng.uniform(0, 9, n),
+                "duration": rng.uniform(0.01, 0.2, n), "transit_time": rng.uniform(0, 5, n)}
+    for want in (64, 512, 4096):
+        cand = _as_candidates(periods, spectrum, want)
+        forced = _as_candidates(periods, spectrum, want, cap=1)
+        forced.written = max(forced.written, 2)  # over the cap: the power row decides
+        _same(runner._candidate_row(prep, cand, settings, cfg),
+              runner._candidate_row(prep, forced, settings, cfg))

@@ -370,3 +370,36 @@ def test_parametric_width_cuts_equal_materialised_cuts():
             assert ranges == want
             crossing += 1
     assert crossing >= 6
+
+
+def test_native_alias_diverse_selection_equals_numpy_rule():
+    """The library's host-side greedy (cpb_select_diverse_peaks) chooses the same peaks,
+    in the same order, and visits as many candidates as the numpy rule, on spectra
+    built around harmonics, sub-harmonics and day/year aliases of a base frequency."""
+    from cuperiod_amd.core import peaks
+
+    assert peaks._native_library() is not None
+    rng = np.random.default_rng(11)
+    for trial in range(600):
+        n = int(rng.integers(5, 400))
+        base = rng.uniform(0.05, 1.5)
+        f = np.abs(np.sort(np.concatenate([
+            rng.uniform(0.001, 5, n // 2),
+            base * rng.integers(1, 6, n - n // 2) + rng.normal(0, 2e-3, n - n // 2),
+            base / rng.integers(1, 6, 5),
+            base + np.array([1.0027379, 1.0, 2.0, 1 / 365.25, -1.0])]))) + 1e-4
+        score = rng.normal(size=f.size)
+        if trial % 4 == 0:
+            score = np.round(score, 1)
+        cand = rng.permutation(f.size)[: max(1, f.size // 2)]
+        tol = float(rng.choice([0.0035, 0.01, 0.05, 0.2]))
+        args = (f, score, cand, int(rng.integers(0, 25)), tol, int(rng.integers(1, 9)))
+        kw = {"min_separation": None if trial % 2 else float(rng.uniform(0, 0.05))}
+        seen_native, seen_numpy = [], []
+        native = peaks.select_diverse_peaks(*args, examined=seen_native, **kw)
+        peaks._NATIVE[:] = [None]
+        try:
+            rule = peaks.select_diverse_peaks(*args, examined=seen_numpy, **kw)
+        finally:
+            peaks._NATIVE[:] = []
+        assert np.array_equal(native, rule) and seen_native == seen_numpy, trial
