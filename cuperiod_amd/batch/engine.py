@@ -391,13 +391,31 @@ def _device():
     return torch.device("cuda", ctx.device)
 
 
+_UPLOAD_STREAMS: dict[int, Any] = {}
+
+
 def _upload(a: np.ndarray, dtype, device) -> Any:
+    """``a`` on the device. The copy runs on an upload stream of its own (default
+    priority), so packing the next batch does not wait for the search still running on
+    torch's current stream; that stream waits for the copy before any later work, and
+    the tensor is marked as used there for the allocator."""
     import torch
 
     a = np.ascontiguousarray(a, dtype=dtype)
     if a.size == 0:
         a = np.zeros(1, dtype=dtype)
-    return torch.from_numpy(a).to(device)
+    device = torch.device(device)
+    up = _UPLOAD_STREAMS.get(device.index)
+    if up is None:
+        up = _UPLOAD_STREAMS[device.index] = torch.cuda.Stream(device)
+    with torch.cuda.stream(up):
+        t = torch.from_numpy(a).to(device)
+        copied = torch.cuda.Event()
+        copied.record(up)
+    current = torch.cuda.current_stream(device)
+    current.wait_event(copied)
+    t.record_stream(current)
+    return t
 
 
 #: numpy layouts of the descriptors (field for field the ctypes structures, which
