@@ -448,10 +448,12 @@ __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, i
     // fused wrap copy is made by the first pass: the lanes owning an overwritten bin read
     // its source (bins 1..os, which nobody writes in that pass) and store it in place, so
     // the second pass reads every bin as copied.
+    // (Used by the 1,024-thread workgroups, which have waves to spare; with 256 threads
+    // one wave per band scanning rounds of 64 bins, and no exchange, measured faster.)
     constexpr int CMAX = 16;
     const int wpb = gk <= NW ? NW / gk : 1;
     const int c = (span + wpb * kWave - 1) / (wpb * kWave);
-    if (gk <= NW && c <= CMAX) {
+    if (T >= 1024 && gk <= NW && c <= CMAX) {
         __shared__ int64_t s_part[2 * NW];
         const int w_u = __builtin_amdgcn_readfirstlane(wave);
         const int kk = w_u / wpb, part = w_u - kk * wpb;
@@ -682,14 +684,16 @@ __global__ __launch_bounds__(T, T == 256 ? 3 : 1) void bls_joint_group_kernel(
             const int bn = found ? (int)(bkey & 0xFFFFFFu) : 0;
             const int bdur = found ? s_dur[bkey >> 24] : 0;
 
-            // With more than one band group and the wrap copy's sources apart from the bins
-            // it overwrites, the winner's prefix sums come from one pass over the star's
-            // points instead of re-folding every other group: with W = [nb - os, nb) the
+            // In 1,024-thread workgroups with more than one band group and the wrap copy's
+            // sources apart from the bins it overwrites, the winner's prefix sums come from
+            // one pass over the star's points instead of re-folding every other group (in
+            // 256-thread ones the LDS atomics of that pass, all lanes of a wave on one
+            // band's four sums, cost more than the re-fold): with W = [nb - os, nb) the
             // overwritten bins, a point in bin b adds to the prefix at X once for
             // b <= X, b outside W, and once more for 1 <= b <= os with its copy's bin
             // nb - os + b - 1 <= X (exact int64 sums, rounded to FP64 as the folded
             // prefixes are, so the outputs are the same).
-            const bool direct = found && K > group && nb - oversample > oversample;
+            const bool direct = T >= 1024 && found && K > group && nb - oversample > oversample;
             if (direct) {
                 __shared__ unsigned long long s_acc[4 * kJointMaxBands];
                 for (int i = tid; i < 4 * K; i += T) s_acc[i] = 0ull;
