@@ -1,6 +1,6 @@
 """``batch_periodograms``: BLS over many light curves, one launch per chunk.
 
-The driver of cuPeriod's batch API (src/cuperiod/batch/runner.py:151-185, 734-1032)
+cuPeriod's batch API (src/cuperiod/batch/runner.py:151-185, 734-1032)
 with the same arguments, ``BatchSummary``, row format, sinks and resume rules, but a
 different execution model. cuPeriod's ``_process_chunk`` (``:214-250``) calls
 ``method.power`` once per light curve, each with its own uploads, launches and a

@@ -29,6 +29,7 @@
 // the reference's depth^2/(1/W_in + 1/W_out) without its divisions and cancellations.
 
 #include <climits>
+#include <type_traits>
 
 #include "engine_internal.h"
 
@@ -369,10 +370,15 @@ hipError_t dispatch_joint(int dmax, dim3 grid, size_t lds, hipStream_t st, const
 // FP64 operations and a Newton-refined reciprocal. The winning box's per-band sums are
 // read from the resident group and from re-folds of the others.
 
-// Workgroup sizes of the band-group kernel: 1,024 threads for wide spans (one workgroup
-// per CU holds up to 160 KiB of band bins), 256 for spans up to 1,024 bins (smaller band
-// groups, several workgroups per CU).
+// Workgroup sizes of the band-group kernel. Up to 8 durations: 512 threads, for spans up
+// to 1,024 bins with two starts per thread and band groups of at most a third of the LDS
+// (three workgroups per CU), for spans up to 2,048 bins with four starts per thread and
+// half of the LDS (two per CU). More durations: 256 threads over spans up to 512 bins
+// (a third of the LDS), else 1,024 (one workgroup per CU, the whole LDS).
 constexpr int kGroupThreads = 1024;
+template <int V>
+using IntC = std::integral_constant<int, V>;
+constexpr int kGroupThreadsMid = 512;
 constexpr int kGroupThreadsNarrow = 256;
 
 struct BandTerm {
@@ -448,12 +454,13 @@ __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, i
     // fused wrap copy is made by the first pass: the lanes owning an overwritten bin read
     // its source (bins 1..os, which nobody writes in that pass) and store it in place, so
     // the second pass reads every bin as copied.
-    // (Used by the 1,024-thread workgroups, which have waves to spare; with 256 threads
-    // one wave per band scanning rounds of 64 bins, and no exchange, measured faster.)
+    // (Used by workgroups of 512 or more threads, which have waves to spare; with 256
+    // threads one wave per band scanning rounds of 64 bins, and no exchange, measured
+    // faster.)
     constexpr int CMAX = 16;
     const int wpb = gk <= NW ? NW / gk : 1;
     const int c = (span + wpb * kWave - 1) / (wpb * kWave);
-    if (T >= 1024 && gk <= NW && c <= CMAX) {
+    if (T >= 512 && gk <= NW && c <= CMAX) {
         __shared__ int64_t s_part[2 * NW];
         const int w_u = __builtin_amdgcn_readfirstlane(wave);
         const int kk = w_u / wpb, part = w_u - kk * wpb;
@@ -536,7 +543,8 @@ __device__ __forceinline__ void fold_group(int64_t* S, int wl, int g0, int gk, i
 }
 
 template <int T, int DMAX, int RMAX, int OBJ>
-__global__ __launch_bounds__(T, T == 256 ? 3 : 1) void bls_joint_group_kernel(
+__global__ __launch_bounds__(T, T == 256 ? 3 : (T == 512 ? (RMAX == 4 ? 4 : 6) : 1)) void
+bls_joint_group_kernel(
     const double* __restrict__ tau, const int64_t* __restrict__ yq,
     const int64_t* __restrict__ wq, const CurveParams* __restrict__ bp,
     const cpb_star* __restrict__ stars, const cpb_segment* __restrict__ segs,
@@ -684,16 +692,18 @@ __global__ __launch_bounds__(T, T == 256 ? 3 : 1) void bls_joint_group_kernel(
             const int bn = found ? (int)(bkey & 0xFFFFFFu) : 0;
             const int bdur = found ? s_dur[bkey >> 24] : 0;
 
-            // In 1,024-thread workgroups with more than one band group and the wrap copy's
+            // In workgroups that hold the whole LDS or half of it (1,024 threads; 512 with
+            // four starts per thread), with more than one band group and the wrap copy's
             // sources apart from the bins it overwrites, the winner's prefix sums come from
             // one pass over the star's points instead of re-folding every other group (in
-            // 256-thread ones the LDS atomics of that pass, all lanes of a wave on one
-            // band's four sums, cost more than the re-fold): with W = [nb - os, nb) the
+            // the third-of-LDS ones the LDS atomics of that pass, all lanes of a wave on
+            // one band's four sums, cost more than the re-fold): with W = [nb - os, nb) the
             // overwritten bins, a point in bin b adds to the prefix at X once for
             // b <= X, b outside W, and once more for 1 <= b <= os with its copy's bin
             // nb - os + b - 1 <= X (exact int64 sums, rounded to FP64 as the folded
             // prefixes are, so the outputs are the same).
-            const bool direct = T >= 1024 && found && K > group && nb - oversample > oversample;
+            constexpr bool kWide = T >= 1024 || (T == 512 && RMAX == 4);
+            const bool direct = kWide && found && K > group && nb - oversample > oversample;
             if (direct) {
                 __shared__ unsigned long long s_acc[4 * kJointMaxBands];
                 for (int i = tid; i < 4 * K; i += T) s_acc[i] = 0ull;
@@ -825,21 +835,24 @@ hipError_t dispatch_group(int dmax, int span, dim3 grid, size_t lds, hipStream_t
                           const WorkItem* items, int64_t n_items, const double* periods,
                           const int32_t* dur, int32_t os, int32_t wl, int32_t group,
                           double* out, int64_t n_out) {
-    // 1,024-thread workgroups over spans of up to 2,048 bins need two starts per thread:
-    // the smaller tile of running totals leaves registers for the sweep's chains
-    if constexpr (T >= 1024)
-        if (span <= 2 * T && dmax <= 8)
-            return launch_group<T, 8, 2, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs,
-                                              items, n_items, periods, dur, os, wl, group, out,
-                                              n_out);
-    if (dmax <= 8)
-        return launch_group<T, 8, 4, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
-                                          n_items, periods, dur, os, wl, group, out, n_out);
-    if (dmax <= 16)
-        return launch_group<T, 16, 2, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
-                                           n_items, periods, dur, os, wl, group, out, n_out);
-    return launch_group<T, 32, 1, OBJ>(grid, lds, st, tau, yq, wq, bp, stars, segs, items,
-                                       n_items, periods, dur, os, wl, group, out, n_out);
+    auto go = [&](auto d, auto r) {
+        return launch_group<T, decltype(d)::value, decltype(r)::value, OBJ>(
+            grid, lds, st, tau, yq, wq, bp, stars, segs, items, n_items, periods, dur, os, wl,
+            group, out, n_out);
+    };
+    // 512-thread workgroups (launched for at most 8 durations): two starts per thread over
+    // spans of up to 2T bins, four up to 4T. 1,024-thread ones over spans of up to 2T with
+    // at most 8 durations also take two: the smaller tile of running totals leaves
+    // registers for the sweep's chains.
+    if constexpr (T == kGroupThreadsMid) {
+        return span <= 2 * T ? go(IntC<8>{}, IntC<2>{}) : go(IntC<8>{}, IntC<4>{});
+    } else {
+        if constexpr (T >= 1024)
+            if (span <= 2 * T && dmax <= 8) return go(IntC<8>{}, IntC<2>{});
+        if (dmax <= 8) return go(IntC<8>{}, IntC<4>{});
+        if (dmax <= 16) return go(IntC<16>{}, IntC<2>{});
+        return go(IntC<32>{}, IntC<1>{});
+    }
 }
 
 }  // namespace
@@ -921,7 +934,7 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
     const int64_t target_items = (int64_t)ctx->n_cu * 64;
     int64_t tile = live / std::max<int64_t>(1, target_items);
     tile = std::max<int64_t>(1, std::min<int64_t>(tile, 32));
-    // Span classes of the band-group kernel: 0 up to 1,024 bins (256 threads), 1 wider.
+    // Span classes of the band-group kernel: 0 up to 1,024 bins, 1 wider.
     struct SpanClass {
         int64_t begin = 0, count = 0;
         int wmax = 2, dmax = 1;
@@ -1007,8 +1020,12 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
         const size_t bb = (size_t)16 * ((cls[c].wmax + 63) / 64 * 64);
         const int span = cls[c].wmax - 1;
         const int rmax = group_rmax(cls[c].dmax);
-        if (c == 0 && span <= rmax * kGroupThreadsNarrow) threads_of[c] = kGroupThreadsNarrow;
-        else if (span <= rmax * kGroupThreads) threads_of[c] = kGroupThreads;
+        if (cls[c].dmax <= 8 && span <= (c == 0 ? 2 : 4) * kGroupThreadsMid)
+            threads_of[c] = kGroupThreadsMid;
+        else if (c == 0 && span <= rmax * kGroupThreadsNarrow)
+            threads_of[c] = kGroupThreadsNarrow;
+        else if (span <= rmax * kGroupThreads)
+            threads_of[c] = kGroupThreads;
         if (threads_of[c] == 0 || bb + static_lds > (size_t)ctx->lds_limit) group_fits = false;
     }
     const int mode = ctx->joint_kernel;
@@ -1021,33 +1038,32 @@ extern "C" int cpb_bls_joint_search(cpb_context* ctx, void* stream, const double
             const int T = threads_of[c];
             const int wlc = (cls[c].wmax + 63) / 64 * 64;
             const size_t bb = (size_t)16 * wlc;
-            // 256-thread workgroups keep their band groups to a third of the LDS, so three
-            // or more share a CU; 1,024-thread ones use all of it.
-            const size_t budget = T == kGroupThreadsNarrow
-                                      ? (size_t)ctx->lds_limit / 3 - static_lds
-                                      : (size_t)ctx->lds_limit - static_lds;
+            // Band groups of a third of the LDS in the narrow class (three workgroups per
+            // CU), of half of it for 512-thread workgroups in the wide class (two per CU),
+            // of all of it for 1,024-thread ones.
+            const int share = T == kGroupThreads ? 1 : (c == 0 ? 3 : 2);
+            const size_t budget = (size_t)ctx->lds_limit / share - static_lds;
             const int gmax = std::max<int>(1, (int)(budget / bb));
             const int n_groups = (kmax + gmax - 1) / gmax;
             const int group = (kmax + n_groups - 1) / n_groups;
             const size_t lds = bb * group;
             const dim3 grid((unsigned)std::min<int64_t>(cls[c].count, ((int64_t)1 << 31) / T));
             const WorkItem* items = ip + cls[c].begin;
+            auto run = [&](auto threads, auto obj) {
+                return dispatch_group<decltype(threads)::value, decltype(obj)::value>(
+                    cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp,
+                    items, cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out);
+            };
+            auto run_obj = [&](auto threads) {
+                return obj_flag == 0 ? run(threads, IntC<0>{})
+                                     : run(threads, IntC<1>{});
+            };
             if (T == kGroupThreadsNarrow)
-                e = obj_flag == 0
-                        ? dispatch_group<kGroupThreadsNarrow, 0>(
-                              cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp, items,
-                              cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out)
-                        : dispatch_group<kGroupThreadsNarrow, 1>(
-                              cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp, items,
-                              cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out);
+                e = run_obj(IntC<kGroupThreadsNarrow>{});
+            else if (T == kGroupThreadsMid)
+                e = run_obj(IntC<kGroupThreadsMid>{});
             else
-                e = obj_flag == 0
-                        ? dispatch_group<kGroupThreads, 0>(
-                              cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp, items,
-                              cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out)
-                        : dispatch_group<kGroupThreads, 1>(
-                              cls[c].dmax, cls[c].wmax - 1, grid, lds, st, tau, yq, wq, params, sp, gp, items,
-                              cls[c].count, periods, dur_bins, oversample, wlc, group, out, n_out);
+                e = run_obj(IntC<kGroupThreads>{});
             if (e != hipSuccess) return hip_fail(ctx, e, "bls_joint_group_kernel launch");
         }
         if (timed) {

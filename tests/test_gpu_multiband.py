@@ -94,6 +94,20 @@ def test_shared_matches_oracle(objective, joint_kernel):
     _check_shared(pg, _oracle_shared(bands, grid, settings))
 
 
+@pytest.mark.parametrize("bins_per_duration", [2, 5])
+def test_shared_many_durations_matches_oracle(bins_per_duration, joint_kernel):
+    # 11-12 durations (the 512-thread band-group variants hold at most 8): over spans of
+    # at most 403 bins (2 bins per duration) the 256-thread variant, over 1,006 bins (5)
+    # the 1,024-thread one
+    bands, _ = multiband_star(11)
+    mblc = _mblc(bands)
+    settings = SETTINGS.model_copy(update={"n_durations": 12,
+                                           "bins_per_duration": bins_per_duration})
+    grid = _grid(mblc, settings)
+    pg = bls_mb.bls_multiband_power(grid, mblc, settings, "hip", BLS, ephemeris="shared")
+    _check_shared(pg, _oracle_shared(bands, grid, settings))
+
+
 @pytest.mark.parametrize("n_bands", [3, 6])
 def test_shared_default_grid_matches_oracle(n_bands, joint_kernel):
     # default settings over a year: up to ~2000 phase bins per period, so 3 bands keep
