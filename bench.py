@@ -685,7 +685,8 @@ def main() -> int:
     ap.add_argument("--long-curves", type=int, default=8)
     ap.add_argument("--learned-curves", type=int, default=16384)
     ap.add_argument("--learned-plain-curves", type=int, default=2048)
-    ap.add_argument("--e2e-chunk", type=int, default=512)
+    ap.add_argument("--e2e-chunk", type=int, default=256,
+                    help="batch_periodograms chunk size (its own default, 256)")
     ap.add_argument("--mb-stars", type=int, default=32)
     ap.add_argument("--tls-curves", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
