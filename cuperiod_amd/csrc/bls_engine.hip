@@ -366,7 +366,7 @@ bls_main_kernel(
 
     const int tid = threadIdx.x;
     const int lane = tid % kWave;
-    const int wave = tid / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
 
     const int n_slots = slot_of(wl - 1) + 1;
     int64_t* S;  // slots: S[2*s] = y, S[2*s + 1] = w
@@ -389,8 +389,12 @@ bls_main_kernel(
         __syncthreads();
         if (tid < n_dur) s_dur[tid] = dur_bins[sg.dur_offset + tid];
         // The item's periods, their reciprocals and bin counts, one lane per period.
-        if (tid >= kWave && tid < kWave + it.count) {
-            const int q = tid - kWave;
+        // lane-dependent values are formed where they are used (the empty asm keeps them
+        // from being hoisted and held, or spilled, across the work item)
+        int ltid = tid;
+        asm volatile("" : "+v"(ltid));
+        if (ltid >= kWave && ltid < kWave + it.count) {
+            const int q = ltid - kWave;
             const double P = segment_period(sg, periods, it.begin + q);
             int64_t nb = (int64_t)ceil(P / bd) + oversample;
             if (nb > width - 1) nb = width - 1;
@@ -767,8 +771,10 @@ bls_main_kernel(
         // (c) finalize in FP64 from each winner's exact sums and write the seven outputs:
         // one lane per period of the item (coalesced rows).
         __syncthreads();
-        if (!(cv.flags & 2) && tid < it.count) {
-            const int q = tid;
+        int ftid = tid;
+        asm volatile("" : "+v"(ftid));
+        if (!(cv.flags & 2) && ftid < it.count) {
+            const int q = ftid;
             const double P = segment_period(sg, periods, it.begin + q);
             const int64_t col = sg.out_offset + it.begin + q;
             const uint32_t bk = (uint32_t)s_nb[q];
