@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "cpb_set_joint_kernel",
     "cpb_tls_search",
     "cpb_linspace_grid",
+    "cpb_spectrum_candidates",
     "cpb_select_diverse_peaks",
     "cpb_set_timing",
     "cpb_main_kernel_ms",
@@ -198,6 +199,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.cpb_select_diverse_peaks.restype = ctypes.c_int
     lib.cpb_select_diverse_peaks.argtypes = [vp, vp, i64, i32, f64, f64, vp, i32, vp, i32, vp,
                                              ctypes.POINTER(i32), ctypes.POINTER(i64)]
+    lib.cpb_spectrum_candidates.restype = ctypes.c_int
+    lib.cpb_spectrum_candidates.argtypes = [vp, i64, vp, vp, ctypes.POINTER(i64)]
     lib.cpb_set_timing.restype = ctypes.c_int
     lib.cpb_set_timing.argtypes = [vp, ctypes.c_int]
     lib.cpb_main_kernel_ms.restype = ctypes.c_int

@@ -132,27 +132,42 @@ class CurvePlan:
         raise IndexError("period index out of range")
 
     def periods_at(self, k: np.ndarray) -> np.ndarray:
-        """Periods of ascending indices ``k`` (an int array), by the device's recipe."""
+        """Periods of ascending indices ``k`` (an int array), by the device's recipe:
+        each index's segment is found by bisection and the parametric periods are formed
+        in one vectorised pass."""
         k = np.asarray(k, dtype=np.int64)
-        out = np.empty(k.shape, dtype=np.float64)
-        start = 0
-        for sg in self.segments:
-            sel = (k >= start) & (k < start + sg.n_periods)
-            if sel.any():
-                kk = k[sel] - start
-                if sg.mode == 0:
-                    out[sel] = sg.periods[kk]
-                else:
-                    i = sg.lin_num - 1 - (sg.lin_k0 + kk)
-                    f = i.astype(np.float64) * sg.lin_step + sg.lin_start
-                    if sg.mode == 1:
-                        f = np.where(i == sg.lin_num - 1, sg.lin_stop, f)
-                    else:
-                        f = np.where(i == 0, sg.lin_start, f)
-                    out[sel] = 1.0 / f
-            start += sg.n_periods
-        if np.any((k < 0) | (k >= start)):
+        sizes = np.array([sg.n_periods for sg in self.segments], dtype=np.int64)
+        ends = np.cumsum(sizes)
+        if np.any((k < 0) | (k >= (int(ends[-1]) if ends.size else 0))):
             raise IndexError("period index out of range")
+        out = np.empty(k.shape, dtype=np.float64)
+        if k.size == 0:
+            return out
+        sid = np.searchsorted(ends, k, side="right")
+        segs = self.segments
+        modes = np.array([sg.mode for sg in segs], dtype=np.int64)
+        # grid index i of each period: num - 1 - (k0 + k - segment start), parametric ones
+        top = np.array([sg.lin_num - 1 - sg.lin_k0 for sg in segs], dtype=np.int64)
+        top += ends - sizes
+        i = np.take(top, sid) - k
+        start = np.take(np.array([sg.lin_start for sg in segs], dtype=np.float64), sid)
+        f = i.astype(np.float64) * np.take(np.array([sg.lin_step for sg in segs]), sid) + start
+        # the grid's end samples are its exact bounds (mode 1: i = num - 1 is stop; mode 2:
+        # i = 0 is start)
+        last = np.take(np.array([sg.lin_num - 1 for sg in segs], dtype=np.int64), sid)
+        for j in np.flatnonzero((i == 0) | (i == last)).tolist():
+            s = int(sid[j])
+            if modes[s] == 1 and i[j] == last[j]:
+                f[j] = segs[s].lin_stop
+            elif modes[s] == 2 and i[j] == 0:
+                f[j] = segs[s].lin_start
+        with np.errstate(divide="ignore"):  # explicit segments' f is 0, replaced below
+            np.divide(1.0, f, out=out)
+        if (modes == 0).any():
+            explicit = np.take(modes, sid) == 0
+            for s in np.unique(sid[explicit]).tolist():
+                sel = sid == s
+                out[sel] = segs[s].periods[k[sel] - int(ends[s] - sizes[s])]
         return out
 
     def periods(self) -> np.ndarray:
@@ -549,11 +564,11 @@ def _parametric_need(par: dict[str, np.ndarray], idx: np.ndarray, k: np.ndarray,
     return np.ceil((1.0 / f) / par["bd"][idx]).astype(np.int64) + int(oversample) + 1
 
 
-def _parametric_width_ranges(sgs: Sequence[_SegPlan], oversample: int
-                             ) -> list[list[tuple[int, int, int]]]:
-    """``core._width_ranges`` of many parametric segments at once: their periods ascend
-    with the index, so the first index past each span limit is found by a vectorised
-    bisection instead of materialising every period."""
+def _parametric_width_ranges(sgs: Sequence[_SegPlan], oversample: int) -> list[np.ndarray]:
+    """``core._width_ranges`` of many parametric segments at once, per segment an int64
+    array of ``(start, stop, width)`` rows: their periods ascend with the index, so the
+    first index past each span limit is found by a vectorised bisection instead of
+    materialising every period."""
     par = {
         "start": np.array([sg.lin_start for sg in sgs], dtype=np.float64),
         "step": np.array([sg.lin_step for sg in sgs], dtype=np.float64),
@@ -585,13 +600,9 @@ def _parametric_width_ranges(sgs: Sequence[_SegPlan], oversample: int
     starts, ends, seg_of = starts[keep], ends[keep], seg_of[keep]
     widths = _parametric_need(par, seg_of, ends - 1, oversample)
     counts = np.bincount(seg_of, minlength=n_seg)
-    rows = list(zip(starts.tolist(), ends.tolist(), widths.tolist()))
-    out: list[list[tuple[int, int, int]]] = []
-    pos = 0
-    for m in counts.tolist():
-        out.append(rows[pos:pos + m])
-        pos += m
-    return out
+    rows = np.stack([starts, ends, widths], axis=1).astype(np.int64)
+    bounds = np.concatenate([[0], np.cumsum(counts)]).tolist()
+    return [rows[a:b] for a, b in zip(bounds[:-1], bounds[1:])]
 
 
 def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) -> PackedBatch:
@@ -603,42 +614,35 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
     accumulation = core._accumulation_code(settings.accumulation, settings.precision)
     plans = list(plans)
     # descriptor pieces: a segment whose periods need different phase-bin layouts is cut
-    # where the width crosses the engine's span limits (outputs do not depend on it)
-    pieces: list[tuple[int, _SegPlan, int, int, int]] = []
-    parametric: list[tuple[int, int, _SegPlan]] = []
+    # where the width crosses the engine's span limits (outputs do not depend on it);
+    # per segment an int64 array of (start, stop, width) rows
+    seg_rows: list[tuple[int, _SegPlan, np.ndarray | None]] = []
     group_of: dict[int, int] = {}
-    explicit_cuts: dict[int, list] = {}
+    bd_of: dict[int, float] = {}
+    explicit_cuts: dict[int, np.ndarray] = {}
+    todo: list[_SegPlan] = []
     for c, plan in enumerate(plans):
         for pos, sg in enumerate(plan.segments):
             group_of[id(sg)] = sg.group if sg.group >= 0 else pos
-            bd = float(sg.durations.min()) / oversample
+            bd = bd_of[id(sg)] = float(sg.durations.min()) / oversample
             width = int(np.ceil(sg.p_max / bd)) + oversample + 1
+            rows = None
             if width <= core._SPAN_LIMITS[0] or sg.n_periods < 2:
-                pieces.append((c, sg, 0, sg.n_periods, width))
+                rows = np.array([[0, sg.n_periods, width]], dtype=np.int64)
             elif sg.mode == 0:
-                rng = explicit_cuts.get(id(sg))
-                if rng is None:
-                    rng = explicit_cuts[id(sg)] = core._width_ranges(sg.periods, bd, oversample)
-                pieces.extend((c, sg, a, b, w) for a, b, w in rng)
-            else:
-                parametric.append((len(pieces), c, sg))
-                pieces.append((c, sg, -1, -1, -1))  # filled in below
-    if parametric:
-        todo = [sg for _, _, sg in parametric if oversample not in sg.cuts]
-        if todo:
-            for sg, rng in zip(todo, _parametric_width_ranges(todo, oversample)):
-                sg.cuts[oversample] = rng
-        expanded: list[tuple[int, _SegPlan, int, int, int]] = []
-        slots = {pos: (c, sg, sg.cuts[oversample]) for pos, c, sg in parametric}
-        for pos, piece in enumerate(pieces):
-            if pos in slots:
-                c, sg, rng = slots[pos]
-                expanded.extend((c, sg, a, b, w) for a, b, w in rng)
-            else:
-                expanded.append(piece)
-        pieces = expanded
-    n_pieces = len(pieces)
-    table = np.zeros(max(1, n_pieces), dtype=SEG_DTYPE)
+                rows = explicit_cuts.get(id(sg))
+                if rows is None:
+                    rows = explicit_cuts[id(sg)] = np.array(
+                        core._width_ranges(sg.periods, bd, oversample), dtype=np.int64
+                    ).reshape(-1, 3)
+            elif oversample not in sg.cuts:
+                todo.append(sg)
+            seg_rows.append((c, sg, rows))
+    if todo:
+        for sg, rng in zip(todo, _parametric_width_ranges(todo, oversample)):
+            sg.cuts[oversample] = rng
+    seg_rows = [(c, sg, sg.cuts[oversample] if rows is None else rows)
+                for c, sg, rows in seg_rows]
     dur_parts: list[np.ndarray] = []
     per_parts: list[np.ndarray] = []
     p_off = d_off = 0
@@ -646,22 +650,16 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
     # per segment, once: its bin duration, box widths (deduplicated by value) and group
     seg_info: dict[int, tuple[float, int, int, int]] = {}
     dur_cache: dict[tuple[float, bytes], tuple[int, int]] = {}
-    per_cache: dict[tuple[int, int, int], int] = {}
-    col_c = np.empty(n_pieces, dtype=np.int64)
-    col_a = np.empty(n_pieces, dtype=np.int64)
-    col_b = np.empty(n_pieces, dtype=np.int64)
-    col_w = np.empty(n_pieces, dtype=np.int64)
-    col_s = np.empty(n_pieces, dtype=np.int64)
-    col_p = np.zeros(n_pieces, dtype=np.int64)
     seg_list: list[_SegPlan] = []
     seg_pos: dict[int, int] = {}
-    for j, (c, sg, a, b, width) in enumerate(pieces):
+    entry_pos = np.empty(len(seg_rows), dtype=np.int64)
+    for e, (c, sg, rows) in enumerate(seg_rows):
         key = id(sg)
         pos = seg_pos.get(key)
         if pos is None:
             pos = seg_pos[key] = len(seg_list)
             seg_list.append(sg)
-            bd = float(sg.durations.min()) / oversample
+            bd = bd_of[key]
             dkey = (bd, sg.durations.tobytes())
             hit = dur_cache.get(dkey)
             if hit is None:
@@ -670,16 +668,30 @@ def pack_plans(cs: CurveSet, plans: Sequence[CurvePlan], settings: BLSSettings) 
                 dur_parts.append(db)
                 d_off += int(db.size)
             seg_info[key] = (bd, hit[0], hit[1], group_of[key])
-        col_c[j], col_a[j], col_b[j], col_w[j], col_s[j] = c, a, b, width, pos
-        if sg.mode == 0:
-            # the same explicit periods (one array shared by several plans) go up once
+        entry_pos[e] = pos
+    counts = np.array([rows.shape[0] for _, _, rows in seg_rows], dtype=np.int64)
+    n_pieces = int(counts.sum())
+    table = np.zeros(max(1, n_pieces), dtype=SEG_DTYPE)
+    if n_pieces:
+        cuts = np.concatenate([rows for _, _, rows in seg_rows])
+        col_a, col_b, col_w = cuts[:, 0], cuts[:, 1], cuts[:, 2]
+        col_c = np.repeat(np.array([c for c, _, _ in seg_rows], dtype=np.int64), counts)
+        col_s = np.repeat(entry_pos, counts)
+    col_p = np.zeros(n_pieces, dtype=np.int64)
+    # the same explicit periods (one array shared by several plans) go up once
+    per_cache: dict[tuple[int, int, int], int] = {}
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]]) if counts.size else counts
+    for e, (c, sg, rows) in enumerate(seg_rows):
+        if sg.mode != 0:
+            continue
+        for r, (a, b, _) in enumerate(rows.tolist()):
             pkey = (id(sg.periods), a, b)
             off = per_cache.get(pkey)
             if off is None:
                 off = per_cache[pkey] = p_off
                 per_parts.append(sg.periods[a:b])
                 p_off += b - a
-            col_p[j] = off
+            col_p[first[e] + r] = off
     if n_pieces:
         info = np.array([seg_info[id(sg)] for sg in seg_list], dtype=np.float64)
         lin = np.array([(sg.mode, sg.lin_start, sg.lin_step, sg.lin_stop, sg.lin_num,

@@ -26,6 +26,7 @@ every ``(key, method)`` pair the sink already holds.
 
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import hashlib
 import itertools
@@ -40,6 +41,7 @@ from typing import Any
 
 import numpy as np
 
+from cuperiod_amd import _lib
 from cuperiod_amd.batch.io import (
     InputItem,
     alias_diverse_for,
@@ -559,7 +561,7 @@ def _collect(handle: Any, *, candidates: bool = False) -> list:
             cap=pk.cap, mean=float(summ[c, 0]), std=float(summ[c, 1]), fetch=fetch,
             fetch_power=lambda c=c: reader.power_rows([c])[c],
             fetch_extras=lambda k, c=c: reader.extras([(c, k)])[0],
-            all_periods=plan.periods, reader=reader, curve=c,
+            periods_at=plan.periods_at, reader=reader, curve=c,
         ))
     return results
 
@@ -618,7 +620,7 @@ class _Candidates:
     fetch: Callable[[], tuple]
     fetch_power: Callable[[], np.ndarray] | None = None
     fetch_extras: Callable[[np.ndarray], np.ndarray] | None = None
-    all_periods: Callable[[], np.ndarray] | None = None
+    periods_at: Callable[[np.ndarray], np.ndarray] | None = None
     reader: Any = None
     curve: int = -1
 
@@ -675,18 +677,39 @@ def _row_from_candidates(prep: _Prepared, cand: _Candidates, settings: Any,
                                           chosen, cand.mean, cand.std))
 
 
+def _spectrum_candidates(power: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """The single-curve path's peak candidates of an engine power row (ascending period)
+    on its ascending-frequency spectrum, non-finite powers read as 0: their indices, in
+    the order that path lists them, and their scores (``cpb_spectrum_candidates``)."""
+    lib = _lib.load_library()
+    row = np.ascontiguousarray(power, dtype=np.float64)
+    index = np.empty(row.size // 2 + 2, dtype=np.int64)
+    score = np.empty(row.size // 2 + 2, dtype=np.float64)
+    count = ctypes.c_int64()
+    status = lib.cpb_spectrum_candidates(row.ctypes.data, row.size, index.ctypes.data,
+                                         score.ctypes.data, ctypes.byref(count))
+    if status != 0:
+        raise RuntimeError(f"cpb_spectrum_candidates: status {status}")
+    return index[:count.value], score[:count.value]
+
+
 def _select_on_power(prep: _Prepared, cand: _Candidates, settings: Any, cfg: _ChunkConfig,
                      power: np.ndarray) -> tuple:
     """The single-curve path's selection on the full power row (ascending period):
-    ``(period, freq, score, chosen, n)`` in ascending frequency."""
-    periods_all = cand.all_periods()
-    freq = (1.0 / periods_all)[::-1]
+    ``(period, freq, score, chosen, columns)``, the first three over the row's peak
+    candidates (``chosen`` indexes them), ``columns`` the chosen samples' engine columns.
+    The candidates' scores are the array the single-curve path sorts (the same values in
+    the same order), so equal scores fall into the same order; only the candidates'
+    periods are formed."""
+    n = int(power.size)
+    index, score = _spectrum_candidates(power)
+    columns = n - 1 - index
+    freq = 1.0 / cand.periods_at(columns)
     period = 1.0 / freq
-    score = np.nan_to_num(power[::-1], nan=0.0, posinf=0.0, neginf=0.0)
     pg = _stub(prep, settings, freq, period, score)
-    chosen = pg.select_peaks(freq, score, pg._peak_candidates(score), max(int(cfg.n_best), 1),
+    chosen = pg.select_peaks(freq, score, np.arange(score.size), max(int(cfg.n_best), 1),
                              alias_diverse=alias_diverse_for("BLS"))
-    return period, freq, score, chosen, periods_all.size
+    return period, freq, score, chosen, columns[chosen]
 
 
 def _row_from_selection(prep: _Prepared, cand: _Candidates, cfg: _ChunkConfig, selection,
@@ -708,11 +731,11 @@ def _candidate_row(prep: _Prepared, cand: _Candidates, settings: Any,
     row = _row_from_candidates(prep, cand, settings, cfg)
     if row is not None:
         return row
-    if cand.fetch_power is None or cand.fetch_extras is None or cand.all_periods is None:
+    if cand.fetch_power is None or cand.fetch_extras is None or cand.periods_at is None:
         return None
     selection = _select_on_power(prep, cand, settings, cfg, cand.fetch_power())
-    chosen, n = selection[3], selection[4]
-    extras = cand.fetch_extras(n - 1 - chosen) if chosen.size else np.zeros((0, 4))
+    columns = selection[4]
+    extras = cand.fetch_extras(columns) if columns.size else np.zeros((0, 4))
     return _row_from_selection(prep, cand, cfg, selection, extras)
 
 
@@ -731,7 +754,7 @@ def _candidate_rows(prepared: Sequence[_Prepared], cands: Sequence[_Candidates],
         except Exception as exc:  # noqa: BLE001 - recorded per light curve
             out[i] = exc
             continue
-        if out[i] is None and cand.reader is not None and cand.all_periods is not None:
+        if out[i] is None and cand.reader is not None and cand.periods_at is not None:
             undecided.append(i)
     if not undecided:
         return out
@@ -745,8 +768,7 @@ def _candidate_rows(prepared: Sequence[_Prepared], cands: Sequence[_Candidates],
         except Exception as exc:  # noqa: BLE001 - recorded per light curve
             out[i] = exc
     order = list(selections)
-    extras = reader.extras([(cands[i].curve, selections[i][4] - 1 - selections[i][3])
-                            for i in order])
+    extras = reader.extras([(cands[i].curve, selections[i][4]) for i in order])
     for i, ex in zip(order, extras):
         try:
             out[i] = _row_from_selection(prepared[i], cands[i], cfg, selections[i], ex)

@@ -2224,6 +2224,47 @@ int cpb_select_diverse_peaks(const double* freq, const int64_t* order, int64_t n
     return CPB_OK;
 }
 
+int cpb_spectrum_candidates(const double* power, int64_t n, int64_t* index, double* score,
+                            int64_t* count) {
+    if (n < 0 || !count || (n > 0 && (!power || !index || !score))) return CPB_E_INVALID;
+    // sample i of the ascending-frequency spectrum, non-finite values read as 0
+    auto s = [&](int64_t i) {
+        const double v = power[n - 1 - i];
+        return std::isfinite(v) ? v : 0.0;
+    };
+    int64_t m = 0;
+    auto keep = [&](int64_t i, double v) {
+        index[m] = i;
+        score[m] = v;
+        ++m;
+    };
+    if (n >= 3) {
+        // interior local maxima, ascending: above the left neighbour, at least the right
+        double left = s(0), mid = s(1);
+        for (int64_t i = 1; i + 1 < n; ++i) {
+            const double right = s(i + 1);
+            if (mid > left && mid >= right) keep(i, mid);
+            left = mid;
+            mid = right;
+        }
+    }
+    if (n == 1) {
+        keep(0, s(0));
+    } else if (n >= 2) {
+        if (s(0) >= s(1)) keep(0, s(0));
+        if (s(n - 1) > s(n - 2)) keep(n - 1, s(n - 1));
+    }
+    if (m == 0 && n > 0) {
+        // no candidate: the first largest sample
+        int64_t best = 0;
+        for (int64_t i = 1; i < n; ++i)
+            if (s(i) > s(best)) best = i;
+        keep(best, s(best));
+    }
+    *count = m;
+    return CPB_OK;
+}
+
 int cpb_set_timing(cpb_context* ctx, int enable) {
     if (!ctx) return CPB_E_INVALID;
     CPB_HIP(ctx, hipSetDevice(ctx->device));

@@ -269,6 +269,20 @@ int cpb_tls_search(cpb_context* ctx, void* stream, const double* tau, const doub
  * start, last = stop), computed on the HOST: the values mode-1 segments use. */
 int cpb_linspace_grid(double start, double stop, int64_t num, double* out);
 
+/* Peak candidates of one spectrum on the HOST (replaces the candidate rule of
+ * src/cuperiod/core/result.py:369-406, Periodogram._peak_candidates, which best_periods,
+ * :285, applies to a BLS spectrum whose non-finite powers _assemble has set to 0,
+ * src/cuperiod/methods/bls.py:280). power[0..n) is one output row of the engine, in ascending
+ * period, i.e. descending frequency; the rule runs on the ascending-frequency spectrum
+ * s[i] = power[n-1-i] (0 where not finite): interior local maxima (s[i] > s[i-1] and
+ * s[i] >= s[i+1]) in ascending i, then i = 0 if s[0] >= s[1], then i = n-1 if
+ * s[n-1] > s[n-2] (n == 1: i = 0); with none, the first largest sample.
+ * index[0..*count) receives the ascending-frequency indices i in that order and
+ * score[0..*count) their values s[i]; both hold up to n/2 + 2 entries (no two interior
+ * maxima are adjacent, and an end that qualifies excludes its neighbour). */
+int cpb_spectrum_candidates(const double* power, int64_t n, int64_t* index, double* score,
+                            int64_t* count);
+
 /* Greedy alias-diverse peak choice on the HOST (replaces the loop of
  * src/cuperiod/core/peaks.py:169-231, select_diverse_peaks, with its alias test
  * is_alias_related, :121-166): candidates are visited in the given order (most

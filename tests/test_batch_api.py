@@ -279,7 +279,7 @@ def _as_candidates(periods, spectrum, want, cap=None):
         fetch_extras=lambda kk: np.stack([spectrum[q][kk] for q in
                                           ("depth", "depth_snr", "duration", "transit_time")],
                                          axis=1),
-        all_periods=lambda: periods)
+        periods_at=lambda k: periods[k])
 
 
 @pytest.mark.parametrize("seed", [1, 3, 7, 11])

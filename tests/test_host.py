@@ -364,12 +364,57 @@ def test_parametric_width_cuts_equal_materialised_cuts():
         per = CurvePlan(segments=[sg], n_periods=sg.n_periods).periods_at(
             np.arange(sg.n_periods))
         want = core._width_ranges(per, bd, settings.bins_per_duration)
+        ranges = [tuple(r) for r in ranges.tolist()]
         if len(want) == 1:
             assert len(ranges) == 1 and ranges[0][:2] == (0, sg.n_periods)
         else:
             assert ranges == want
             crossing += 1
     assert crossing >= 6
+
+
+def test_native_spectrum_candidates_equal_the_periodogram_rule():
+    """cpb_spectrum_candidates lists an engine power row's candidates (indices into the
+    ascending-frequency spectrum, in the order the selection sees them) and scores as
+    Periodogram._peak_candidates does on the row reversed with non-finite powers read as
+    0: flat tops, plateaus at both ends, NaN/inf, 0-3 samples and many exact ties."""
+    import ctypes
+
+    from cuperiod_amd import _lib
+    from cuperiod_amd.core.result import Periodogram
+
+    lib = _lib.load_library()
+    rng = np.random.default_rng(5)
+    rows = [np.zeros(0), np.array([3.0]), np.array([1.0, 2.0]), np.array([2.0, 1.0]),
+            np.array([1.0, 1.0]), np.array([1.0, 2.0, 3.0]), np.array([3.0, 2.0, 1.0]),
+            np.array([np.nan, np.inf, -np.inf]), np.full(7, 4.0), np.array([1.0, 3.0, 3.0, 1.0]),
+            np.tile([0.0, 1.0], 500), np.append(np.tile([1.0, 0.0], 500), 1.0)]
+    for trial in range(400):
+        n = int(rng.integers(1, 3000))
+        row = np.round(rng.gamma(2.0, 1.0, n) * int(rng.integers(1, 30))) if trial % 2 else \
+            rng.normal(size=n)
+        bad = rng.random(n) < 0.02
+        row[bad] = rng.choice([np.nan, np.inf, -np.inf], int(bad.sum()))
+        rows.append(row)
+    for row in rows:
+        n = row.size
+        score = np.nan_to_num(row[::-1], nan=0.0, posinf=0.0, neginf=0.0)
+        freq = np.linspace(0.1, 2.0, n)
+        pg = Periodogram(method="BLS", backend="hip", frequency=freq, period=1.0 / freq,
+                         power=score, objective_sense="max", n_samples=10, baseline=50.0)
+        want = pg._peak_candidates(score)
+        r = np.ascontiguousarray(row, dtype=np.float64)
+        index = np.empty(n // 2 + 2, dtype=np.int64)
+        got = np.empty(n // 2 + 2, dtype=np.float64)
+        count = ctypes.c_int64()
+        assert lib.cpb_spectrum_candidates(r.ctypes.data, n, index.ctypes.data,
+                                           got.ctypes.data, ctypes.byref(count)) == 0
+        m = count.value
+        assert np.array_equal(index[:m], want), row[:12]
+        assert np.array_equal(got[:m], score[want])
+        # the order numpy's argsort gives the candidates' scores is the same
+        assert np.array_equal(index[:m][np.argsort(got[:m])[::-1]],
+                              want[np.argsort(score[want])[::-1]])
 
 
 def test_native_alias_diverse_selection_equals_numpy_rule():
