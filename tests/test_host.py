@@ -344,6 +344,33 @@ def test_local_maxima_and_downsample():
     assert pd.max() == np.float32(5.0) and fd[np.argmax(pd)] == np.float32(1234.0)
 
 
+def test_periods_at_equals_the_materialised_grid():
+    """CurvePlan.periods_at gives the periods() array's values bit for bit at any
+    indices (unordered, repeated, segment edges) of linspace, default and explicit plans
+    and of a plan mixing explicit and parametric segments."""
+    from cuperiod_amd.batch.engine import (
+        CurvePlan, LinspaceGrid, _default_plan, _explicit_plan, _linspace_plan)
+    from cuperiod_amd.core.grid import GridSpec
+
+    settings = cup.BLSSettings(n_durations=6)
+    rng = np.random.default_rng(8)
+    t = np.sort(rng.uniform(0.0, 300.0, 500))
+    plans = [_linspace_plan(0, LinspaceGrid(1.0 / 700.0, 2.0, 30_000), settings),
+             _default_plan(1, float(t[-1] - t[0]), t, settings),
+             _explicit_plan(2, GridSpec(kind="period",
+                                        values=np.sort(rng.uniform(0.3, 40.0, 5_000))), settings)]
+    plans.append(CurvePlan(segments=plans[2].segments + plans[0].segments,
+                           n_periods=plans[2].n_periods + plans[0].n_periods))
+    for plan in plans:
+        every = plan.periods()
+        edges = np.cumsum([0] + [sg.n_periods for sg in plan.segments])
+        k = np.concatenate([rng.integers(0, plan.n_periods, 4_000), edges[:-1], edges[1:] - 1,
+                            np.arange(plan.n_periods)[::-7]])
+        assert np.array_equal(plan.periods_at(k).view(np.int64), every[k].view(np.int64))
+    with pytest.raises(IndexError):
+        plans[0].periods_at(np.array([plans[0].n_periods]))
+
+
 def test_parametric_width_cuts_equal_materialised_cuts():
     # The batch planner cuts parametric (linspace) segments at the engine's span classes
     # by bisection on the period index; the cuts must equal those of the materialised
