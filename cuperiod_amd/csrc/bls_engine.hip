@@ -332,10 +332,15 @@ __device__ __forceinline__ float screen_value(int2 h0, int2 h1, int s_hi, int e_
 
 // KMODE 3 is MODE 0 for spans of up to 1,152 bins: its 19 KiB of LDS admit 8 waves per
 // SIMD, and the register budget is capped for them; KMODE 4 is MODE 0 capped at 7 waves
-// per SIMD (spans of up to 1,344 bins: 22 KiB, seven workgroups per CU); MODE 0 is capped
-// at 6 waves per SIMD.
+// per SIMD (spans of up to 1,344 bins: 22 KiB, seven workgroups per CU); KMODE 5 is MODE 0
+// for spans of up to 640 bins in 128-thread workgroups (11 KiB: fourteen of them, two
+// waves each, per CU; the per-wave work of a period, its scans, maxima and winner, is
+// then shared by two waves instead of four); MODE 0 is capped at 6 waves per SIMD.
 template <int T, int DMAX, int OBJ, int KMODE>
-__global__ __launch_bounds__(T, KMODE == 3 ? 8 : (KMODE == 4 ? 7 : (KMODE == 0 ? 6 : 1))) void
+__global__ __launch_bounds__(T, KMODE == 3 || KMODE == 5 ? 8
+                                : KMODE == 4             ? 7
+                                : KMODE == 0             ? 6
+                                                         : 1) void
 bls_main_kernel(
     const double* __restrict__ tau, const int64_t* __restrict__ yq,
     const int64_t* __restrict__ wq, const CurveParams* __restrict__ curves,
@@ -344,11 +349,13 @@ bls_main_kernel(
     const int32_t* __restrict__ dur_bins, int32_t oversample, int32_t wl,
     int64_t* __restrict__ wide_scratch, double* __restrict__ out, int64_t n_out) {
     extern __shared__ int64_t dyn_lds[];
-    constexpr int MODE = (KMODE == 3 || KMODE == 4) ? 0 : KMODE;
-    // bins per thread of the MODE 0 prefix: spans of up to 1,152 bins (KMODE 3) need at
-    // most 1,024/T + 1 of them (the chunk is odd: CHUNK + 1 registers), the wider ones
-    // kChunk
-    constexpr int CHUNK = KMODE == 3 ? (1024 / T < kChunk ? 1024 / T : kChunk) : kChunk;
+    constexpr int MODE = (KMODE == 3 || KMODE == 4 || KMODE == 5) ? 0 : KMODE;
+    // bins per thread of the MODE 0 prefix: spans of up to 640 bins (KMODE 5) need at
+    // most the odd ceil(640/T), spans of up to 1,152 bins (KMODE 3) at most 1,024/T + 1
+    // (CHUNK + 1 registers each), the wider ones kChunk
+    constexpr int CHUNK = KMODE == 5   ? (((640 + T - 1) / T) | 1) - 1
+                          : KMODE == 3 ? (1024 / T < kChunk ? 1024 / T : kChunk)
+                                       : kChunk;
     constexpr int NW = T / kWave;
     constexpr bool GLOBAL = MODE == 2;
     __shared__ int32_t s_dur[DMAX];
@@ -1594,6 +1601,7 @@ hipError_t dispatch_d(int dmax, const LaunchArgs& a) {
 
 template <int T, int OBJ>
 hipError_t dispatch_mode(int mode, int dmax, const LaunchArgs& a) {
+    if (mode == 5) return dispatch_d<128, OBJ, 5>(dmax, a);
     if (mode == 3) return dispatch_d<T, OBJ, 3>(dmax, a);
     if (mode == 4) {
         if constexpr (T == 256) return dispatch_d<T, OBJ, 4>(dmax, a);
@@ -1737,24 +1745,27 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
     // launch per layout, so a wide segment does not push narrow ones out of LDS. (The
     // reduced-precision kernel takes every item in one launch.)
     const int threads = ctx->threads;
-    // Classes: 0 MODE 0 up to 1,152 bins (18 KiB of LDS; its own instantiation for 8
-    // waves per SIMD: 1,152 is the widest span whose odd prefix chunk fits its registers
-    // at 128, 256 and 512 threads), 1 up to 1,344 bins (21 KiB, 7 waves per SIMD with 256
-    // threads), 2-4 MODE 0 up to 1,600 / 1,920 / threads*8 bins (16 bytes a bin: 25, 30,
-    // 32 KiB, the most bins that leave room for 6, 5, 4 workgroups in a CU's 160 KiB with
-    // up to 1.25 KiB of static LDS each), 5 MODE 1, 6 MODE 2.
-    constexpr int kClasses = 7;
+    // Classes: 0 MODE 0 up to 640 bins in 128-thread workgroups whatever the launch
+    // shape (10 KiB of LDS, 8 waves per SIMD), 1 up to 1,152 bins (18 KiB; its own
+    // instantiation for 8 waves per SIMD: 1,152 is the widest span whose odd prefix chunk
+    // fits its registers at 128, 256 and 512 threads), 2 up to 1,344 bins (21 KiB, 7
+    // waves per SIMD with 256 threads), 3-5 MODE 0 up to 1,600 / 1,920 / threads*8 bins
+    // (16 bytes a bin: 25, 30, 32 KiB, the most bins that leave room for 6, 5, 4
+    // workgroups in a CU's 160 KiB with up to 1.25 KiB of static LDS each), 6 MODE 1,
+    // 7 MODE 2.
+    constexpr int kClasses = 8;
     auto layout_of = [&](int width) {
         if (ctx->accumulation != 0) return 0;
         const int w = (width + 63) / 64 * 64;
         const size_t need = (size_t)16 * (w + (w - 1) / 8 + 1);
-        if (need + 2048 > (size_t)ctx->lds_limit) return 6;
-        if (w > threads * kChunk) return 5;
-        // class 0's prefix holds an odd chunk of at most 1,024/threads + 1 bins a thread
-        if (w <= 1152 && (((w + threads - 1) / threads) | 1) <= 1024 / threads + 1) return 0;
-        return w <= 1344 ? 1 : (w <= 1600 ? 2 : (w <= 1920 ? 3 : 4));
+        if (need + 2048 > (size_t)ctx->lds_limit) return 7;
+        if (w > threads * kChunk) return 6;
+        if (w <= 640) return 0;
+        // class 1's prefix holds an odd chunk of at most 1,024/threads + 1 bins a thread
+        if (w <= 1152 && (((w + threads - 1) / threads) | 1) <= 1024 / threads + 1) return 1;
+        return w <= 1344 ? 2 : (w <= 1600 ? 3 : (w <= 1920 ? 4 : 5));
     };
-    constexpr int kModeOf[kClasses] = {3, 4, 0, 0, 0, 1, 2};
+    constexpr int kModeOf[kClasses] = {5, 3, 4, 0, 0, 0, 1, 2};
     struct Layout {
         int64_t begin = 0, count = 0;
         int wmax = 2, dmax = 1;
@@ -1911,7 +1922,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         const int wlL = (lay.wmax + 63) / 64 * 64;
         const int mode = kModeOf[L];
         const int n_slots = wlL + (wlL - 1) / 8 + 1;
-        const size_t lds_need = (size_t)16 * (mode == 0 || mode == 3 || mode == 4 ? wlL : n_slots);
+        const size_t lds_need = (size_t)16 * (mode == 0 || mode >= 3 ? wlL : n_slots);
         (void)static_lds;
         int64_t* wide_ptr = nullptr;
         LaunchArgs la;
@@ -1926,14 +1937,15 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
         } else {
             // fewer than 2^32 work-items per launch (a graph kernel node refuses more);
             // the kernel loops over the items its grid does not cover
-            const int64_t cap = ((int64_t)1 << 31) / threads;
+            const int tl = mode == 5 ? 128 : threads;
+            const int64_t cap = ((int64_t)1 << 31) / tl;
             // a persistent grid holds no more workgroups per CU than can be resident at
             // once (the class's LDS, with up to 1.25 KiB of static LDS each, and the
             // waves its register cap admits); more would run after the others in a tail
             int64_t g = cap;
             if (persistent > 0) {
-                const int waves_cap = mode == 3 ? 8 : (mode == 4 ? 7 : 6);
-                const int by_waves = std::max(1, 4 * waves_cap / (threads / kWave));
+                const int waves_cap = mode == 3 || mode == 5 ? 8 : (mode == 4 ? 7 : 6);
+                const int by_waves = std::max(1, 4 * waves_cap / (tl / kWave));
                 const int by_lds = std::max<int>(1, ctx->lds_per_cu / (int)(lds_need + 1280));
                 g = (int64_t)ctx->n_cu * std::min(ctx->grid_per_cu, std::min(by_waves, by_lds));
             }

@@ -284,12 +284,12 @@ def _bls_search_hip_specs(
 
 
 #: Phase-bin spans (bins per period, padded) where the engine's launch classes change:
-#: registers and LDS with 8 waves per SIMD (1,152), registers and LDS sized for 7, 6, 5
-#: and 4 workgroups per CU (1,344, 1,600, 1,920, 256 threads x 8 bins), LDS only
-#: (160 KiB); wider spans go to global memory. The engine classifies a segment by its
-#: widest span; cutting segments at the same limits keeps each period in the class
-#: with the most workgroups per CU its span allows.
-_SPAN_LIMITS = (1152, 1344, 1600, 1920, 2048, 8960)
+#: 128-thread workgroups (640), registers and LDS with 8 waves per SIMD (1,152), registers
+#: and LDS sized for 7, 6, 5 and 4 workgroups per CU (1,344, 1,600, 1,920, 256 threads x 8
+#: bins), LDS only (160 KiB); wider spans go to global memory. The engine classifies a
+#: segment by its widest span; cutting segments at the same limits keeps each period in
+#: the class with the most workgroups per CU its span allows.
+_SPAN_LIMITS = (640, 1152, 1344, 1600, 1920, 2048, 8960)
 
 
 def _width_of(p_max: float, bin_duration: float, oversample: int) -> int:
