@@ -470,26 +470,46 @@ bls_main_kernel(
                         atomicAdd(acc + 2 * sl + 1, (unsigned long long)b);
                     }
                 };
+                // two points per thread per pass, both loaded before either is folded
+                auto fold_range = [&](int j0, int step, int end) {
+                    for (; j0 + step < end; j0 += 2 * step) {
+                        double x0, x1;
+                        int64_t a0, b0, a1, b1;
+                        point(j0, x0, a0, b0);
+                        point(j0 + step, x1, a1, b1);
+                        const int i0 = fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r);
+                        const int i1 = fold_bin_tol(x1, P, invP, bd, inv_bd, width, tol_q, tol_r);
+                        add_point(i0, a0, b0);
+                        add_point(i1, a1, b1);
+                    }
+                    if (j0 < end) {
+                        double x0;
+                        int64_t a0, b0;
+                        point(j0, x0, a0, b0);
+                        add_point(fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r), a0, b0);
+                    }
+                };
                 // the lane's first point is formed per period (the empty asm keeps it
                 // from being hoisted and held in a register across the work item)
                 int j0 = tid;
                 asm volatile("" : "+v"(j0));
-                // two points per thread per pass, both loaded before either is folded
-                for (; j0 + T < npts; j0 += 2 * T) {
-                    double x0, x1;
-                    int64_t a0, b0, a1, b1;
-                    point(j0, x0, a0, b0);
-                    point(j0 + T, x1, a1, b1);
-                    const int i0 = fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r);
-                    const int i1 = fold_bin_tol(x1, P, invP, bd, inv_bd, width, tol_q, tol_r);
-                    add_point(i0, a0, b0);
-                    add_point(i1, a1, b1);
-                }
-                if (j0 < npts) {
-                    double x0;
-                    int64_t a0, b0;
-                    point(j0, x0, a0, b0);
-                    add_point(fold_bin_tol(x0, P, invP, bd, inv_bd, width, tol_q, tol_r), a0, b0);
+                // Consecutive points are consecutive lanes. When they advance the phase by
+                // less than a bin on average (periods long against the point spacing, as
+                // in the chunked coarse search of the learned proposer), a wave's lanes
+                // fall into the same few bins and their LDS atomics serialise; then groups
+                // of 8 lanes walk T/8 contiguous blocks of the curve instead. (Only in the
+                // 128-thread class, where the coarse search runs: in the wider classes the
+                // test and the second loop cost more than they saved.)
+                bool spread = false;
+                if constexpr (KMODE == 5)
+                    spread = (double)cv.x_max * (double)n_bins < (double)npts * P;
+                if (spread) {
+                    constexpr int GL = 8, NG = T / GL;
+                    const int blk = ((npts + NG - 1) / NG + GL - 1) / GL * GL;
+                    const int g0 = (j0 / GL) * blk;
+                    fold_range(g0 + j0 % GL, GL, min(g0 + blk, npts));
+                } else {
+                    fold_range(j0, T, npts);
                 }
             }
             if constexpr (GLOBAL) __threadfence();

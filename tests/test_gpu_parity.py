@@ -157,6 +157,28 @@ def test_mid_span_histogram_path_matches_oracle():
         _say(f"  errors {errs}")
 
 
+def test_long_periods_on_dense_sampling_match_oracle():
+    # 2,500 points over 20 d searched at 2-60 d with 0.1 d bins (spans of up to ~600
+    # bins, the 128-thread class): consecutive points advance the phase by under a bin,
+    # the case whose fold walks the curve in blocks of 8 lanes (as in the learned
+    # proposer's coarse search of time chunks)
+    _say("case long periods, dense sampling")
+    rng = np.random.default_rng(12)
+    t = np.sort(rng.uniform(0.0, 20.0, 2500))
+    flux = 1.0 + 1e-3 * rng.standard_normal(t.size)
+    flux[np.mod(t - 2.0, 7.3) < 0.4] -= 2e-3
+    err = np.full(t.size, 1e-3) * (1.0 + 0.5 * rng.random(t.size))
+    periods = np.geomspace(2.0, 60.0, 300)
+    durations = np.array([0.3, 0.6, 1.2])
+    assert float(np.diff(t).mean()) * (60.0 / 0.1) / 60.0 < 1.0
+    for objective in ("snr", "likelihood"):
+        got = _hip_power(t, flux, err, periods, durations, 3, objective)
+        ref = oracle.bls_power(t, flux, err, periods, durations, 3, objective=objective)
+        errs = compare(got, ref, name=f"dense/{objective}",
+                       inputs=BoxInputs.for_specs(t, [(periods, durations)], 3))
+        _say(f"  errors {errs}")
+
+
 @pytest.mark.parametrize("objective", ["snr", "likelihood"])
 def test_skewed_flux_and_wide_boxes_match_oracle(objective):
     # The FP32 screen's error bounds at their edges: skewed flux (median far from the
