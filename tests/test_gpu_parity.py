@@ -157,6 +157,28 @@ def test_mid_span_histogram_path_matches_oracle():
         _say(f"  errors {errs}")
 
 
+@pytest.mark.parametrize("n_dur", [7, 20])
+def test_many_durations_across_launch_classes_match_oracle(n_dur):
+    # 7 and 20 box widths (the 8- and 32-duration builds) over periods whose spans run
+    # from ~300 to ~1,500 bins: the segment is cut into the 128-thread, 8-wave,
+    # 7-workgroup and 6-workgroup launch classes
+    _say(f"case {n_dur} durations across launch classes")
+    rng = np.random.default_rng(30 + n_dur)
+    t = np.sort(rng.uniform(0.0, 120.0, 1500))
+    flux = 1.0 + 1e-3 * rng.standard_normal(t.size)
+    flux[np.mod(t - 1.3, 6.1) < 0.15] -= 2.5e-3
+    err = np.full(t.size, 1e-3)
+    periods = np.linspace(3.0, 15.0, 400)
+    durations = np.geomspace(0.03, 0.5, n_dur)
+    spec = core._search_spec(periods, durations, 3)
+    assert spec.width > 1400
+    got = _hip_power(t, flux, err, periods, durations, 3)
+    ref = oracle.bls_power(t, flux, err, periods, durations, 3)
+    errs = compare(got, ref, name=f"durations-{n_dur}",
+                   inputs=BoxInputs.for_specs(t, [(periods, durations)], 3))
+    _say(f"  errors {errs}")
+
+
 def test_long_periods_on_dense_sampling_match_oracle():
     # 2,500 points over 20 d searched at 2-60 d with 0.1 d bins (spans of up to ~600
     # bins, the 128-thread class): consecutive points advance the phase by under a bin,
