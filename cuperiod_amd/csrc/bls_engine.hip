@@ -496,15 +496,17 @@ bls_main_kernel(
                 // Consecutive points are consecutive lanes. When they advance the phase by
                 // less than a bin on average (periods long against the point spacing, as
                 // in the chunked coarse search of the learned proposer), a wave's lanes
-                // fall into the same few bins and their LDS atomics serialise; then groups
-                // of 8 lanes walk T/8 contiguous blocks of the curve instead. (Only in the
+                // fall into the same few bins and their LDS atomics serialise; then pairs
+                // of lanes walk T/2 contiguous blocks of the curve instead (each pair's
+                // loads are still 16 contiguous bytes; groups of 8 and 4 lanes were 20%
+                // and 4% slower on the coarse search, single lanes 62%). (Only in the
                 // 128-thread class, where the coarse search runs: in the wider classes the
                 // test and the second loop cost more than they saved.)
                 bool spread = false;
                 if constexpr (KMODE == 5)
                     spread = (double)cv.x_max * (double)n_bins < (double)npts * P;
                 if (spread) {
-                    constexpr int GL = 8, NG = T / GL;
+                    constexpr int GL = 2, NG = T / GL;
                     const int blk = ((npts + NG - 1) / NG + GL - 1) / GL * GL;
                     const int g0 = (j0 / GL) * blk;
                     fold_range(g0 + j0 % GL, GL, min(g0 + blk, npts));

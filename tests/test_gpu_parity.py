@@ -182,7 +182,7 @@ def test_many_durations_across_launch_classes_match_oracle(n_dur):
 def test_long_periods_on_dense_sampling_match_oracle():
     # 2,500 points over 20 d searched at 2-60 d with 0.1 d bins (spans of up to ~600
     # bins, the 128-thread class): consecutive points advance the phase by under a bin,
-    # the case whose fold walks the curve in blocks of 8 lanes (as in the learned
+    # the case whose fold walks the curve in blocks by lane pairs (as in the learned
     # proposer's coarse search of time chunks)
     _say("case long periods, dense sampling")
     rng = np.random.default_rng(12)
