@@ -334,11 +334,12 @@ __device__ __forceinline__ float screen_value(int2 h0, int2 h1, int s_hi, int e_
 // SIMD, and the register budget is capped for them; KMODE 4 is MODE 0 capped at 7 waves
 // per SIMD (spans of up to 1,344 bins: 22 KiB, seven workgroups per CU); KMODE 5 is MODE 0
 // for spans of up to 640 bins in 128-thread workgroups (11 KiB: fourteen of them, two
-// waves each, per CU; the per-wave work of a period, its scans, maxima and winner, is
-// then shared by two waves instead of four); MODE 0 is capped at 6 waves per SIMD.
+// waves each, per CU, the 7 waves per SIMD its register cap allows; the per-wave work of
+// a period, its scans, maxima and winner, is then shared by two waves instead of four);
+// MODE 0 is capped at 6 waves per SIMD.
 template <int T, int DMAX, int OBJ, int KMODE>
-__global__ __launch_bounds__(T, KMODE == 3 || KMODE == 5 ? 8
-                                : KMODE == 4             ? 7
+__global__ __launch_bounds__(T, KMODE == 3 ? 8
+                                : KMODE == 4 || KMODE == 5 ? 7
                                 : KMODE == 0             ? 6
                                                          : 1) void
 bls_main_kernel(
