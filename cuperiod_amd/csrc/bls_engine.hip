@@ -1967,7 +1967,7 @@ int run_search(cpb_context* ctx, hipStream_t st, const double* tau, const double
             // waves its register cap admits); more would run after the others in a tail
             int64_t g = cap;
             if (persistent > 0) {
-                const int waves_cap = mode == 3 || mode == 5 ? 8 : (mode == 4 ? 7 : 6);
+                const int waves_cap = mode == 3 ? 8 : (mode == 4 || mode == 5 ? 7 : 6);
                 const int by_waves = std::max(1, 4 * waves_cap / (tl / kWave));
                 const int by_lds = std::max<int>(1, ctx->lds_per_cu / (int)(lds_need + 1280));
                 g = (int64_t)ctx->n_cu * std::min(ctx->grid_per_cu, std::min(by_waves, by_lds));
